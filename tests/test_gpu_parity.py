@@ -1,0 +1,111 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the
+same seeded inputs.  Octree contents, point order, neighbour counts and class
+bytes must be bit-identical."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import wolkenbase_amd as W
+from wolkenbase_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def decode(c):
+    x = c["offset"][0] + c["scale"][0] * c["xi"].astype(np.float64)
+    y = c["offset"][1] + c["scale"][1] * c["yi"].astype(np.float64)
+    z = c["offset"][2] + c["scale"][2] * c["zi"].astype(np.float64)
+    return x, y, z
+
+
+def build_both(ctx, cloud):
+    center, side = W.Octree.size_fit(cloud["bbmin"], cloud["bbmax"])
+    oc, os_ = O.sizefit(cloud["bbmin"], cloud["bbmax"])
+    assert np.array_equal(center, oc) and side == os_
+    tree = W.Octree.build(ctx, cloud["xi"], cloud["yi"], cloud["zi"], cloud["scale"],
+                          cloud["offset"], center, side)
+    x, y, z = decode(cloud)
+    ref = O.Tree(x, y, z, center, side)
+    return tree, ref, (x, y, z)
+
+
+CLOUDS = [
+    dict(seed=1, extent=30.0, density=25.0),
+    dict(seed=2, extent=60.0, density=10.0, embankments=2, outliers=0.02),
+    dict(seed=3, extent=20.0, density=60.0, trees=900),
+]
+
+
+@pytest.mark.parametrize("cfg", CLOUDS)
+def test_tree_matches_oracle(ctx, cfg):
+    cloud = synth.generate(**cfg)
+    tree, ref, _ = build_both(ctx, cloud)
+    g = tree.leaves()
+    r = ref.leaves()
+    print("n", tree.n, "leaves", len(g["side"]), "depth", tree.info())
+    assert len(g["side"]) == ref.nleaves
+    assert np.array_equal(g["center"], r["center"])
+    assert np.array_equal(g["side"], r["side"])
+    assert np.array_equal(g["count"], r["count"])
+    assert np.array_equal(g["first"], r["first"])
+    assert np.array_equal(g["depth"], r["depth"])
+    assert np.array_equal(tree.order().astype(np.int64), ref.order())
+
+
+def test_tree_small_sizes(ctx):
+    for n in (1, 2, 536, 537, 538, 4296, 4297):
+        cloud = synth.generate(seed=100 + n, extent=max(2.0, np.sqrt(n / 25.0)), density=25.0,
+                               buildings=0, trees=0)
+        k = min(n, len(cloud["xi"]))
+        for key in ("xi", "yi", "zi"):
+            cloud[key] = cloud[key][:k]
+        x, y, z = decode(cloud)
+        cloud["bbmin"] = np.array([x.min(), y.min(), z.min()])
+        cloud["bbmax"] = np.array([x.max(), y.max(), z.max()])
+        tree, ref, _ = build_both(ctx, cloud)
+        g, r = tree.leaves(), ref.leaves()
+        assert np.array_equal(g["count"], r["count"]), n
+        assert np.array_equal(g["center"], r["center"]), n
+        assert np.array_equal(tree.order().astype(np.int64), ref.order()), n
+
+
+def random_shapes(rng, x, y, z, kind, m):
+    i = rng.integers(0, len(x), m)
+    if kind == W.WB_SHAPE_CYLINDER:
+        return np.stack([x[i] + rng.normal(0, 0.3, m), y[i] + rng.normal(0, 0.3, m),
+                         rng.uniform(0.05, 3.0, m), np.zeros(m), np.zeros(m)], 1)
+    if kind == W.WB_SHAPE_SPHERE:
+        return np.stack([x[i], y[i], z[i] + rng.normal(0, 0.5, m), rng.uniform(0.05, 4.0, m),
+                         np.zeros(m)], 1)
+    return np.stack([x[i], y[i], z[i], rng.uniform(0.05, 1.0, m), rng.choice([1.0, 0.7, -1.0], m)], 1)
+
+
+@pytest.mark.parametrize("kind", [W.WB_SHAPE_CYLINDER, W.WB_SHAPE_SPHERE, W.WB_SHAPE_HYPERBOLOID])
+def test_count_and_points_in(ctx, kind):
+    cloud = synth.generate(seed=5, extent=30.0, density=25.0)
+    tree, ref, (x, y, z) = build_both(ctx, cloud)
+    rng = np.random.default_rng(kind)
+    P = random_shapes(rng, x, y, z, kind, 300)
+    counts = tree.count_points_in((kind, P))
+    lists = tree.points_in((kind, P[:60]))
+    for s in range(len(P)):
+        want = ref.count_points_in(kind, P[s])
+        assert counts[s] == want, (s, P[s])
+    for s in range(60):
+        assert np.array_equal(lists[s].astype(np.int64), ref.points_in(kind, P[s])), s
+
+
+@pytest.mark.parametrize("cfg", [dict(seed=21, extent=24.0, density=25.0),
+                                 dict(seed=22, extent=30.0, density=20.0, embankments=2,
+                                      outliers=0.02)])
+def test_classify_matches_oracle(ctx, cfg):
+    cloud = synth.generate(**cfg)
+    tree, _, _ = build_both(ctx, cloud)
+    cls, tiles, st = W.classify(tree, cloud["bbmin"], cloud["bbmax"])
+    rcls, rtiles, rst = O.run(cloud)
+    print("stats", st, "oracle", rst.n_class1, rst.n_class2, rst.n_unclassified)
+    for f in ("npoints", "tree_flags"):
+        assert np.array_equal(tiles[f], rtiles[f]), f
+    for f in ("density", "hs_scan", "hs", "height"):
+        assert np.array_equal(tiles[f], rtiles[f]), f
+    assert int((cls != rcls).sum()) == 0

@@ -1,0 +1,308 @@
+"""MI355X octree build and ground classification (wolkenbase's hot path).
+
+Python mirror of the reference's Octree / Shape / classify interface over
+the C ABI of include/wb_gpu.h (libwb_gpu.so, built in-tree for gfx950).
+There is no CPU fallback: every call runs the HIP kernels, and importing a
+GPU entry point without the built library raises.
+
+    tree = Octree.build(ctx, xi, yi, zi, scale, offset, *Octree.size_fit(bbmin, bbmax))
+    counts = tree.count_points_in(Cylinder(cx, cy, r), ...)
+    cls, tiles, stats = classify(tree, bbmin, bbmax)
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GPU_SO = os.path.join(HERE, "libwb_gpu.so")
+
+WB_SHAPE_CYLINDER, WB_SHAPE_SPHERE, WB_SHAPE_HYPERBOLOID = 0, 1, 2
+WB_SHAPE_NPARAMS = 5
+WB_DEVICE_INPUT = 1
+
+
+class WbError(RuntimeError):
+    pass
+
+
+class LasXform(C.Structure):
+    _fields_ = [("scale", C.c_double * 3), ("offset", C.c_double * 3), ("unit", C.c_double)]
+
+
+class Leaf(C.Structure):
+    _fields_ = [("center", C.c_double * 3), ("side", C.c_double), ("first", C.c_uint64),
+                ("count", C.c_uint32), ("depth", C.c_int32)]
+
+
+class ClassifyParams(C.Structure):
+    _fields_ = [("tile_size", C.c_double), ("max_slope", C.c_double),
+                ("thickness", C.c_double), ("min_hs", C.c_double)]
+
+
+class TileRecord(C.Structure):
+    _fields_ = [("npoints", C.c_int32), ("nground", C.c_int32), ("tree_flags", C.c_int32),
+                ("flowsnake_x", C.c_int32), ("flowsnake_y", C.c_int32),
+                ("density", C.c_double), ("hs_scan", C.c_double), ("hs", C.c_double),
+                ("height", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_class1", C.c_uint64), ("n_class2", C.c_uint64),
+                ("n_unclassified", C.c_uint64), ("ntiles", C.c_uint64),
+                ("nonempty_tiles", C.c_uint64), ("sector_borderline", C.c_uint64),
+                ("exact_gap_points", C.c_uint64), ("atan_exact_calls", C.c_uint64),
+                ("sort_passes", C.c_uint32), ("maxdepth", C.c_uint32),
+                ("nleaves", C.c_uint64), ("nnodes", C.c_uint64),
+                ("ms_build", C.c_double), ("ms_tiles", C.c_double), ("ms_scan", C.c_double),
+                ("ms_classify", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def default_params(tile_size=1.0, max_slope=1.0, thickness=0.0, min_hs=0.1):
+    """Reference defaults (mainwindow.cpp:403-410)."""
+    return ClassifyParams(tile_size, max_slope, thickness, min_hs)
+
+
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_u64p = C.POINTER(C.c_uint64)
+_u32p = C.POINTER(C.c_uint32)
+
+# exported symbols of include/wb_gpu.h with their ctypes signatures
+SIGNATURES = {
+    "wb_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "wb_ctx_destroy": (None, [_vp]),
+    "wb_last_error": (C.c_char_p, [_vp]),
+    "wb_ctx_sync": (C.c_int, [_vp]),
+    "wb_size_fit": (C.c_int, [_dp, C.c_int, _dp, _dp]),
+    "wb_tree_build": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint64, C.POINTER(LasXform), _dp,
+                                C.c_double, C.c_int, C.POINTER(_vp)]),
+    "wb_tree_free": (C.c_int, [_vp]),
+    "wb_tree_info": (C.c_int, [_vp, _u64p, _u64p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "wb_tree_leaves": (C.c_int, [_vp, C.POINTER(Leaf), C.c_uint64, _u64p]),
+    "wb_tree_order": (C.c_int, [_vp, _u32p]),
+    "wb_count_points_in": (C.c_int, [_vp, C.c_int, _dp, C.c_uint64, _u64p]),
+    "wb_points_in": (C.c_int, [_vp, C.c_int, _dp, C.c_uint64, _u64p, _u32p, C.c_uint64]),
+    "wb_classify": (C.c_int, [_vp, _dp, _dp, C.POINTER(ClassifyParams), _vp, C.c_int,
+                              C.POINTER(TileRecord), C.c_uint64, C.POINTER(Stats)]),
+    "wb_tile_count": (C.c_int, [_dp, _dp, C.c_double, _u64p]),
+}
+
+_lib = None
+
+
+def lib():
+    """The HIP library; raises if it was not built (no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(GPU_SO):
+            raise WbError("libwb_gpu.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(GPU_SO)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _arr(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """wb_ctx: one HIP device + stream + device memory pool."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        rc = lib().wb_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise WbError(f"wb_ctx_create({device}) failed with {rc}")
+        self.h = h
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise WbError(f"{what}: {lib().wb_last_error(self.h).decode()} ({rc})")
+
+    def sync(self):
+        self.check(lib().wb_ctx_sync(self.h), "wb_ctx_sync")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().wb_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- shapes (shape.h) -------------------------------------------------------
+def Cylinder(cx, cy, r):
+    """Cylinder(xy c, double r) -- shape.h:136-157."""
+    return (WB_SHAPE_CYLINDER, (cx, cy, r, 0.0, 0.0))
+
+
+def Sphere(cx, cy, cz, r):
+    """Sphere(xyz c, double r) -- shape.h:120-134."""
+    return (WB_SHAPE_SPHERE, (cx, cy, cz, r, 0.0))
+
+
+def Hyperboloid(vx, vy, vz, r, slope):
+    """Hyperboloid(xyz vertex, double radius of curvature, double slope) -- shape.h:96-118."""
+    return (WB_SHAPE_HYPERBOLOID, (vx, vy, vz, r, slope))
+
+
+def _shape_batch(shapes):
+    """shapes: list of shape tuples of one kind, or (kind, params array Nx5)."""
+    if isinstance(shapes, tuple) and len(shapes) == 2 and not isinstance(shapes[0], tuple):
+        kind, params = shapes
+        params = np.atleast_2d(_arr(params, np.float64))
+        if params.shape[1] < WB_SHAPE_NPARAMS:
+            params = np.pad(params, ((0, 0), (0, WB_SHAPE_NPARAMS - params.shape[1])))
+        return kind, _arr(params, np.float64)
+    kinds = {s[0] for s in shapes}
+    if len(kinds) != 1:
+        raise WbError("one shape kind per batch")
+    return kinds.pop(), _arr([s[1] for s in shapes], np.float64)
+
+
+class Octree:
+    """Canonical octree resident in HBM (Octree + OctStore of octree.h)."""
+
+    @staticmethod
+    def size_fit(bbmin, bbmax):
+        """Octree::sizeFit over the two header corners (octree.cpp:268-310)."""
+        pts = _arr(np.concatenate([bbmin, bbmax]), np.float64)
+        c = np.zeros(3)
+        s = np.zeros(1)
+        rc = lib().wb_size_fit(_dptr(pts), 2, _dptr(c), _dptr(s))
+        if rc != 0:
+            raise WbError("wb_size_fit failed")
+        return c, float(s[0])
+
+    @classmethod
+    def build(cls, ctx, xi, yi, zi, scale, offset, center, side, unit=1.0, sort_passes=None):
+        """Build from int32 LAS coordinates (numpy arrays, or device tensors
+        exposing data_ptr() for input already resident in HBM)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        xf = LasXform((C.c_double * 3)(*scale), (C.c_double * 3)(*offset), unit)
+        cen = _arr(center, np.float64)
+        dev = hasattr(xi, "data_ptr")
+        if dev:
+            n = int(xi.numel())
+            px, py, pz = xi.data_ptr(), yi.data_ptr(), zi.data_ptr()
+            self._keep = None
+        else:
+            xi, yi, zi = (_arr(a, np.int32) for a in (xi, yi, zi))
+            n = len(xi)
+            px, py, pz = xi.ctypes.data, yi.ctypes.data, zi.ctypes.data
+            self._keep = None
+        h = _vp()
+        rc = lib().wb_tree_build(ctx.h, px, py, pz, n, C.byref(xf), _dptr(cen), float(side),
+                                 WB_DEVICE_INPUT if dev else 0, C.byref(h))
+        ctx.check(rc, "wb_tree_build")
+        self.h = h
+        self.n = n
+        self.center = cen
+        self.side = float(side)
+        return self
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().wb_tree_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def info(self):
+        nl, nn = C.c_uint64(), C.c_uint64()
+        md, sp = C.c_int(), C.c_int()
+        self.ctx.check(lib().wb_tree_info(self.h, C.byref(nl), C.byref(nn), C.byref(md),
+                                          C.byref(sp)), "wb_tree_info")
+        return dict(nleaves=nl.value, nnodes=nn.value, maxdepth=md.value, sort_passes=sp.value)
+
+    def leaves(self):
+        """Leaves in DFS order: dict of arrays center (k,3), side, first, count, depth."""
+        nl = self.info()["nleaves"]
+        buf = (Leaf * max(nl, 1))()
+        got = C.c_uint64()
+        self.ctx.check(lib().wb_tree_leaves(self.h, buf, nl, C.byref(got)), "wb_tree_leaves")
+        a = np.ctypeslib.as_array(buf)[:nl]
+        return dict(center=np.array([list(r["center"]) for r in a]) if nl else np.zeros((0, 3)),
+                    side=a["side"].copy(), first=a["first"].astype(np.int64),
+                    count=a["count"].astype(np.int64), depth=a["depth"].astype(np.int32))
+
+    def order(self):
+        """Input indices in canonical order (leaf DFS order, input order in a leaf)."""
+        out = np.zeros(self.n, dtype=np.uint32)
+        self.ctx.check(lib().wb_tree_order(self.h, out.ctypes.data_as(_u32p)), "wb_tree_order")
+        return out
+
+    def count_points_in(self, shapes):
+        """OctStore::countPointsIn for a batch of shapes (octree.cpp:1244-1264)."""
+        kind, p = _shape_batch(shapes)
+        out = np.zeros(len(p), dtype=np.uint64)
+        self.ctx.check(lib().wb_count_points_in(self.h, kind, _dptr(p), len(p),
+                                                out.ctypes.data_as(_u64p)), "wb_count_points_in")
+        return out
+
+    def points_in(self, shapes):
+        """OctStore::pointsIn (unsorted) for a batch: list of input-index arrays
+        in the reference's order (octree.cpp:1214-1242)."""
+        kind, p = _shape_batch(shapes)
+        ns = len(p)
+        offs = np.zeros(ns + 1, dtype=np.uint64)
+        L = lib()
+        self.ctx.check(L.wb_points_in(self.h, kind, _dptr(p), ns, offs.ctypes.data_as(_u64p),
+                                      None, 0), "wb_points_in")
+        total = int(offs[-1])
+        idx = np.zeros(max(total, 1), dtype=np.uint32)
+        self.ctx.check(L.wb_points_in(self.h, kind, _dptr(p), ns, offs.ctypes.data_as(_u64p),
+                                      idx.ctypes.data_as(_u32p), total), "wb_points_in")
+        return [idx[int(offs[s]):int(offs[s + 1])].copy() for s in range(ns)]
+
+
+def tile_count(bbmin, bbmax, tile_size=1.0):
+    a, b = _arr(bbmin, np.float64), _arr(bbmax, np.float64)
+    n = C.c_uint64()
+    if lib().wb_tile_count(_dptr(a), _dptr(b), float(tile_size), C.byref(n)) != 0:
+        raise WbError("wb_tile_count failed")
+    return n.value
+
+
+def classify(tree, bbmin, bbmax, params=None, want_tiles=True, out=None):
+    """SCAN + POSTSCAN + CLASSIFY of every flowsnake tile (scan.cpp:31-179,
+    classify.cpp:96-173).  Returns (class bytes in input order, tile records
+    or None, stats dict).  If `out` is a device tensor (uint8, n), the classes
+    are written there and the first return value is `out`."""
+    params = params or default_params()
+    a, b = _arr(bbmin, np.float64), _arr(bbmax, np.float64)
+    ntiles = tile_count(a, b, params.tile_size) if want_tiles else 0
+    tiles = (TileRecord * max(ntiles, 1))() if want_tiles else None
+    st = Stats()
+    if out is not None and hasattr(out, "data_ptr"):
+        cls = out
+        ptr, flags = out.data_ptr(), WB_DEVICE_INPUT
+    else:
+        cls = np.zeros(tree.n, dtype=np.uint8)
+        ptr, flags = cls.ctypes.data, 0
+    rc = lib().wb_classify(tree.h, _dptr(a), _dptr(b), C.byref(params), ptr, flags,
+                           tiles if want_tiles else None, ntiles, C.byref(st))
+    tree.ctx.check(rc, "wb_classify")
+    tarr = np.ctypeslib.as_array(tiles)[:ntiles].copy() if want_tiles else None
+    return cls, tarr, st.as_dict()

@@ -1,0 +1,387 @@
+// wb_api.hip -- the C ABI of include/wb_gpu.h.
+#include "../../include/wb_gpu.h"
+#include "wb_internal.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+
+void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
+                     const wb_classify_params *p, uint8_t *d_class_out, bool want_tiles,
+                     ClassifyOut *out);
+
+// ---------------------------------------------------------------------------
+// device memory cache: buffers are returned to a size-keyed pool and reused,
+// so repeated builds of equal-sized tiles do not call hipMalloc again
+// ---------------------------------------------------------------------------
+static size_t round_size(size_t b)
+{
+  size_t r = 256;
+  while (r < b) r = (r < (1u << 20)) ? r * 2 : r + (r >> 2);
+  return r;
+}
+
+void *wb_alloc(wb_ctx *c, size_t bytes)
+{
+  size_t b = round_size(bytes ? bytes : 1);
+  auto it = c->cache.find(b);
+  if (it != c->cache.end()) {
+    void *p = it->second;
+    c->cache.erase(it);
+    c->cached_bytes -= b;
+    return p;
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess) {
+    // release the pool and retry once
+    WB_CHECK(hipStreamSynchronize(c->stream));
+    for (auto &kv : c->cache) hipFree(kv.second);
+    c->cache.clear();
+    c->cached_bytes = 0;
+    (void)hipGetLastError();
+    e = hipMalloc(&p, b);
+    if (e != hipSuccess) throw std::string("hipMalloc failed for ") + std::to_string(b) + " bytes";
+  }
+  return p;
+}
+
+void wb_free(wb_ctx *c, void *p, size_t bytes)
+{
+  if (!p) return;
+  size_t b = round_size(bytes ? bytes : 1);
+  c->cache.insert({b, p});
+  c->cached_bytes += b;
+}
+
+static int fail(wb_ctx *c, const std::string &msg, int code)
+{
+  if (c) c->err = msg;
+  return code;
+}
+
+#define API_TRY(ctx) try {
+#define API_CATCH(ctx)                                             \
+  }                                                                \
+  catch (const std::string &m) { return fail(ctx, m, WB_EDEVICE); } \
+  catch (...) { return fail(ctx, "unexpected exception", WB_EDEVICE); }
+
+extern "C" {
+
+int wb_ctx_create(int device, wb_ctx **out)
+{
+  if (!out) return WB_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return WB_EDEVICE;
+  wb_ctx *c = new wb_ctx();
+  c->device = device;
+  c->cached_bytes = 0;
+  try {
+    WB_CHECK(hipSetDevice(device));
+    WB_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    double tabs[511 + 512 + 512];
+    wb::fill_atan_tables(tabs, tabs + 511, tabs + 1023);
+    WB_CHECK(hipMalloc(&c->d_tan, sizeof tabs));
+    WB_CHECK(hipMemcpy(c->d_tan, tabs, sizeof tabs, hipMemcpyHostToDevice));
+    c->d_cos = c->d_tan + 511;
+    c->d_sin = c->d_tan + 1023;
+    c->kconst = wb::atan_const();
+  } catch (const std::string &m) {
+    delete c;
+    return WB_EDEVICE;
+  }
+  *out = c;
+  return WB_OK;
+}
+
+void wb_ctx_destroy(wb_ctx *c)
+{
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  for (auto &kv : c->cache) hipFree(kv.second);
+  hipFree(c->d_tan);
+  hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *wb_last_error(wb_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+int wb_ctx_sync(wb_ctx *c)
+{
+  if (!c) return WB_EINVAL;
+  return hipStreamSynchronize(c->stream) == hipSuccess ? WB_OK : WB_EDEVICE;
+}
+
+int wb_size_fit(const double *corners, int ncorners, double center[3], double *side)
+{
+  if (!corners || ncorners <= 0 || !center || !side) return WB_EINVAL;
+  wb::size_fit(corners, ncorners, center, side);
+  return WB_OK;
+}
+
+int wb_tree_build(wb_ctx *c, const int32_t *x, const int32_t *y, const int32_t *z, uint64_t n,
+                  const wb_lasxform *xf, const double root_center[3], double root_side, int flags,
+                  wb_tree **out)
+{
+  if (!c || !out || !xf || !root_center) return fail(c, "null argument", WB_EINVAL);
+  if (n > 0xffffffffull) return fail(c, "more than 2^32-1 points", WB_ERANGE);
+  if (n > 0 && (!x || !y || !z)) return fail(c, "null coordinate array", WB_EINVAL);
+  if (!(root_side > 0) || !isfinite(root_side)) return fail(c, "root side must be positive", WB_EINVAL);
+  if ((root_side / (1 << 21)) * (double)(1 << 21) != root_side)
+    return fail(c, "root side is not representable", WB_ERANGE);
+  *out = nullptr;
+  wb_tree *t = new wb_tree();
+  t->ctx = c;
+  t->n = n;
+  for (int a = 0; a < 3; a++) {
+    t->center[a] = root_center[a];
+    t->scale[a] = xf->scale[a] * xf->unit;
+    t->offset[a] = xf->offset[a] * xf->unit;
+  }
+  t->side = root_side;
+  t->sort_passes = 0;
+  if (xf->unit != 1.0) {
+    delete t;
+    return fail(c, "only unit == 1 is supported (metres)", WB_ERANGE);
+  }
+  for (int a = 0; a < 3; a++) {
+    t->scale[a] = xf->scale[a];
+    t->offset[a] = xf->offset[a];
+  }
+  API_TRY(c)
+  WB_CHECK(hipSetDevice(c->device));
+  const int32_t *dx = x, *dy = y, *dz = z;
+  int32_t *tmp = nullptr;
+  if (!(flags & WB_DEVICE_INPUT) && n > 0) {
+    tmp = (int32_t *)wb_alloc(c, 12 * n);
+    WB_CHECK(hipMemcpyAsync(tmp, x, 4 * n, hipMemcpyHostToDevice, c->stream));
+    WB_CHECK(hipMemcpyAsync(tmp + n, y, 4 * n, hipMemcpyHostToDevice, c->stream));
+    WB_CHECK(hipMemcpyAsync(tmp + 2 * n, z, 4 * n, hipMemcpyHostToDevice, c->stream));
+    dx = tmp;
+    dy = tmp + n;
+    dz = tmp + 2 * n;
+  }
+  try {
+    wb_build_tree(t, dx, dy, dz);
+  } catch (...) {
+    if (tmp) wb_free(c, tmp, 12 * n);
+    for (auto &o : t->owned) wb_free(c, o.first, o.second);
+    delete t;
+    throw;
+  }
+  if (tmp) wb_free(c, tmp, 12 * n);
+  WB_CHECK(hipStreamSynchronize(c->stream));
+  *out = t;
+  return WB_OK;
+  API_CATCH(c)
+}
+
+int wb_tree_free(wb_tree *t)
+{
+  if (!t) return WB_EINVAL;
+  wb_ctx *c = t->ctx;
+  hipStreamSynchronize(c->stream);
+  for (auto &o : t->owned) wb_free(c, o.first, o.second);
+  delete t;
+  return WB_OK;
+}
+
+int wb_tree_info(wb_tree *t, uint64_t *nleaves, uint64_t *nnodes, int *maxdepth, int *sort_passes)
+{
+  if (!t) return WB_EINVAL;
+  if (nleaves) *nleaves = t->nleaves;
+  if (nnodes) *nnodes = t->nnodes;
+  if (maxdepth) *maxdepth = t->maxdepth;
+  if (sort_passes) *sort_passes = t->sort_passes;
+  return WB_OK;
+}
+
+int wb_tree_leaves(wb_tree *t, wb_leaf *out, uint64_t cap, uint64_t *nleaves)
+{
+  if (!t) return WB_EINVAL;
+  wb_ctx *c = t->ctx;
+  if (nleaves) *nleaves = t->nleaves;
+  if (!out) return WB_OK;
+  if (cap < t->nleaves) return fail(c, "leaf buffer too small", WB_EINVAL);
+  API_TRY(c)
+  std::vector<uint32_t> st(t->nleaves + 1);
+  std::vector<double> cube(4ull * t->nleaves + 4);
+  WB_CHECK(hipMemcpyAsync(st.data(), t->leaf_start, 4ull * (t->nleaves + 1), hipMemcpyDeviceToHost, c->stream));
+  if (t->nleaves)
+    WB_CHECK(hipMemcpyAsync(cube.data(), t->leaf_cube, 32ull * t->nleaves, hipMemcpyDeviceToHost, c->stream));
+  WB_CHECK(hipStreamSynchronize(c->stream));
+  for (uint32_t i = 0; i < t->nleaves; i++) {
+    wb_leaf &L = out[i];
+    L.center[0] = cube[4 * i];
+    L.center[1] = cube[4 * i + 1];
+    L.center[2] = cube[4 * i + 2];
+    L.side = cube[4 * i + 3];
+    L.first = st[i];
+    L.count = st[i + 1] - st[i];
+    L.depth = (int32_t)lrint(log2(t->side / L.side));
+  }
+  return WB_OK;
+  API_CATCH(c)
+}
+
+int wb_tree_order(wb_tree *t, uint32_t *order)
+{
+  if (!t || !order) return WB_EINVAL;
+  wb_ctx *c = t->ctx;
+  API_TRY(c)
+  std::vector<uint32_t> canon(t->n), sidx(t->n);
+  if (t->n) {
+    WB_CHECK(hipMemcpyAsync(canon.data(), t->canon, 4 * t->n, hipMemcpyDeviceToHost, c->stream));
+    WB_CHECK(hipMemcpyAsync(sidx.data(), t->sidx, 4 * t->n, hipMemcpyDeviceToHost, c->stream));
+  }
+  WB_CHECK(hipStreamSynchronize(c->stream));
+  for (uint64_t k = 0; k < t->n; k++) order[k] = sidx[canon[k]];
+  return WB_OK;
+  API_CATCH(c)
+}
+
+static int check_kind(int kind) { return kind == WB_SHAPE_CYLINDER || kind == WB_SHAPE_SPHERE || kind == WB_SHAPE_HYPERBOLOID; }
+
+int wb_count_points_in(wb_tree *t, int kind, const double *params, uint64_t ns, uint64_t *counts)
+{
+  if (!t || (!params && ns) || (!counts && ns)) return WB_EINVAL;
+  wb_ctx *c = t->ctx;
+  if (!check_kind(kind)) return fail(c, "unknown shape kind", WB_EINVAL);
+  if (ns > 0x7fffffffull) return fail(c, "too many shapes", WB_ERANGE);
+  if (ns == 0) return WB_OK;
+  API_TRY(c)
+  size_t pb = sizeof(double) * WB_SHAPE_NPARAMS * ns;
+  double *dp = (double *)wb_alloc(c, pb);
+  uint64_t *dc = (uint64_t *)wb_alloc(c, 8 * ns);
+  WB_CHECK(hipMemcpyAsync(dp, params, pb, hipMemcpyHostToDevice, c->stream));
+  wb_run_count(t, kind, dp, ns, dc);
+  WB_CHECK(hipMemcpyAsync(counts, dc, 8 * ns, hipMemcpyDeviceToHost, c->stream));
+  WB_CHECK(hipStreamSynchronize(c->stream));
+  wb_free(c, dp, pb);
+  wb_free(c, dc, 8 * ns);
+  return WB_OK;
+  API_CATCH(c)
+}
+
+int wb_points_in(wb_tree *t, int kind, const double *params, uint64_t ns, uint64_t *offsets,
+                 uint32_t *idx, uint64_t cap)
+{
+  if (!t || (!params && ns) || !offsets) return WB_EINVAL;
+  wb_ctx *c = t->ctx;
+  if (!check_kind(kind)) return fail(c, "unknown shape kind", WB_EINVAL);
+  if (ns > 0x7fffffffull) return fail(c, "too many shapes", WB_ERANGE);
+  offsets[0] = 0;
+  if (ns == 0) return WB_OK;
+  API_TRY(c)
+  size_t pb = sizeof(double) * WB_SHAPE_NPARAMS * ns;
+  double *dp = (double *)wb_alloc(c, pb);
+  uint64_t *dc = (uint64_t *)wb_alloc(c, 8 * (ns + 1));
+  WB_CHECK(hipMemcpyAsync(dp, params, pb, hipMemcpyHostToDevice, c->stream));
+  wb_run_count(t, kind, dp, ns, dc);
+  std::vector<uint64_t> cnt(ns);
+  WB_CHECK(hipMemcpyAsync(cnt.data(), dc, 8 * ns, hipMemcpyDeviceToHost, c->stream));
+  WB_CHECK(hipStreamSynchronize(c->stream));
+  for (uint64_t s = 0; s < ns; s++) offsets[s + 1] = offsets[s] + cnt[s];
+  if (idx) {
+    uint64_t total = offsets[ns];
+    if (cap < total) {
+      wb_free(c, dp, pb);
+      wb_free(c, dc, 8 * (ns + 1));
+      return fail(c, "index buffer too small", WB_EINVAL);
+    }
+    WB_CHECK(hipMemcpyAsync(dc, offsets, 8 * (ns + 1), hipMemcpyHostToDevice, c->stream));
+    uint32_t *dout = (uint32_t *)wb_alloc(c, 4 * total + 4);
+    wb_run_points(t, kind, dp, ns, dc, dout);
+    WB_CHECK(hipMemcpyAsync(idx, dout, 4 * total, hipMemcpyDeviceToHost, c->stream));
+    WB_CHECK(hipStreamSynchronize(c->stream));
+    wb_free(c, dout, 4 * total + 4);
+  }
+  wb_free(c, dp, pb);
+  wb_free(c, dc, 8 * (ns + 1));
+  return WB_OK;
+  API_CATCH(c)
+}
+
+int wb_tile_count(const double bbmin[3], const double bbmax[3], double tile_size, uint64_t *ntiles)
+{
+  if (!bbmin || !bbmax || !ntiles || !(tile_size > 0)) return WB_EINVAL;
+  wb::SnakeSetup s = wb::snake_setup(bbmin, bbmax, tile_size);
+  *ntiles = (uint64_t)((int64_t)s.hi - s.lo + 1);
+  return WB_OK;
+}
+
+int wb_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
+                const wb_classify_params *p, uint8_t *class_out, int flags, wb_tile *tiles_out,
+                uint64_t tiles_cap, wb_stats *stats)
+{
+  if (!t || !bbmin || !bbmax || !p || !class_out) return WB_EINVAL;
+  wb_ctx *c = t->ctx;
+  if (!(p->tile_size > 0) || !(p->max_slope > 0) || !(p->min_hs >= 0) || !isfinite(p->thickness))
+    return fail(c, "bad classification parameters", WB_EINVAL);
+  // the early-exit argument needs every direction >= 30 units below +180 deg
+  for (int a = 0; a < 2; a++)
+    if (!(bbmax[a] - bbmin[a] < 1e7 * t->scale[1 - a]))
+      return fail(c, "cloud extent above 1e7 coordinate units is not supported", WB_ERANGE);
+  uint64_t ntiles;
+  wb_tile_count(bbmin, bbmax, p->tile_size, &ntiles);
+  if (ntiles > 0xffffffffull) return fail(c, "too many tiles", WB_ERANGE);
+  if (tiles_out && tiles_cap < ntiles) return fail(c, "tile buffer too small", WB_EINVAL);
+  API_TRY(c)
+  WB_CHECK(hipSetDevice(c->device));
+  auto t0 = std::chrono::steady_clock::now();
+  uint8_t *dcls = class_out;
+  bool host_out = !(flags & WB_DEVICE_INPUT);
+  if (host_out) dcls = (uint8_t *)wb_alloc(c, t->n + 16);
+  ClassifyOut co;
+  wb_run_classify(t, bbmin, bbmax, p, dcls, tiles_out != nullptr || stats != nullptr, &co);
+  if (host_out) {
+    if (t->n) WB_CHECK(hipMemcpyAsync(class_out, dcls, t->n, hipMemcpyDeviceToHost, c->stream));
+    WB_CHECK(hipStreamSynchronize(c->stream));
+    wb_free(c, dcls, t->n + 16);
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  if (tiles_out) {
+    for (uint64_t k = 0; k < ntiles; k++) {
+      const TileRec &r = co.tiles[k];
+      wb_tile &o = tiles_out[k];
+      o.npoints = r.npoints;
+      o.nground = r.nground;
+      o.tree_flags = r.tree_flags;
+      o.flowsnake_x = co.tex[k];
+      o.flowsnake_y = co.tey[k];
+      o.density = r.density;
+      o.hs_scan = r.hs_scan;
+      o.hs = r.hs;
+      o.height = r.height;
+    }
+  }
+  if (stats) {
+    memset(stats, 0, sizeof *stats);
+    stats->n_unclassified = co.hist[0];
+    stats->n_class1 = co.hist[1];
+    stats->n_class2 = co.hist[2];
+    stats->ntiles = ntiles;
+    uint64_t ne = 0;
+    for (auto &r : co.tiles) ne += r.npoints != 0;
+    stats->nonempty_tiles = ne;
+    stats->sector_borderline = co.borderline;
+    stats->exact_gap_points = co.nslow;
+    stats->atan_exact_calls = co.atan_exact;
+    stats->sort_passes = t->sort_passes;
+    stats->maxdepth = t->maxdepth;
+    stats->nleaves = t->nleaves;
+    stats->nnodes = t->nnodes;
+    stats->ms_tiles = co.ms_tiles;
+    stats->ms_scan = co.ms_scan;
+    stats->ms_classify = co.ms_classify;
+    stats->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  }
+  return WB_OK;
+  API_CATCH(c)
+}
+
+}  // extern "C"

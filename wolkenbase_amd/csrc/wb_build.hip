@@ -1,0 +1,656 @@
+// wb_build.hip -- octree build on the GPU: Morton keys from LAS int32
+// coordinates, LSD radix sort, canonical leaf split, SoA permute.
+//
+// Reference behaviour (octree.cpp:199-347, 620-662, 849-876, 1295-1338): a
+// block holds at most RECORDS=537 distinct locations and splits one level
+// when another arrives, so the final tree is the unique one in which a node
+// is internal iff its region holds more than 537 points.  A point's region
+// at each level is decided by `coordinate >= centre` compares (child index
+// z*4+y*2+x), and the DFS visits children 0..7, so DFS order is ascending
+// order of the Morton key built from those compare bits.
+#include "wb_internal.h"
+
+#include <string.h>
+
+#include <algorithm>
+
+#define WAVE 64
+
+// ---------------------------------------------------------------------------
+// wave / block primitives
+// ---------------------------------------------------------------------------
+__device__ inline unsigned lane_id() { return __lane_id(); }
+__device__ inline uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+// inclusive scan across a 64-lane wave
+__device__ inline uint32_t wave_incl_scan(uint32_t v)
+{
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    uint32_t t = __shfl_up(v, d, WAVE);
+    if ((int)lane_id() >= d) v += t;
+  }
+  return v;
+}
+
+// exclusive scan of one value per thread over a 256-thread block; returns
+// the exclusive prefix and writes the block total to *total
+__device__ inline uint32_t block_excl_scan256(uint32_t v, uint32_t *total)
+{
+  __shared__ uint32_t wsum[4];
+  uint32_t inc = wave_incl_scan(v);
+  int w = threadIdx.x / WAVE;
+  if (lane_id() == WAVE - 1) wsum[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int i = 0; i < 4; i++) {
+    if (i < w) off += wsum[i];
+    tot += wsum[i];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+// ---------------------------------------------------------------------------
+// device-wide exclusive scan (u32), 1024 elements per block
+// ---------------------------------------------------------------------------
+__global__ void k_scan_reduce(const uint32_t *in, uint64_t n, uint32_t *bsum)
+{
+  uint64_t base = (uint64_t)blockIdx.x * 1024;
+  uint32_t s = 0;
+  for (int k = 0; k < 4; k++) {
+    uint64_t i = base + threadIdx.x * 4 + k;
+    if (i < n) s += in[i];
+  }
+  uint32_t tot;
+  block_excl_scan256(s, &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_down(uint32_t *data, uint64_t n, const uint32_t *boff)
+{
+  uint64_t base = (uint64_t)blockIdx.x * 1024;
+  uint32_t v[4], s = 0;
+  for (int k = 0; k < 4; k++) {
+    uint64_t i = base + threadIdx.x * 4 + k;
+    v[k] = (i < n) ? data[i] : 0;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t ex = block_excl_scan256(s, &tot) + boff[blockIdx.x];
+  for (int k = 0; k < 4; k++) {
+    uint64_t i = base + threadIdx.x * 4 + k;
+    if (i < n) data[i] = ex;
+    ex += v[k];
+  }
+}
+
+static void scan_rec(wb_ctx *c, uint32_t *data, uint64_t n)
+{
+  unsigned nb = wb_blocks(n, 1024);
+  uint32_t *bs = (uint32_t *)wb_alloc(c, sizeof(uint32_t) * (nb + 1));
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c->stream, data, n, bs);
+  if (nb > 1) scan_rec(c, bs, nb);
+  else WB_CHECK(hipMemsetAsync(bs, 0, sizeof(uint32_t), c->stream));
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, c->stream, data, n, bs);
+  wb_free(c, bs, sizeof(uint32_t) * (nb + 1));
+}
+
+uint64_t wb_exclusive_scan_u32(wb_ctx *c, uint32_t *data, uint64_t n)
+{
+  if (n == 0) return 0;
+  uint32_t last_in = 0, last_out = 0;
+  WB_CHECK(hipMemcpyAsync(&last_in, data + n - 1, 4, hipMemcpyDeviceToHost, c->stream));
+  scan_rec(c, data, n);
+  WB_CHECK(hipMemcpyAsync(&last_out, data + n - 1, 4, hipMemcpyDeviceToHost, c->stream));
+  WB_CHECK(hipStreamSynchronize(c->stream));
+  return (uint64_t)last_in + last_out;
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort, 8-bit digits.  Per pass: per-block digit histograms in LDS,
+// a device-wide exclusive scan of the digit-major histogram table, then a
+// stable scatter in which each wave ranks its keys with 8 ballots (peer
+// mask of equal digits) and one LDS atomic per distinct digit per round.
+// ---------------------------------------------------------------------------
+#define RS_THREADS 256
+#define RS_ITEMS 16
+#define RS_TILE (RS_THREADS * RS_ITEMS)
+
+template <typename K>
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K *keys, uint64_t n, int shift,
+                                                          uint32_t *hist, unsigned nblocks)
+{
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+  for (int k = 0; k < RS_ITEMS; k++) {
+    uint64_t i = base + (uint64_t)k * RS_THREADS + threadIdx.x;
+    if (i < n) atomicAdd(&h[(unsigned)(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+template <typename K>
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K *kin, const uint32_t *vin,
+                                                             K *kout, uint32_t *vout, uint64_t n,
+                                                             int shift, const uint32_t *offs,
+                                                             unsigned nblocks)
+{
+  __shared__ uint32_t wcnt[4][256];
+  const int w = threadIdx.x / WAVE;
+  const unsigned lane = lane_id();
+  for (int d = threadIdx.x; d < 4 * 256; d += RS_THREADS) (&wcnt[0][0])[d] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * RS_TILE + (uint64_t)w * (RS_TILE / 4);
+  K key[RS_ITEMS];
+  uint32_t val[RS_ITEMS];
+  uint32_t rank[RS_ITEMS];
+  uint32_t dig[RS_ITEMS];
+  for (int r = 0; r < RS_ITEMS; r++) {
+    uint64_t i = base + (uint64_t)r * WAVE + lane;
+    bool ok = i < n;
+    key[r] = ok ? kin[i] : (K)0;
+    val[r] = ok ? vin[i] : 0u;
+    uint32_t d = (unsigned)(key[r] >> shift) & 255u;
+    dig[r] = d;
+    uint64_t peers = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      uint64_t bb = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    uint32_t before = (uint32_t)__popcll(peers & lanemask_lt());
+    uint32_t cnt = (uint32_t)__popcll(peers);
+    int leader = peers ? (__ffsll((long long)peers) - 1) : 0;
+    uint32_t old = 0;
+    if (ok && (int)lane == leader) old = atomicAdd(&wcnt[w][d], cnt);
+    old = __shfl(old, leader, WAVE);
+    rank[r] = old + before;
+  }
+  __syncthreads();
+  // per-digit prefix over the four waves
+  {
+    int d = threadIdx.x;
+    uint32_t a0 = wcnt[0][d], a1 = wcnt[1][d], a2 = wcnt[2][d];
+    __syncthreads();
+    wcnt[0][d] = 0;
+    wcnt[1][d] = a0;
+    wcnt[2][d] = a0 + a1;
+    wcnt[3][d] = a0 + a1 + a2;
+  }
+  __syncthreads();
+  for (int r = 0; r < RS_ITEMS; r++) {
+    uint64_t i = base + (uint64_t)r * WAVE + lane;
+    if (i >= n) continue;
+    uint32_t d = dig[r];
+    uint64_t pos = (uint64_t)offs[(uint64_t)d * nblocks + blockIdx.x] + wcnt[w][d] + rank[r];
+    kout[pos] = key[r];
+    vout[pos] = val[r];
+  }
+}
+
+template <typename K>
+static void radix_sort_impl(wb_ctx *c, K *key, uint32_t *val, uint64_t n, int first_shift,
+                            int passes)
+{
+  if (n <= 1 || passes <= 0) return;
+  unsigned nb = wb_blocks(n, RS_TILE);
+  uint64_t hsz = (uint64_t)nb * 256;
+  uint32_t *hist = (uint32_t *)wb_alloc(c, sizeof(uint32_t) * hsz);
+  K *k2 = (K *)wb_alloc(c, sizeof(K) * n);
+  uint32_t *v2 = (uint32_t *)wb_alloc(c, sizeof(uint32_t) * n);
+  K *ka = key, *kb = k2;
+  uint32_t *va = val, *vb = v2;
+  for (int p = 0; p < passes; p++) {
+    int shift = first_shift + 8 * p;
+    hipLaunchKernelGGL(k_rs_hist<K>, dim3(nb), dim3(RS_THREADS), 0, c->stream, ka, n, shift, hist,
+                       nb);
+    scan_rec(c, hist, hsz);
+    hipLaunchKernelGGL(k_rs_scatter<K>, dim3(nb), dim3(RS_THREADS), 0, c->stream, ka, va, kb, vb,
+                       n, shift, hist, nb);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (ka != key) {
+    WB_CHECK(hipMemcpyAsync(key, ka, sizeof(K) * n, hipMemcpyDeviceToDevice, c->stream));
+    WB_CHECK(hipMemcpyAsync(val, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, c->stream));
+  }
+  wb_free(c, hist, sizeof(uint32_t) * hsz);
+  wb_free(c, k2, sizeof(K) * n);
+  wb_free(c, v2, sizeof(uint32_t) * n);
+}
+
+void wb_radix_sort_pairs(wb_ctx *c, uint64_t *key, uint32_t *val, uint64_t n, int passes)
+{
+  radix_sort_impl<uint64_t>(c, key, val, n, 64 - 8 * passes, passes);
+}
+
+void wb_radix_sort_u32_pairs(wb_ctx *c, uint32_t *key, uint32_t *val, uint64_t n)
+{
+  radix_sort_impl<uint32_t>(c, key, val, n, 0, 4);
+}
+
+// ---------------------------------------------------------------------------
+// K1: Morton keys
+// ---------------------------------------------------------------------------
+struct Xform {
+  double sx, sy, sz, ox, oy, oz;
+  double lox, loy, loz;  // root cube minimum corner
+  double cell;           // side / 2^21
+  double inv_cell;       // 2^21 / side
+};
+
+// 21-bit cell index along one axis, equal to the bits of the reference's
+// `x >= centre` descent: the level-l centre of the node with index j is
+// lo + (2j+1)*side/2^(l+1), a dyadic value, so the descent equals counting
+// the exact cell bounds lo + k*cell <= x.  Points outside the root cube go to
+// the extreme children, as the compares send them.
+__device__ inline uint32_t cell_index(double x, double lo, double cell, double inv_cell)
+{
+  const long long kmax = (1ll << WB_MAX_LEVEL) - 1;
+  double t = (x - lo) * inv_cell;
+  long long k;
+  if (!(t > 0)) k = 0;                      // also NaN
+  else if (t >= (double)kmax) k = kmax;
+  else k = (long long)t;
+  if (k > 0 && x < lo + (double)k * cell) k--;
+  else if (k < kmax && x >= lo + (double)(k + 1) * cell) k++;
+  return (uint32_t)k;
+}
+
+__device__ inline uint64_t spread3(uint32_t v)
+{
+  uint64_t x = v & 0x1fffff;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+__device__ inline double decode(int32_t v, double s, double o) { return o + s * (double)v; }
+
+__global__ void k_keygen(const int32_t *xi, const int32_t *yi, const int32_t *zi, uint64_t n,
+                         Xform f, uint64_t *key, uint32_t *val)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x = decode(xi[i], f.sx, f.ox), y = decode(yi[i], f.sy, f.oy), z = decode(zi[i], f.sz, f.oz);
+  uint32_t ix = cell_index(x, f.lox, f.cell, f.inv_cell);
+  uint32_t iy = cell_index(y, f.loy, f.cell, f.inv_cell);
+  uint32_t iz = cell_index(z, f.loz, f.cell, f.inv_cell);
+  uint64_t k = (spread3(iz) << 2) | (spread3(iy) << 1) | spread3(ix);
+  key[i] = k << 1;
+  val[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// K3: canonical split, one level per launch.  Each active node (range of the
+// sorted keys sharing its prefix) finds its eight child ranges by binary
+// search on the next 3-bit digit; children with more than 537 points become
+// the next level's active nodes, the others non-empty children are leaves.
+// ---------------------------------------------------------------------------
+struct Active {
+  uint32_t start, end;
+  int32_t node;
+  int32_t pad;
+};
+
+struct LeafTmp {
+  uint32_t start, count;
+  int32_t parent, depth;
+  double c[4];  // cx, cy, cz, side
+};
+
+struct SplitOut {
+  uint32_t n_next, n_leaf, n_node, overflow, maxdepth;
+};
+
+__device__ inline uint32_t digit_at(uint64_t k, int level) { return (uint32_t)(k >> (64 - 3 * level)) & 7u; }
+
+__global__ void k_split_level(const uint64_t *key, const Active *act, uint32_t nact, int level,
+                              int max_level, Active *next, uint32_t next_cap, LeafTmp *leaves,
+                              uint32_t leaf_cap, double *node_cube, int32_t *node_child,
+                              int32_t *node_parent, uint32_t node_cap, SplitOut *out)
+{
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nact) return;
+  Active A = act[a];
+  int node = A.node;
+  const double cx = node_cube[4 * node], cy = node_cube[4 * node + 1], cz = node_cube[4 * node + 2];
+  const double side = node_cube[4 * node + 3];
+  uint32_t bound[9];
+  bound[0] = A.start;
+  bound[8] = A.end;
+  int lv = level + 1;
+  for (int c = 1; c < 8; c++) {
+    // first index in [bound[c-1], end) whose digit >= c
+    uint32_t lo = bound[c - 1], hi = A.end;
+    while (lo < hi) {
+      uint32_t mid = lo + (hi - lo) / 2;
+      if (digit_at(key[mid], lv) < (uint32_t)c) lo = mid + 1;
+      else hi = mid;
+    }
+    bound[c] = lo;
+  }
+  for (int c = 0; c < 8; c++) {
+    uint32_t cnt = bound[c + 1] - bound[c];
+    int32_t code = -1;
+    if (cnt > 0) {
+      // Octree::cube(c), octree.cpp:349-358
+      int xb = c & 1, yb = (c & 2) / 2, zb = (c & 4) / 4;
+      double ncx = cx + (2 * xb - 1) * side / 4;
+      double ncy = cy + (2 * yb - 1) * side / 4;
+      double ncz = cz + (2 * zb - 1) * side / 4;
+      double ns = side / 2;
+      if (cnt > WB_RECORDS) {
+        if (lv >= max_level) {
+          atomicOr(&out->overflow, 1u);
+        } else {
+          uint32_t nid = atomicAdd(&out->n_node, 1u);
+          uint32_t slot = atomicAdd(&out->n_next, 1u);
+          if (nid < node_cap && slot < next_cap) {
+            node_cube[4 * nid] = ncx;
+            node_cube[4 * nid + 1] = ncy;
+            node_cube[4 * nid + 2] = ncz;
+            node_cube[4 * nid + 3] = ns;
+            node_parent[nid] = node;
+            for (int k = 0; k < 8; k++) node_child[8 * nid + k] = -1;
+            Active na;
+            na.start = bound[c];
+            na.end = bound[c + 1];
+            na.node = (int32_t)nid;
+            na.pad = 0;
+            next[slot] = na;
+            code = (int32_t)nid;
+          } else {
+            atomicOr(&out->overflow, 2u);
+          }
+        }
+      } else {
+        uint32_t ls = atomicAdd(&out->n_leaf, 1u);
+        if (ls < leaf_cap) {
+          LeafTmp L;
+          L.start = bound[c];
+          L.count = cnt;
+          L.parent = node;
+          L.depth = lv;
+          L.c[0] = ncx;
+          L.c[1] = ncy;
+          L.c[2] = ncz;
+          L.c[3] = ns;
+          leaves[ls] = L;
+          code = -2 - (int32_t)ls;
+          atomicMax(&out->maxdepth, (uint32_t)lv);
+        } else {
+          atomicOr(&out->overflow, 4u);
+        }
+      }
+    }
+    node_child[8 * node + c] = code;
+  }
+}
+
+__global__ void k_leaf_keys(const LeafTmp *lt, uint32_t nl, uint32_t *k, uint32_t *v)
+{
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nl) return;
+  k[i] = lt[i].start;
+  v[i] = i;
+}
+
+__global__ void k_leaf_final(const LeafTmp *lt, const uint32_t *perm, uint32_t nl, uint64_t n,
+                             uint32_t *leaf_start, double *leaf_cube, int32_t *leaf_parent,
+                             uint32_t *inv)
+{
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nl) return;
+  if (i == nl) {
+    leaf_start[nl] = (uint32_t)n;
+    return;
+  }
+  LeafTmp L = lt[perm[i]];
+  leaf_start[i] = L.start;
+  leaf_cube[4 * i] = L.c[0];
+  leaf_cube[4 * i + 1] = L.c[1];
+  leaf_cube[4 * i + 2] = L.c[2];
+  leaf_cube[4 * i + 3] = L.c[3];
+  leaf_parent[i] = L.parent;
+  inv[perm[i]] = i;
+}
+
+__global__ void k_patch_children(int32_t *child, uint32_t nn, const uint32_t *inv)
+{
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn * 8u) return;
+  int32_t c = child[i];
+  if (c <= -2) child[i] = -2 - (int32_t)inv[-2 - c];
+}
+
+__global__ void k_fill_pt_leaf(const uint32_t *leaf_start, uint32_t nl, uint32_t *pt_leaf)
+{
+  uint32_t l = blockIdx.x;
+  if (l >= nl) return;
+  uint32_t s = leaf_start[l], e = leaf_start[l + 1];
+  for (uint32_t i = s + threadIdx.x; i < e; i += blockDim.x) pt_leaf[i] = l;
+}
+
+// ---------------------------------------------------------------------------
+// K4: SoA permute (decode in sorted order) and 32-slot bounding boxes
+// ---------------------------------------------------------------------------
+__global__ void k_permute(const int32_t *xi, const int32_t *yi, const int32_t *zi,
+                          const uint32_t *sidx, uint64_t n, Xform f, double *px, double *py,
+                          double *pz)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t s = sidx[i];
+  px[i] = decode(xi[s], f.sx, f.ox);
+  py[i] = decode(yi[s], f.sy, f.oy);
+  pz[i] = decode(zi[s], f.sz, f.oz);
+}
+
+__global__ void k_chunk_box(const double *px, const double *py, const double *pz, uint64_t n,
+                            double *box)
+{
+  // one half-wave (32 lanes) per chunk
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t chunk = g / WB_CHUNK;
+  uint64_t i = g;
+  bool ok = i < n;
+  double x0 = ok ? px[i] : __builtin_inf(), x1 = ok ? px[i] : -__builtin_inf();
+  double y0 = ok ? py[i] : __builtin_inf(), y1 = ok ? py[i] : -__builtin_inf();
+  double z0 = ok ? pz[i] : __builtin_inf(), z1 = ok ? pz[i] : -__builtin_inf();
+#pragma unroll
+  for (int d = 1; d < WB_CHUNK; d <<= 1) {
+    x0 = fmin(x0, __shfl_xor(x0, d, WAVE));
+    x1 = fmax(x1, __shfl_xor(x1, d, WAVE));
+    y0 = fmin(y0, __shfl_xor(y0, d, WAVE));
+    y1 = fmax(y1, __shfl_xor(y1, d, WAVE));
+    z0 = fmin(z0, __shfl_xor(z0, d, WAVE));
+    z1 = fmax(z1, __shfl_xor(z1, d, WAVE));
+  }
+  uint64_t nchunks = (n + WB_CHUNK - 1) / WB_CHUNK;
+  if ((g % WB_CHUNK) == 0 && chunk < nchunks) {
+    double *b = box + 6 * chunk;
+    b[0] = x0; b[1] = x1; b[2] = y0; b[3] = y1; b[4] = z0; b[5] = z1;
+  }
+}
+
+// per-leaf canonical order: input order inside each leaf (bitonic sort of up
+// to 1024 slots in LDS, one block per leaf)
+__global__ void __launch_bounds__(256) k_leaf_canon(const uint32_t *leaf_start, uint32_t nl,
+                                                     const uint32_t *sidx, uint32_t *canon,
+                                                     uint32_t *crank)
+{
+  __shared__ uint32_t sk[1024], sv[1024];
+  uint32_t l = blockIdx.x;
+  if (l >= nl) return;
+  uint32_t s = leaf_start[l], e = leaf_start[l + 1], m = e - s;
+  uint32_t P = 1;
+  while (P < m) P <<= 1;
+  for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+    sk[i] = (i < m) ? sidx[s + i] : 0xffffffffu;
+    sv[i] = s + i;
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+        uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          bool up = (i & k) == 0;
+          uint32_t a = sk[i], b = sk[ixj];
+          if ((a > b) == up) {
+            sk[i] = b; sk[ixj] = a;
+            uint32_t t = sv[i]; sv[i] = sv[ixj]; sv[ixj] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
+    canon[s + r] = sv[r];
+    crank[sv[r]] = s + r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+static int default_passes(double side)
+{
+  // deepest level whose cells are still >= ~1 m, plus one
+  int lv = 1;
+  while (lv < WB_MAX_LEVEL && side / (double)(1ll << lv) > 1.0) lv++;
+  int p = (3 * (lv + 1) + 7) / 8;
+  return std::min(8, std::max(2, p));
+}
+
+void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32_t *dz)
+{
+  wb_ctx *c = t->ctx;
+  const uint64_t n = t->n;
+  Xform f;
+  f.sx = t->scale[0]; f.sy = t->scale[1]; f.sz = t->scale[2];
+  f.ox = t->offset[0]; f.oy = t->offset[1]; f.oz = t->offset[2];
+  f.lox = t->center[0] - t->side / 2;
+  f.loy = t->center[1] - t->side / 2;
+  f.loz = t->center[2] - t->side / 2;
+  f.cell = t->side / (double)(1 << WB_MAX_LEVEL);
+  f.inv_cell = (double)(1 << WB_MAX_LEVEL) / t->side;
+
+  t->key = wb_talloc<uint64_t>(t, n);
+  t->sidx = wb_talloc<uint32_t>(t, n);
+  int passes = t->sort_passes > 0 ? t->sort_passes : default_passes(t->side);
+  const uint32_t node_cap = (uint32_t)std::min<uint64_t>(n / 64 + 1024, 0x7fffffff);
+  t->node_cube = wb_talloc<double>(t, 4 * (size_t)node_cap);
+  t->node_child = wb_talloc<int32_t>(t, 8 * (size_t)node_cap);
+  t->node_parent = wb_talloc<int32_t>(t, node_cap);
+  uint32_t leaf_cap = (uint32_t)std::min<uint64_t>(n / 8 + 1024, 0x7fffffff);
+  LeafTmp *lt = nullptr;
+  SplitOut *d_out = (SplitOut *)wb_alloc(c, sizeof(SplitOut));
+  SplitOut h_out;
+  Active *act = (Active *)wb_alloc(c, sizeof(Active) * node_cap);
+  Active *nxt = (Active *)wb_alloc(c, sizeof(Active) * node_cap);
+  for (;;) {
+    if (n > 0) {
+      hipLaunchKernelGGL(k_keygen, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, dx, dy, dz,
+                         n, f, t->key, t->sidx);
+      wb_radix_sort_pairs(c, t->key, t->sidx, n, passes);
+    }
+    // root node 0
+    double root[4] = {t->center[0], t->center[1], t->center[2], t->side};
+    int32_t empty8[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    int32_t m1 = -1;
+    WB_CHECK(hipMemcpyAsync(t->node_cube, root, sizeof root, hipMemcpyHostToDevice, c->stream));
+    WB_CHECK(hipMemcpyAsync(t->node_child, empty8, sizeof empty8, hipMemcpyHostToDevice, c->stream));
+    WB_CHECK(hipMemcpyAsync(t->node_parent, &m1, 4, hipMemcpyHostToDevice, c->stream));
+    memset(&h_out, 0, sizeof h_out);
+    h_out.n_node = 1;
+    WB_CHECK(hipMemcpyAsync(d_out, &h_out, sizeof h_out, hipMemcpyHostToDevice, c->stream));
+    if (lt) wb_free(c, lt, sizeof(LeafTmp) * leaf_cap);
+    lt = (LeafTmp *)wb_alloc(c, sizeof(LeafTmp) * leaf_cap);
+    int max_level = std::min(WB_MAX_LEVEL, (8 * passes) / 3);
+    uint32_t nact = 0;
+    if (n > 0) {
+      Active r0 = {0u, (uint32_t)n, 0, 0};
+      WB_CHECK(hipMemcpyAsync(act, &r0, sizeof r0, hipMemcpyHostToDevice, c->stream));
+      nact = 1;
+    }
+    for (int level = 0; nact > 0; level++) {
+      uint32_t zero = 0;
+      WB_CHECK(hipMemcpyAsync(&d_out->n_next, &zero, 4, hipMemcpyHostToDevice, c->stream));
+      hipLaunchKernelGGL(k_split_level, dim3(wb_blocks(nact, 128)), dim3(128), 0, c->stream,
+                         t->key, act, nact, level, max_level, nxt, node_cap, lt, leaf_cap,
+                         t->node_cube, t->node_child, t->node_parent, node_cap, d_out);
+      WB_CHECK(hipMemcpyAsync(&h_out, d_out, sizeof h_out, hipMemcpyDeviceToHost, c->stream));
+      WB_CHECK(hipStreamSynchronize(c->stream));
+      if (h_out.overflow) break;
+      nact = h_out.n_next;
+      std::swap(act, nxt);
+    }
+    if (h_out.overflow & 1u) {
+      if (passes >= 8) throw std::string("octree deeper than 21 levels (more than 537 points in a level-21 cell)");
+      passes++;
+      continue;  // the sort did not reach deep enough: redo with one more digit
+    }
+    if (h_out.overflow) throw std::string("octree node/leaf capacity exceeded");
+    break;
+  }
+  wb_free(c, act, sizeof(Active) * node_cap);
+  wb_free(c, nxt, sizeof(Active) * node_cap);
+  t->sort_passes = passes;
+  t->nleaves = h_out.n_leaf;
+  t->nnodes = h_out.n_node;
+  t->maxdepth = (int)h_out.maxdepth;
+  wb_free(c, d_out, sizeof(SplitOut));
+
+  // leaves in DFS order = ascending start
+  const uint32_t nl = t->nleaves;
+  t->leaf_start = wb_talloc<uint32_t>(t, nl + 1);
+  t->leaf_cube = wb_talloc<double>(t, 4 * (size_t)nl + 4);
+  t->leaf_parent = wb_talloc<int32_t>(t, nl + 1);
+  if (nl > 0) {
+    uint32_t *lk = (uint32_t *)wb_alloc(c, 4 * (size_t)nl), *lv = (uint32_t *)wb_alloc(c, 4 * (size_t)nl);
+    uint32_t *inv = (uint32_t *)wb_alloc(c, 4 * (size_t)nl);
+    hipLaunchKernelGGL(k_leaf_keys, dim3(wb_blocks(nl, 256)), dim3(256), 0, c->stream, lt, nl, lk, lv);
+    wb_radix_sort_u32_pairs(c, lk, lv, nl);
+    hipLaunchKernelGGL(k_leaf_final, dim3(wb_blocks(nl + 1, 256)), dim3(256), 0, c->stream, lt, lv,
+                       nl, n, t->leaf_start, t->leaf_cube, t->leaf_parent, inv);
+    hipLaunchKernelGGL(k_patch_children, dim3(wb_blocks((uint64_t)t->nnodes * 8, 256)), dim3(256),
+                       0, c->stream, t->node_child, t->nnodes, inv);
+    wb_free(c, lk, 4 * (size_t)nl);
+    wb_free(c, lv, 4 * (size_t)nl);
+    wb_free(c, inv, 4 * (size_t)nl);
+  } else {
+    uint32_t zero = 0;
+    WB_CHECK(hipMemcpyAsync(t->leaf_start, &zero, 4, hipMemcpyHostToDevice, c->stream));
+  }
+  wb_free(c, lt, sizeof(LeafTmp) * leaf_cap);
+
+  t->pt_leaf = wb_talloc<uint32_t>(t, n);
+  t->px = wb_talloc<double>(t, n);
+  t->py = wb_talloc<double>(t, n);
+  t->pz = wb_talloc<double>(t, n);
+  t->canon = wb_talloc<uint32_t>(t, n);
+  t->crank = wb_talloc<uint32_t>(t, n);
+  uint64_t nchunks = (n + WB_CHUNK - 1) / WB_CHUNK;
+  t->chunk_box = wb_talloc<double>(t, 6 * nchunks + 6);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_fill_pt_leaf, dim3(nl), dim3(256), 0, c->stream, t->leaf_start, nl,
+                       t->pt_leaf);
+    hipLaunchKernelGGL(k_permute, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, dx, dy, dz,
+                       t->sidx, n, f, t->px, t->py, t->pz);
+    hipLaunchKernelGGL(k_chunk_box, dim3(wb_blocks(nchunks * WB_CHUNK, 256)), dim3(256), 0,
+                       c->stream, t->px, t->py, t->pz, n, t->chunk_box);
+    hipLaunchKernelGGL(k_leaf_canon, dim3(nl), dim3(256), 0, c->stream, t->leaf_start, nl,
+                       t->sidx, t->canon, t->crank);
+  }
+}

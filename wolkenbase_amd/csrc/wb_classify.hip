@@ -1,0 +1,859 @@
+// wb_classify.hip -- shape queries and the SCAN / POSTSCAN / CLASSIFY pass.
+//
+// Equivalences used (each is exact, not approximate):
+//  * A point's class depends only on its own point, the hyperboloid size of
+//    the last tile (flowsnake counter order) whose pointsIn(cylinder) holds
+//    it, and the set D of directions to the other points of
+//    pointsIn(downward hyperboloid) (classify.cpp:96-173).  surround()'s
+//    pruning only removes directions whose two neighbour gaps are both under
+//    30 deg and never the first or last one, so it never creates or removes
+//    a gap of 144 deg; and the first loop over the previous point's
+//    downPoints only inserts directions of points that the second loop
+//    inserts too.  So class 1 <=> |D|>1 and every gap between successive
+//    sorted directions, and the wrap-around gap, is < DEG144.
+//  * When every direction is in [-2^30, 2^30), those gaps are the circular
+//    gaps of the directions mod 2^31, and adding directions only shrinks
+//    them: the test may stop as soon as the directions seen so far pass.
+//    The host checks the cloud extent (< 1e7 scale units) that guarantees
+//    the smallest non-zero dy/|dx| keeps every direction >= 30 units below
+//    +180 deg.
+//  * Gaps are tracked with a 64-bit occupancy mask of 2^25-unit bins.  A
+//    longest circular run of <= 23 empty bins bounds every gap by
+//    25*2^25 < DEG144 (pass); >= 26 empty bins gives a gap > 26*2^25 > DEG144
+//    (fail); a run of 24 or 25 is decided in a second pass that records the
+//    exact extreme directions in the two bins bordering the run.
+//  * Skipping a 32-point group whose bounding box fails the shape test is
+//    exact: clamping the shape centre into the box makes each coordinate
+//    difference no larger than for any point inside, and every predicate is
+//    monotone in those differences.
+#include "wb_flowsnake.h"
+#include "wb_internal.h"
+#include "../../include/wb_gpu.h"
+
+#define WAVE 64
+constexpr double kSqrt34 = 0.86602540378443864676372317;  // eisenstein.h:33
+constexpr double kSqrt7 = 2.6457513110645905905016;       // scan.h:23
+
+struct TreeView {
+  const double *px, *py, *pz;
+  const uint32_t *leaf_start;
+  const double *leaf_cube;
+  const int32_t *leaf_parent;
+  const double *node_cube;
+  const int32_t *node_child, *node_parent;
+  const double *chunk_box;
+  const uint32_t *pt_leaf, *canon, *crank, *sidx;
+  uint32_t nleaves, nnodes;
+  uint64_t n;
+};
+
+static TreeView view(const wb_tree *t)
+{
+  TreeView v;
+  v.px = t->px; v.py = t->py; v.pz = t->pz;
+  v.leaf_start = t->leaf_start; v.leaf_cube = t->leaf_cube; v.leaf_parent = t->leaf_parent;
+  v.node_cube = t->node_cube; v.node_child = t->node_child; v.node_parent = t->node_parent;
+  v.chunk_box = t->chunk_box; v.pt_leaf = t->pt_leaf; v.canon = t->canon; v.crank = t->crank;
+  v.sidx = t->sidx; v.nleaves = t->nleaves; v.nnodes = t->nnodes; v.n = t->n;
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Octree::findBlocks (octree.cpp:234-251) without a stack: the next child
+// to visit at each depth is kept in 4-bit fields of two registers and the
+// walk goes back up through parent links.  Leaves are visited in DFS order.
+// f(leaf) returns true to stop the walk.
+// ---------------------------------------------------------------------------
+struct Iter {
+  uint64_t lo, hi;
+  __device__ int get(int d) const { return d < 16 ? (int)((lo >> (4 * d)) & 15) : (int)((hi >> (4 * (d - 16))) & 15); }
+  __device__ void set(int d, int v)
+  {
+    if (d < 16) lo = (lo & ~(15ull << (4 * d))) | ((uint64_t)v << (4 * d));
+    else hi = (hi & ~(15ull << (4 * (d - 16)))) | ((uint64_t)v << (4 * (d - 16)));
+  }
+};
+
+template <class F>
+__device__ inline void for_leaves(const TreeView &T, const wb::Shape &S, F &&f)
+{
+  if (T.nleaves == 0) return;
+  if (!wb::shape_intersect(S, T.node_cube[0], T.node_cube[1], T.node_cube[2], T.node_cube[3])) return;
+  int node = 0, depth = 0;
+  Iter it = {0, 0};
+  for (;;) {
+    int c = it.get(depth);
+    if (c == 8) {
+      if (depth == 0) return;
+      node = T.node_parent[node];
+      depth--;
+      continue;
+    }
+    it.set(depth, c + 1);
+    int ch = T.node_child[8 * node + c];
+    if (ch == -1) continue;
+    if (ch >= 0) {
+      const double *q = T.node_cube + 4 * ch;
+      if (wb::shape_intersect(S, q[0], q[1], q[2], q[3])) {
+        node = ch;
+        depth++;
+        it.set(depth, 0);
+      }
+      continue;
+    }
+    int leaf = -2 - ch;
+    const double *q = T.leaf_cube + 4 * leaf;
+    if (!wb::shape_intersect(S, q[0], q[1], q[2], q[3])) continue;
+    if (f(leaf)) return;
+  }
+}
+
+__device__ inline wb::Shape make_shape(int kind, const double *p)
+{
+  if (kind == wb::SHAPE_CYLINDER) return wb::make_cylinder(p[0], p[1], p[2]);
+  if (kind == wb::SHAPE_SPHERE) return wb::make_sphere(p[0], p[1], p[2], p[3]);
+  return wb::make_hyperboloid(p[0], p[1], p[2], p[3], p[4]);
+}
+
+// ---------------------------------------------------------------------------
+// Batched countPointsIn / pointsIn: one wave per shape, lanes split a leaf.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_count_points_in(TreeView T, int kind, const double *params,
+                                                        uint64_t nshapes, uint64_t *counts)
+{
+  uint64_t s = blockIdx.x;
+  if (s >= nshapes) return;
+  wb::Shape S = make_shape(kind, params + WB_SHAPE_NPARAMS * s);
+  uint64_t cnt = 0;
+  for_leaves(T, S, [&](int leaf) {
+    const double *q = T.leaf_cube + 4 * leaf;
+    uint32_t b = T.leaf_start[leaf], e = T.leaf_start[leaf + 1];
+    if (wb::shape_in_cube(S, q[0], q[1], q[2], q[3])) {
+      cnt += e - b;
+      return false;
+    }
+    for (uint32_t j0 = b; j0 < e; j0 += WAVE) {
+      uint32_t j = j0 + threadIdx.x;
+      bool in = j < e && wb::shape_in(S, T.px[j], T.py[j], T.pz[j]);
+      cnt += __popcll(__ballot(in));
+    }
+    return false;
+  });
+  if (threadIdx.x == 0) counts[s] = cnt;
+}
+
+__global__ void __launch_bounds__(64) k_points_in(TreeView T, int kind, const double *params,
+                                                  uint64_t nshapes, const uint64_t *offs,
+                                                  uint32_t *out)
+{
+  uint64_t s = blockIdx.x;
+  if (s >= nshapes) return;
+  wb::Shape S = make_shape(kind, params + WB_SHAPE_NPARAMS * s);
+  uint64_t pos = offs[s];
+  const uint64_t lanelt = (1ull << __lane_id()) - 1ull;
+  for_leaves(T, S, [&](int leaf) {
+    const double *q = T.leaf_cube + 4 * leaf;
+    uint32_t b = T.leaf_start[leaf], e = T.leaf_start[leaf + 1];
+    bool all = wb::shape_in_cube(S, q[0], q[1], q[2], q[3]);
+    for (uint32_t j0 = b; j0 < e; j0 += WAVE) {
+      uint32_t c = j0 + threadIdx.x;  // canonical position
+      bool in = false;
+      uint32_t slot = 0;
+      if (c < e) {
+        slot = T.canon[c];
+        in = all || wb::shape_in(S, T.px[slot], T.py[slot], T.pz[slot]);
+      }
+      uint64_t bal = __ballot(in);
+      if (in) out[pos + __popcll(bal & lanelt)] = T.sidx[slot];
+      pos += __popcll(bal);
+    }
+    return false;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Tiles
+// ---------------------------------------------------------------------------
+struct TileGeo {
+  double scx, scy, spacing, r;
+  int lo, hi;
+  uint32_t ntiles;
+};
+
+// Flowsnake::cyl, flowsnake.cpp:263-271
+__device__ inline void tile_center(const TileGeo &g, int ex, int ey, double *cx, double *cy)
+{
+  double zr = ex - ey / 2., zi = ey * kSqrt34;
+  zr *= g.spacing;
+  zi *= g.spacing;
+  *cx = zr + g.scx;
+  *cy = zi + g.scy;
+}
+
+__global__ void k_tile_geo(TileGeo g, int32_t *tex, int32_t *tey)
+{
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.ntiles) return;
+  int ex, ey;
+  wb::to_flowsnake_dev((int64_t)g.lo + k, &ex, &ey);
+  tex[k] = ex;
+  tey[k] = ey;
+}
+
+#define MAX_MEMB 4
+
+// pointsIn(cylinder) membership of every point, evaluated from the point's
+// side: the candidates are the lattice tiles near it; a tile holds the point
+// iff findBlocks reaches its leaf (every ancestor cube and the leaf cube
+// intersect the cylinder) and either all eight leaf corners are in or the
+// point itself is.
+__global__ void k_membership(TreeView T, TileGeo g, int32_t *memb, uint32_t *nmemb,
+                             uint32_t *tile_cnt, uint32_t *err)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= T.n) return;
+  const double x = T.px[i], y = T.py[i];
+  const uint32_t leaf = T.pt_leaf[i];
+  const double *lc = T.leaf_cube + 4 * leaf;
+  double u = (x - g.scx) / g.spacing, v = (y - g.scy) / g.spacing;
+  double fy = v / kSqrt34;
+  int Y = (int)__builtin_rint(fy);
+  int X = (int)__builtin_rint(u + fy / 2);
+  const double lim = g.r * (1 + 1e-9) + 1e-9;
+  int nm = 0;
+  int32_t mem[MAX_MEMB];
+  for (int b = -2; b <= 2; b++)
+    for (int a = -2; a <= 2; a++) {
+      if (a - b > 2 || b - a > 2) continue;
+      int ex = X + a, ey = Y + b;
+      double cx, cy;
+      tile_center(g, ex, ey, &cx, &cy);
+      double dx = cx - x, dy = cy - y;
+      if (dx * dx + dy * dy > lim * lim) continue;
+      int cnt;
+      if (!wb::from_flowsnake_dev(ex, ey, g.lo, g.hi, &cnt)) continue;
+      wb::Shape cyl = wb::make_cylinder(cx, cy, g.r);
+      bool in = wb::shape_in_cube(cyl, lc[0], lc[1], lc[2], lc[3]) || wb::cyl_in(cyl, x, y);
+      if (!in) continue;
+      if (!wb::shape_intersect(cyl, lc[0], lc[1], lc[2], lc[3])) continue;
+      bool reach = true;
+      for (int nd = T.leaf_parent[leaf]; nd >= 0; nd = T.node_parent[nd]) {
+        const double *q = T.node_cube + 4 * nd;
+        if (!wb::shape_intersect(cyl, q[0], q[1], q[2], q[3])) { reach = false; break; }
+      }
+      if (!reach) continue;
+      if (nm < MAX_MEMB) mem[nm] = cnt;
+      nm++;
+    }
+  if (nm > MAX_MEMB) {
+    atomicOr(err, 1u);
+    nm = MAX_MEMB;
+  }
+  for (int k = 0; k < MAX_MEMB; k++) memb[MAX_MEMB * i + k] = k < nm ? mem[k] : INT32_MIN;
+  nmemb[i] = nm;
+  for (int k = 0; k < nm; k++) atomicAdd(&tile_cnt[mem[k] - g.lo], 1u);
+}
+
+__global__ void k_tile_lists(uint64_t n, const int32_t *memb, const uint32_t *nmemb, int lo,
+                             const uint32_t *crank, const uint32_t *tile_off, uint32_t *tile_fill,
+                             uint32_t *lists)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t nm = nmemb[i];
+  for (uint32_t k = 0; k < nm; k++) {
+    int t = memb[MAX_MEMB * i + k] - lo;
+    uint32_t slot = atomicAdd(&tile_fill[t], 1u);
+    lists[tile_off[t] + slot] = crank[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SCAN: one wave per tile (scanCylinder, scan.cpp:31-140)
+// ---------------------------------------------------------------------------
+#define SCAN_CAP 1024
+
+// pairwisesum (manysum.cpp:120-154) of m <= SCAN_CAP values produced by
+// val(j): one balanced tree per set bit of m, over the aligned block that
+// bit owns (largest block first), block sums added to 0 in ascending bit
+// order.  Balanced trees of up to 64 values are xor butterflies across the
+// wave; larger blocks combine their 64-value chunk sums the same way.
+template <class V>
+__device__ inline double wave_pairwise(uint32_t m, V &&val)
+{
+  const int lane = __lane_id();
+  double acc = 0;
+  for (int k = 0; k < 11; k++) {
+    if (!((m >> k) & 1u)) continue;
+    uint32_t off = (m >> (k + 1)) << (k + 1);
+    uint32_t sz = 1u << k;
+    double blk;
+    if (sz <= 64) {
+      double v = (uint32_t)lane < sz ? val(off + lane) : 0.0;
+      for (uint32_t s = 1; s < sz; s <<= 1) v = v + __shfl_xor(v, (int)s, WAVE);
+      blk = __shfl(v, 0, WAVE);
+    } else {
+      // chunk sums of 64, then a balanced tree over the chunks
+      uint32_t nch = sz / 64;
+      double mine = 0;
+      for (uint32_t ch = 0; ch < nch; ch++) {
+        double v = val(off + ch * 64 + lane);
+        for (int s = 1; s < 64; s <<= 1) v = v + __shfl_xor(v, s, WAVE);
+        if ((uint32_t)lane == ch) mine = v;
+      }
+      for (uint32_t s = 1; s < nch; s <<= 1) mine = mine + __shfl_xor(mine, (int)s, WAVE);
+      blk = __shfl(mine, 0, WAVE);
+    }
+    acc += blk;
+  }
+  return acc;
+}
+
+__device__ inline double pw3(double a0, double a1, double a2, int n)
+{
+  // pairwisesum of n <= 3 values
+  if (n == 1) return 0 + a0;
+  if (n == 2) return 0 + (a0 + a1);
+  return (0 + a2) + (a0 + a1);
+}
+
+// 3x3 Gauss-Jordan of matrix.cpp:262-422 on a (3x3) and b (3x1)
+__device__ inline void swap_rows3(double *a, double *b, int r0, int r1)
+{
+  for (int c = 0; c < 3; c++) { double t = a[3 * r0 + c]; a[3 * r0 + c] = a[3 * r1 + c]; a[3 * r1 + c] = t; }
+  double t = b[r0]; b[r0] = b[r1]; b[r1] = t;
+}
+
+__device__ inline void rowop3(double *a, double *b, int row0, int row1, int piv)
+{
+  double slope = 0, minslope = __builtin_inf(), det;
+  int flags = 0, pivot;
+  double *r0 = a + 3 * row0, *r1 = a + 3 * row1;
+  if (piv >= 0 && r0[piv] == 0 && r1[piv] == 0) piv = -1;
+  pivot = piv;
+  if (piv >= 0 && r0[piv] == 0) flags = 9;
+  for (int i = 0; piv < 0 && i < 3; i++)
+    if (r0[i] != 0 || r1[i] != 0) {
+      if (wb::dabs(r0[i]) > wb::dabs(r1[i]) || row0 >= row1) {
+        slope = wb::dabs(r1[i] / r0[i]);
+        flags &= ~8;
+      } else {
+        slope = wb::dabs(r0[i] / r1[i]);
+        flags |= 8;
+      }
+      if (slope < minslope) {
+        minslope = slope;
+        flags = (flags >> 3) * 9;
+        pivot = i;
+      }
+    }
+  flags &= 1;
+  if (flags) swap_rows3(a, b, row0, row1);
+  det = (pivot < 0) ? 0 : r0[pivot];
+  if (det != 0 && det != 1) {
+    for (int i = 0; i < 3; i++) r0[i] /= det;
+    b[row0] /= det;
+  }
+  if (pivot >= 0) slope = r1[pivot];
+  if (slope != 0 && row0 != row1) {
+    for (int i = 0; i < 3; i++) r1[i] -= r0[i] * slope;
+    b[row1] -= b[row0] * slope;
+  }
+}
+
+__device__ inline void findpivot3(double *a, double *b, int row, int column)
+{
+  int pivotrow = -1;
+  double maxratio = 0;
+  for (; pivotrow < row && column < 3; column++) {
+    for (int i = row; i < 3; i++) {
+      double sq[3] = {0, 0, 0};
+      for (int j = column + 1; j < 3; j++) sq[j - column - 1] = a[3 * i + j] * a[3 * i + j];
+      double piv = a[3 * i + column];
+      double ratio = (piv * piv) / pw3(sq[0], sq[1], sq[2], 3 - column);
+      if (ratio > maxratio) {
+        pivotrow = i;
+        maxratio = ratio;
+      }
+    }
+  }
+  if (pivotrow > row) swap_rows3(a, b, pivotrow, row);
+}
+
+__device__ inline void gausselim3(double *a, double *b)
+{
+  for (int i = 0; i < 3; i++) {
+    findpivot3(a, b, i, i);
+    for (int j = 0; j < 3; j++) rowop3(a, b, i, j, i);
+  }
+  for (int i = 2; i >= 0; i--)
+    for (int j = 0; j < i; j++) rowop3(a, b, i, j, i);
+}
+
+__global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_t *tex,
+                                             const int32_t *tey, const uint32_t *tile_off,
+                                             const uint32_t *tile_cnt, const uint32_t *lists,
+                                             double min_hs, TileRec *rec, uint32_t *err,
+                                             unsigned long long *borderline)
+{
+  __shared__ uint32_t cp[SCAN_CAP];
+  __shared__ double vx[SCAN_CAP], vy[SCAN_CAP], vz[SCAN_CAP];
+  __shared__ int histo[7];
+  const uint32_t t = blockIdx.x;
+  if (t >= g.ntiles) return;
+  const int lane = threadIdx.x;
+  const uint32_t m = tile_cnt[t];
+  if (m == 0) {
+    if (lane == 0) {
+      TileRec r = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0};
+      rec[t] = r;
+    }
+    return;
+  }
+  if (m > SCAN_CAP) {
+    if (lane == 0) atomicOr(err, 2u);
+    return;
+  }
+  uint32_t P = 1;
+  while (P < m) P <<= 1;
+  const uint32_t off = tile_off[t];
+  for (uint32_t j = lane; j < P; j += WAVE) cp[j] = j < m ? lists[off + j] : 0xffffffffu;
+  if (lane < 7) histo[lane] = 0;
+  __syncthreads();
+  // canonical order of the tile's points = ascending canonical position
+  for (uint32_t k = 2; k <= P; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = lane; i < P; i += WAVE) {
+        uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          bool up = (i & k) == 0;
+          uint32_t a = cp[i], b = cp[ixj];
+          if ((a > b) == up) { cp[i] = b; cp[ixj] = a; }
+        }
+      }
+      __syncthreads();
+    }
+  double cx, cy;
+  tile_center(g, tex[t], tey[t], &cx, &cy);
+  const double r = g.r;
+  for (uint32_t j = lane; j < m; j += WAVE) {
+    uint32_t s = T.canon[cp[j]];
+    vx[j] = T.px[s] - cx;   // location - xyz(cyl centre, 0)
+    vy[j] = T.py[s] - cy;
+    vz[j] = T.pz[s] - 0;
+  }
+  __syncthreads();
+  // normal equations of [x y 1] z (transmult and operator*, matrix.cpp:181-215)
+  double sxx = wave_pairwise(m, [&](uint32_t j) { return vx[j] * vx[j]; });
+  double syx = wave_pairwise(m, [&](uint32_t j) { return vy[j] * vx[j]; });
+  double syy = wave_pairwise(m, [&](uint32_t j) { return vy[j] * vy[j]; });
+  double s1x = wave_pairwise(m, [&](uint32_t j) { return 1.0 * vx[j]; });
+  double s1y = wave_pairwise(m, [&](uint32_t j) { return 1.0 * vy[j]; });
+  double s11 = wave_pairwise(m, [&](uint32_t j) { (void)j; return 1.0 * 1.0; });
+  double sxz = wave_pairwise(m, [&](uint32_t j) { return vx[j] * vz[j]; });
+  double syz = wave_pairwise(m, [&](uint32_t j) { return vy[j] * vz[j]; });
+  double s1z = wave_pairwise(m, [&](uint32_t j) { return 1.0 * vz[j]; });
+  double A[9] = {sxx, syx, s1x, syx, syy, s1y, s1x, s1y, s11};
+  double B[3] = {sxz, syz, s1z};
+  gausselim3(A, B);
+  for (int i = 0; i < 3; i++)
+    if (A[3 * i + i] == 0) B[i] = __builtin_nan("");
+  double sx = B[0], sy = B[1];
+  {
+    double len = wb::hypot(sx, sy);
+    if (len > 1) { sx /= len; sy /= len; }
+  }
+  if (wb::disnan(sx) || wb::disnan(sy)) { sx = 0; sy = 0; }
+  // untilt; bottom, bottom2 (the running minimum before its last update), top
+  double bottom = __builtin_inf(), top = -__builtin_inf();
+  for (uint32_t j = lane; j < m; j += WAVE) {
+    double u = vz[j] - (sy * vy[j] + sx * vx[j]);
+    vz[j] = u;
+    bottom = fmin(bottom, u);
+    top = fmax(top, u);
+  }
+  for (int s = 1; s < WAVE; s <<= 1) {
+    bottom = fmin(bottom, __shfl_xor(bottom, s, WAVE));
+    top = fmax(top, __shfl_xor(top, s, WAVE));
+  }
+  __syncthreads();
+  uint32_t first = 0xffffffffu;
+  for (uint32_t j = lane; j < m; j += WAVE)
+    if (vz[j] == bottom && j < first) first = j;
+  for (int s = 1; s < WAVE; s <<= 1) first = min(first, (uint32_t)__shfl_xor((int)first, s, WAVE));
+  double bottom2 = __builtin_inf();
+  for (uint32_t j = lane; j < first; j += WAVE) bottom2 = fmin(bottom2, vz[j]);
+  for (int s = 1; s < WAVE; s <<= 1) bottom2 = fmin(bottom2, __shfl_xor(bottom2, s, WAVE));
+  if (wb::disinf(bottom2)) bottom2 = bottom;
+  // 7-sector histogram of the bottom layer
+  uint32_t nbottom = 0;
+  for (uint32_t j = lane; j < m; j += WAVE) {
+    if (!(vz[j] < bottom2 + 2 * r)) continue;
+    nbottom++;
+    double th = atan2(vy[j], vx[j]) * 3 / M_PI;
+    long sec = lrint(th);
+    double fr = th - floor(th);
+    if (wb::dabs(fr - 0.5) < 1e-12) atomicAdd(borderline, 1ull);
+    if (sec < 0) sec += 6;
+    sec = (sec % 6) + 1;
+    if (wb::hypot(vx[j], vy[j]) < r / kSqrt7) sec = 0;
+    atomicAdd(&histo[sec], 1);
+  }
+  for (int s = 1; s < WAVE; s <<= 1) nbottom += __shfl_xor((int)nbottom, s, WAVE);
+  __syncthreads();
+  if (lane == 0) {
+    double density = 0;
+    int tree = 0;
+    for (int i = 0; i < 7; i++) density += histo[i] * histo[i];
+    if (m > nbottom && density < 7) tree = 1;
+    density = sqrt(density) * kSqrt7 / (r * r) / M_PI;
+    if (m > nbottom && density < 0.5) tree = 1;
+    if (top - bottom > 1.5) tree = 1;
+    TileRec rr;
+    rr.npoints = (int32_t)m;
+    rr.nground = 0;
+    rr.tree_flags = tree;
+    rr.pad = 0;
+    rr.density = density;
+    rr.hs_scan = sqrt(1 / density + min_hs * min_hs);
+    rr.hs = rr.hs_scan;
+    rr.height = top - bottom;
+    rec[t] = rr;
+  }
+}
+
+// POSTSCAN (scan.cpp:142-179): tree tiles widen their hyperboloid by the
+// number of tree tiles met along six rays before an empty ring or a
+// non-tree tile.
+__global__ void k_postscan(TileGeo g, const int32_t *tex, const int32_t *tey, TileRec *rec)
+{
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.ntiles) return;
+  TileRec me = rec[t];
+  if (!me.npoints) return;
+  const int r1x[6] = {1, 1, 0, -1, -1, 0}, r1y[6] = {0, 1, 1, 0, -1, -1};  // root1, eisenstein.cpp:51
+  int i = 1, ring, nontree, count = 0;
+  do {
+    ring = nontree = 0;
+    for (int j = 0; j < 6 && (me.tree_flags & 1); j++) {
+      int o;
+      if (!wb::from_flowsnake_dev(tex[t] + r1x[j] * i, tey[t] + r1y[j] * i, g.lo, g.hi, &o)) continue;
+      const TileRec &q = rec[o - g.lo];
+      if (q.npoints) {
+        ring++;
+        if (q.tree_flags & 1) count++;
+        else nontree++;
+      }
+    }
+    ++i;
+  } while (ring && !nontree);
+  double w = count * g.spacing / 6;
+  rec[t].hs = sqrt(me.hs_scan * me.hs_scan + w * w);
+}
+
+// ---------------------------------------------------------------------------
+// CLASSIFY
+// ---------------------------------------------------------------------------
+struct ClassifyArgs {
+  TreeView T;
+  TileGeo g;
+  const int32_t *memb;
+  const uint32_t *nmemb;
+  const TileRec *rec;
+  double thickness, max_slope;
+  const double *tab;     // tan[511], cos[512], sin[512] packed
+  wb::AtanConst K;
+  uint8_t *cls;          // per sorted slot
+  uint32_t *slow;        // slots needing the exact gap pass
+  uint32_t *nslow;
+  unsigned long long *atan_exact;
+};
+
+__device__ inline uint64_t rotr64(uint64_t x, int a) { return (x >> a) | (x << (64 - a)); }
+
+// classification of the empty-bin mask: 1 pass, 2 fail, 0 undecided
+__device__ inline int gap_verdict(uint64_t occupied)
+{
+  uint64_t e = ~occupied;
+  uint64_t r2 = e & rotr64(e, 1);
+  uint64_t r4 = r2 & rotr64(r2, 2);
+  uint64_t r8 = r4 & rotr64(r4, 4);
+  uint64_t r16 = r8 & rotr64(r8, 8);
+  uint64_t r24 = r16 & rotr64(r8, 16);
+  if (r24 == 0) return 1;
+  uint64_t r26 = r24 & rotr64(e, 24) & rotr64(e, 25);
+  if (r26 != 0) return 2;
+  return 0;
+}
+
+__device__ inline int dir_bin_pos(int a) { return (int)(((uint32_t)a + 0x40000000u) & 0x7fffffffu); }
+
+template <class OnDir>
+__device__ inline void enumerate_down(const ClassifyArgs &A, uint32_t i, const double *tabs,
+                                      OnDir &&on_dir)
+{
+  const TreeView &T = A.T;
+  const double x = T.px[i], y = T.py[i], z = T.pz[i];
+  int w = INT32_MIN;
+  uint32_t nm = A.nmemb[i];
+  for (uint32_t k = 0; k < nm; k++) w = max(w, A.memb[MAX_MEMB * i + k]);
+  const double hs = A.rec[w - A.g.lo].hs;
+  wb::Shape H = wb::make_hyperboloid(x - 0, y - 0, z - A.thickness, hs, A.max_slope);
+  for_leaves(T, H, [&](int leaf) {
+    const double *q = T.leaf_cube + 4 * leaf;
+    const bool all = wb::shape_in_cube(H, q[0], q[1], q[2], q[3]);
+    const uint32_t b = T.leaf_start[leaf], e = T.leaf_start[leaf + 1];
+    for (uint32_t c0 = b; c0 < e;) {
+      uint32_t ch = c0 / WB_CHUNK;
+      uint32_t ce = min(e, (ch + 1) * WB_CHUNK);
+      if (!all) {
+        const double *bx = T.chunk_box + 6 * (uint64_t)ch;
+        if (!wb::shape_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5])) {
+          c0 = ce;
+          continue;
+        }
+      }
+      for (uint32_t j = c0; j < ce; j++) {
+        double qx = T.px[j], qy = T.py[j];
+        if (!all && !wb::hyp_in(H, qx, qy, T.pz[j])) continue;
+        if (qx == x && qy == y) continue;  // dist(xy(p), xy(q)) == 0
+        if (on_dir(qy - y, qx - x)) return true;
+      }
+      c0 = ce;
+    }
+    return false;
+  });
+}
+
+__global__ void __launch_bounds__(256) k_classify(ClassifyArgs A)
+{
+  __shared__ double tabs[511 + 512 + 512];
+  for (int k = threadIdx.x; k < 511 + 512 + 512; k += blockDim.x) tabs[k] = A.tab[k];
+  __syncthreads();
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.T.n) return;
+  if (A.nmemb[i] == 0) {
+    A.cls[i] = 0;
+    return;
+  }
+  const double *ttan = tabs, *tcos = tabs + 511, *tsin = tabs + 1023;
+  uint64_t occ = 0, checked = 0;
+  int verdict = 0, nexact = 0;
+  enumerate_down(A, (uint32_t)i, tabs, [&](double dy, double dx) {
+    int a = wb::atan2i(dy, dx, ttan, tcos, tsin, A.K, &nexact);
+    occ |= 1ull << (dir_bin_pos(a) >> 25);
+    if (occ != checked) {
+      checked = occ;
+      if (gap_verdict(occ) == 1) { verdict = 1; return true; }
+    }
+    return false;
+  });
+  if (verdict == 0) verdict = gap_verdict(occ);
+  if (nexact) atomicAdd(A.atan_exact, (unsigned long long)nexact);
+  if (verdict == 0) {
+    uint32_t s = atomicAdd(A.nslow, 1u);
+    A.slow[s] = (uint32_t)i;
+    A.cls[i] = 0;
+    return;
+  }
+  A.cls[i] = verdict == 1 ? 1 : 2;
+}
+
+// exact pass for points whose longest empty run is 24 or 25 bins: record
+// the largest direction in the bin before each such run and the smallest in
+// the bin after it, and compare the exact gaps with DEG144
+__global__ void __launch_bounds__(64) k_classify_exact(ClassifyArgs A, uint32_t nslow)
+{
+  __shared__ double tabs[511 + 512 + 512];
+  for (int k = threadIdx.x; k < 511 + 512 + 512; k += blockDim.x) tabs[k] = A.tab[k];
+  __syncthreads();
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslow) return;
+  uint32_t i = A.slow[s];
+  const double *ttan = tabs, *tcos = tabs + 511, *tsin = tabs + 1023;
+  // pass 1: occupancy
+  uint64_t occ = 0;
+  enumerate_down(A, i, tabs, [&](double dy, double dx) {
+    int a = wb::atan2i(dy, dx, ttan, tcos, tsin, A.K);
+    occ |= 1ull << (dir_bin_pos(a) >> 25);
+    return false;
+  });
+  // runs of 24 or 25 empty bins
+  int before[2], after[2], nr = 0;
+  for (int st = 0; st < 64; st++) {
+    if ((occ >> st) & 1) continue;
+    if (!((occ >> ((st + 63) & 63)) & 1)) continue;  // not a run start
+    int len = 0;
+    while (len < 64 && !((occ >> ((st + len) & 63)) & 1)) len++;
+    if (len >= 24 && nr < 2) {
+      before[nr] = (st + 63) & 63;
+      after[nr] = (st + len) & 63;
+      nr++;
+    }
+  }
+  int hi[2] = {-1, -1}, lo[2] = {0x7fffffff, 0x7fffffff};
+  enumerate_down(A, i, tabs, [&](double dy, double dx) {
+    int a = wb::atan2i(dy, dx, ttan, tcos, tsin, A.K);
+    int pos = dir_bin_pos(a), bin = pos >> 25;
+    for (int k = 0; k < nr; k++) {
+      if (bin == before[k] && pos > hi[k]) hi[k] = pos;
+      if (bin == after[k] && pos < lo[k]) lo[k] = pos;
+    }
+    return false;
+  });
+  bool pass = true;
+  for (int k = 0; k < nr; k++) {
+    int gap = (int)(((uint32_t)lo[k] - (uint32_t)hi[k]) & 0x7fffffffu);
+    if (gap >= wb::DEG144) pass = false;
+  }
+  A.cls[i] = pass ? 1 : 2;
+}
+
+__global__ void k_scatter_class(uint64_t n, const uint8_t *cls, const uint32_t *sidx, uint8_t *out)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[sidx[i]] = cls[i];
+}
+
+__global__ void k_count_ground(uint64_t n, const uint8_t *cls, const int32_t *memb,
+                               const uint32_t *nmemb, int lo, int32_t *nground,
+                               unsigned long long *hist3)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint8_t c = cls[i];
+  atomicAdd(&hist3[c < 3 ? c : 0], 1ull);
+  if (c == 2) {
+    int w = INT32_MIN;
+    for (uint32_t k = 0; k < nmemb[i]; k++) w = max(w, memb[MAX_MEMB * i + k]);
+    atomicAdd(&nground[w - lo], 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+void wb_run_count(wb_tree *t, int kind, const double *dparams, uint64_t ns, uint64_t *dcounts)
+{
+  if (ns == 0) return;
+  hipLaunchKernelGGL(k_count_points_in, dim3((unsigned)ns), dim3(64), 0, t->ctx->stream, view(t),
+                     kind, dparams, ns, dcounts);
+}
+
+void wb_run_points(wb_tree *t, int kind, const double *dparams, uint64_t ns, const uint64_t *doffs,
+                   uint32_t *dout)
+{
+  if (ns == 0) return;
+  hipLaunchKernelGGL(k_points_in, dim3((unsigned)ns), dim3(64), 0, t->ctx->stream, view(t), kind,
+                     dparams, ns, doffs, dout);
+}
+
+void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
+                     const wb_classify_params *p, uint8_t *d_class_out, bool want_tiles,
+                     ClassifyOut *out)
+{
+  wb_ctx *c = t->ctx;
+  hipStream_t st = c->stream;
+  const uint64_t n = t->n;
+  wb::SnakeSetup ss = wb::snake_setup(bbmin, bbmax, p->tile_size);
+  TileGeo g;
+  g.scx = ss.cx; g.scy = ss.cy; g.spacing = ss.spacing; g.r = ss.radius;
+  g.lo = ss.lo; g.hi = ss.hi;
+  g.ntiles = (uint32_t)((int64_t)ss.hi - ss.lo + 1);
+  hipEvent_t ev[4];
+  for (int k = 0; k < 4; k++) WB_CHECK(hipEventCreate(&ev[k]));
+  WB_CHECK(hipEventRecord(ev[0], st));
+  int32_t *tex = (int32_t *)wb_alloc(c, 4ull * g.ntiles), *tey = (int32_t *)wb_alloc(c, 4ull * g.ntiles);
+  uint32_t *tile_cnt = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
+  uint32_t *tile_fill = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
+  TileRec *rec = (TileRec *)wb_alloc(c, sizeof(TileRec) * (size_t)g.ntiles);
+  int32_t *memb = (int32_t *)wb_alloc(c, 4ull * MAX_MEMB * n + 16);
+  uint32_t *nmemb = (uint32_t *)wb_alloc(c, 4ull * n + 4);
+  uint32_t *err = (uint32_t *)wb_alloc(c, 64);
+  unsigned long long *ctr = (unsigned long long *)err + 1;  // borderline, atan_exact, hist[3]
+  uint32_t *nslow = (uint32_t *)(ctr + 6);
+  WB_CHECK(hipMemsetAsync(err, 0, 64, st));
+  WB_CHECK(hipMemsetAsync(tile_cnt, 0, 4ull * g.ntiles, st));
+  WB_CHECK(hipMemsetAsync(tile_fill, 0, 4ull * g.ntiles, st));
+  TreeView T = view(t);
+  hipLaunchKernelGGL(k_tile_geo, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey);
+  if (n > 0)
+    hipLaunchKernelGGL(k_membership, dim3(wb_blocks(n, 256)), dim3(256), 0, st, T, g, memb, nmemb,
+                       tile_cnt, err);
+  // tile lists in canonical order
+  uint32_t *tile_off = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
+  WB_CHECK(hipMemcpyAsync(tile_off, tile_cnt, 4ull * g.ntiles, hipMemcpyDeviceToDevice, st));
+  uint64_t total = wb_exclusive_scan_u32(c, tile_off, g.ntiles);
+  uint32_t *lists = (uint32_t *)wb_alloc(c, 4 * total + 4);
+  if (n > 0)
+    hipLaunchKernelGGL(k_tile_lists, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, memb, nmemb,
+                       g.lo, t->crank, tile_off, tile_fill, lists);
+  WB_CHECK(hipEventRecord(ev[1], st));
+  hipLaunchKernelGGL(k_scan, dim3(g.ntiles), dim3(64), 0, st, T, g, tex, tey, tile_off, tile_cnt,
+                     lists, p->min_hs, rec, err, ctr);
+  hipLaunchKernelGGL(k_postscan, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey, rec);
+  WB_CHECK(hipEventRecord(ev[2], st));
+  // classify
+  uint8_t *cls = (uint8_t *)wb_alloc(c, n + 16);
+  uint32_t *slow = (uint32_t *)wb_alloc(c, 4 * n + 4);
+  ClassifyArgs A;
+  A.T = T; A.g = g; A.memb = memb; A.nmemb = nmemb; A.rec = rec;
+  A.thickness = p->thickness; A.max_slope = p->max_slope;
+  A.tab = c->d_tan; A.K = c->kconst; A.cls = cls; A.slow = slow; A.nslow = nslow;
+  A.atan_exact = ctr + 1;
+  if (n > 0) hipLaunchKernelGGL(k_classify, dim3(wb_blocks(n, 256)), dim3(256), 0, st, A);
+  uint32_t h_nslow = 0;
+  WB_CHECK(hipMemcpyAsync(&h_nslow, nslow, 4, hipMemcpyDeviceToHost, st));
+  WB_CHECK(hipStreamSynchronize(st));
+  if (h_nslow) hipLaunchKernelGGL(k_classify_exact, dim3(wb_blocks(h_nslow, 64)), dim3(64), 0, st, A, h_nslow);
+  int32_t *nground = (int32_t *)tile_fill;  // reuse
+  WB_CHECK(hipMemsetAsync(nground, 0, 4ull * g.ntiles, st));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_count_ground, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, cls, memb,
+                       nmemb, g.lo, nground, ctr + 2);
+    hipLaunchKernelGGL(k_scatter_class, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, cls, t->sidx,
+                       d_class_out);
+  }
+  WB_CHECK(hipEventRecord(ev[3], st));
+  uint32_t h_err[16];
+  WB_CHECK(hipMemcpyAsync(h_err, err, 64, hipMemcpyDeviceToHost, st));
+  if (want_tiles) {
+    out->tiles.resize(g.ntiles);
+    out->tex.resize(g.ntiles);
+    out->tey.resize(g.ntiles);
+    std::vector<int32_t> ng(g.ntiles);
+    WB_CHECK(hipMemcpyAsync(out->tiles.data(), rec, sizeof(TileRec) * (size_t)g.ntiles, hipMemcpyDeviceToHost, st));
+    WB_CHECK(hipMemcpyAsync(out->tex.data(), tex, 4ull * g.ntiles, hipMemcpyDeviceToHost, st));
+    WB_CHECK(hipMemcpyAsync(out->tey.data(), tey, 4ull * g.ntiles, hipMemcpyDeviceToHost, st));
+    WB_CHECK(hipMemcpyAsync(ng.data(), nground, 4ull * g.ntiles, hipMemcpyDeviceToHost, st));
+    WB_CHECK(hipStreamSynchronize(st));
+    for (uint32_t k = 0; k < g.ntiles; k++) out->tiles[k].nground = ng[k];
+  }
+  WB_CHECK(hipStreamSynchronize(st));
+  float ms1 = 0, ms2 = 0, ms3 = 0;
+  hipEventElapsedTime(&ms1, ev[0], ev[1]);
+  hipEventElapsedTime(&ms2, ev[1], ev[2]);
+  hipEventElapsedTime(&ms3, ev[2], ev[3]);
+  for (int k = 0; k < 4; k++) hipEventDestroy(ev[k]);
+  unsigned long long *hc = (unsigned long long *)(h_err + 2);
+  out->borderline = hc[0];
+  out->atan_exact = hc[1];
+  out->hist[0] = hc[2]; out->hist[1] = hc[3]; out->hist[2] = hc[4];
+  out->nslow = h_nslow;
+  out->ms_tiles = ms1; out->ms_scan = ms2; out->ms_classify = ms3;
+  wb_free(c, tex, 4ull * g.ntiles);
+  wb_free(c, tey, 4ull * g.ntiles);
+  wb_free(c, tile_cnt, 4ull * g.ntiles + 4);
+  wb_free(c, tile_fill, 4ull * g.ntiles + 4);
+  wb_free(c, rec, sizeof(TileRec) * (size_t)g.ntiles);
+  wb_free(c, memb, 4ull * MAX_MEMB * n + 16);
+  wb_free(c, nmemb, 4ull * n + 4);
+  wb_free(c, tile_off, 4ull * g.ntiles + 4);
+  wb_free(c, lists, 4 * total + 4);
+  wb_free(c, cls, n + 16);
+  wb_free(c, slow, 4 * n + 4);
+  if (h_err[0] & 1u) { wb_free(c, err, 64); throw std::string("a point lies in more than 4 tile cylinders"); }
+  if (h_err[0] & 2u) { wb_free(c, err, 64); throw std::string("a tile cylinder holds more than 1024 points"); }
+  wb_free(c, err, 64);
+}

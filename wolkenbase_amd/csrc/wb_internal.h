@@ -1,0 +1,116 @@
+// wb_internal.h -- device data layout and launch helpers of libwb_gpu.
+//
+// HBM layout of one octree (n points, nl leaves, nn internal nodes):
+//   key[n]      u64   Morton key << 1 (levels 1..21, z,y,x bits per level)
+//   sidx[n]     u32   input index of each sorted slot
+//   px,py,pz[n] f64   decoded coordinates in sorted (Morton-prefix) order
+//   pt_leaf[n]  u32   leaf of each sorted slot
+//   canon[n]    u32   canonical position -> sorted slot (leaf DFS order, then
+//                     input order inside a leaf, as the reference stores it)
+//   crank[n]    u32   sorted slot -> canonical position
+//   chunk_box[ceil(n/32)]  f64x6  bounding box of each 32-slot group
+//   leaf_start[nl+1] u32, leaf_cube[nl] f64x4 (cx,cy,cz,side), leaf_parent[nl]
+//   node_cube[nn] f64x4, node_child[8*nn] i32 (>=0 node, -2-l leaf, -1 empty),
+//   node_parent[nn] i32
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/wb_gpu.h"
+#include "wb_host.h"
+#include "wb_math.h"
+
+#define WB_RECORDS 537  // octree.h:40-41: a leaf holds at most 537 locations
+#define WB_MAX_LEVEL 21
+#define WB_CHUNK 32
+
+struct wb_stream_alloc;
+
+struct wb_ctx {
+  int device;
+  hipStream_t stream;
+  std::string err;
+  // atan2i tables and constants (device)
+  double *d_tan, *d_cos, *d_sin;
+  wb::AtanConst kconst;
+  // size-keyed cache of device buffers
+  std::multimap<size_t, void *> cache;
+  size_t cached_bytes;
+};
+
+struct wb_tree {
+  wb_ctx *ctx;
+  uint64_t n;
+  double center[3], side;
+  double scale[3], offset[3];
+  int sort_passes, maxdepth;
+  uint32_t nleaves, nnodes;
+  // device arrays
+  uint64_t *key;
+  uint32_t *sidx;
+  double *px, *py, *pz;
+  uint32_t *pt_leaf, *canon, *crank;
+  double *chunk_box;
+  uint32_t *leaf_start;
+  double *leaf_cube;
+  int32_t *leaf_parent;
+  double *node_cube;
+  int32_t *node_child, *node_parent;
+  std::vector<std::pair<void *, size_t>> owned;
+};
+
+// device allocation through the context cache
+void *wb_alloc(wb_ctx *c, size_t bytes);
+void wb_free(wb_ctx *c, void *p, size_t bytes);
+template <typename T>
+T *wb_talloc(wb_tree *t, size_t count)
+{
+  size_t b = count * sizeof(T);
+  if (b == 0) b = 16;
+  T *p = (T *)wb_alloc(t->ctx, b);
+  t->owned.push_back({(void *)p, b});
+  return p;
+}
+
+#define WB_CHECK(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      throw std::string(#call) + ": " + hipGetErrorString(e_);                      \
+    }                                                                               \
+  } while (0)
+
+// per-tile record of the classification pass (device and host)
+struct TileRec {
+  int32_t npoints, nground, tree_flags, pad;
+  double density, hs_scan, hs, height;
+};
+
+struct ClassifyOut {
+  std::vector<TileRec> tiles;
+  std::vector<int32_t> tex, tey;
+  uint64_t borderline, atan_exact, nslow, hist[3];
+  double ms_tiles, ms_scan, ms_classify;
+};
+
+// ---- implemented in wb_classify.hip ----
+void wb_run_count(wb_tree *t, int kind, const double *dparams, uint64_t ns, uint64_t *dcounts);
+void wb_run_points(wb_tree *t, int kind, const double *dparams, uint64_t ns, const uint64_t *doffs,
+                   uint32_t *dout);
+
+// ---- building blocks implemented in wb_build.hip ----
+// exclusive scan of u32 in place, returns the total (synchronous read-back)
+uint64_t wb_exclusive_scan_u32(wb_ctx *c, uint32_t *data, uint64_t n);
+// LSD radix sort of (u64 key, u32 val) pairs on bits [64-8*passes, 64)
+void wb_radix_sort_pairs(wb_ctx *c, uint64_t *key, uint32_t *val, uint64_t n, int passes);
+// LSD radix sort of (u32 key, u32 val) pairs on all 32 bits
+void wb_radix_sort_u32_pairs(wb_ctx *c, uint32_t *key, uint32_t *val, uint64_t n);
+
+// builds the canonical octree of n points (decoded from int32 LAS coordinates)
+void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32_t *dz);
+
+inline unsigned wb_blocks(uint64_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }

@@ -1,0 +1,522 @@
+// wb_math.h -- scalar FP64 geometry shared by the HIP kernels and by the
+// host (C++) side of libwb_gpu.  Every function reproduces, bit for bit, the
+// result the reference gets from glibc 2.35 + x86-64 (SSE2 doubles, x87 long
+// double), so it must be compiled with -ffp-contract=off and no fast-math.
+//
+//   hypot            glibc 2.35 e_hypot: Borges, "An Improved Algorithm for
+//                    hypot(a,b)", ACM TOMS 47(1), 2020 (arXiv:1904.09481),
+//                    unfused correction step, with power-of-two scaling for
+//                    huge/tiny operands.  Called by dist() (point.cpp:168-171)
+//                    and every Shape::in (shape.cpp:79-238).
+//   atan2i           angle.cpp:117-155 (binary angle, 2^31 = one turn).  The
+//                    last step is evaluated by the reference in x87 long
+//                    double; here a double-double evaluation decides the
+//                    rounding unless the value lies within 2^-25 of a
+//                    half-integer, in which case the three long-double
+//                    operations and lrintl are emulated exactly on 64-bit
+//                    significands (X80 below).
+//   shape predicates shape.cpp:38-238 with a margin-checked fast path that
+//                    skips hypot/sqrt only when the answer is certain.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define WB_HD __host__ __device__ inline
+#else
+#include <math.h>
+#define WB_HD inline
+#endif
+
+namespace wb {
+
+// ---------------------------------------------------------------------------
+// Binary angle constants (angle.h:102-119)
+// ---------------------------------------------------------------------------
+constexpr int DEG30 = 0x0aaaaaab;
+constexpr int DEG45 = 0x10000000;
+constexpr int DEG90 = 0x20000000;
+constexpr int DEG144 = 0x33333333;
+constexpr int DEG180 = 0x40000000;
+
+WB_HD double dabs(double x) { return __builtin_fabs(x); }
+WB_HD double dsqrt(double x) { return __builtin_sqrt(x); }
+WB_HD bool disfinite(double x) { return __builtin_isfinite(x); }
+WB_HD bool disnan(double x) { return __builtin_isnan(x); }
+WB_HD bool disinf(double x) { return __builtin_isinf(x); }
+
+// ---------------------------------------------------------------------------
+// hypot (glibc 2.35 result)
+// ---------------------------------------------------------------------------
+WB_HD double hypot_core(double ax, double ay)
+{
+  // ax >= ay >= 0, neither huge nor tiny.  Borges' correction: h = sqrt of
+  // the rounded sum, then h -= (h^2 - ax^2 - ay^2)/(2h) with the residual
+  // computed without FMA from one of two rearrangements.
+  double h = dsqrt(ax * ax + ay * ay);
+  double t1, t2;
+  if (h <= 2.0 * ay) {
+    double d = h - ay;
+    t1 = ax * (2.0 * d - ax);
+    t2 = (d - 2.0 * (ax - ay)) * d;
+  } else {
+    double d = h - ax;
+    t1 = 2.0 * d * (ax - 2.0 * ay);
+    t2 = (4.0 * d - ay) * ay + d * d;
+  }
+  h -= (t1 + t2) / (2.0 * h);
+  return h;
+}
+
+WB_HD double hypot(double x, double y)
+{
+  if (!disfinite(x) || !disfinite(y)) {
+    if (disinf(x) || disinf(y)) return __builtin_inf();
+    return x + y;  // NaN
+  }
+  x = dabs(x);
+  y = dabs(y);
+  double ax = x < y ? y : x, ay = x < y ? x : y;
+  const double big = 0x1p511, tiny = 0x1p-511, scale = 0x1p-600, eps = 0x1p-54;
+  if (ax > big) {
+    if (ay <= ax * eps) return ax + ay;
+    return hypot_core(ax * scale, ay * scale) / scale;
+  }
+  if (ay < tiny) {
+    if (ax >= ay / eps) return ax + ay;
+    return hypot_core(ax / scale, ay / scale) * scale;
+  }
+  if (ay <= ax * eps) return ax + ay;
+  return hypot_core(ax, ay);
+}
+
+// ---------------------------------------------------------------------------
+// 64x64 -> 128 multiply and small 128-bit helpers
+// ---------------------------------------------------------------------------
+struct U128 {
+  uint64_t hi, lo;
+};
+
+WB_HD U128 mul64(uint64_t a, uint64_t b)
+{
+  uint64_t a0 = a & 0xffffffffu, a1 = a >> 32, b0 = b & 0xffffffffu, b1 = b >> 32;
+  uint64_t p00 = a0 * b0, p01 = a0 * b1, p10 = a1 * b0, p11 = a1 * b1;
+  uint64_t mid = (p00 >> 32) + (p01 & 0xffffffffu) + (p10 & 0xffffffffu);
+  U128 r;
+  r.lo = (mid << 32) | (p00 & 0xffffffffu);
+  r.hi = p11 + (p01 >> 32) + (p10 >> 32) + (mid >> 32);
+  return r;
+}
+
+WB_HD int clz64(uint64_t v) { return v ? __builtin_clzll(v) : 64; }
+
+// ---------------------------------------------------------------------------
+// X80: exact emulation of x87 extended precision (64-bit significand,
+// round to nearest even) for the last step of atan2i.
+// value = (-1)^s * m * 2^e, m normalized (bit 63 set) unless zero.
+// ---------------------------------------------------------------------------
+struct X80 {
+  uint64_t m;
+  int e;
+  int s;
+};
+
+WB_HD X80 x80_from_double(double d)
+{
+  X80 r;
+  uint64_t b = __builtin_bit_cast(uint64_t, d);
+  r.s = (int)(b >> 63);
+  int ex = (int)((b >> 52) & 0x7ff);
+  uint64_t fr = b & 0xfffffffffffffull;
+  if (ex == 0) {
+    if (fr == 0) { r.m = 0; r.e = 0; return r; }
+    int lz = clz64(fr);
+    r.m = fr << lz;
+    r.e = -1074 - lz;
+    return r;
+  }
+  r.m = (fr | (1ull << 52)) << 11;
+  r.e = ex - 1075 - 11;
+  return r;
+}
+
+// Round mag (128-bit) * 2^e, with sticky bits below it, to 64 bits.
+WB_HD X80 x80_round(U128 mag, int e, int s, bool sticky)
+{
+  X80 r;
+  r.s = s;
+  if (mag.hi == 0 && mag.lo == 0) { r.m = 0; r.e = 0; return r; }
+  int lead = mag.hi ? 127 - clz64(mag.hi) : 63 - clz64(mag.lo);
+  int sh = lead - 63;  // bits to drop, in [-63, 64]
+  if (sh <= 0) {
+    // fits in 64 bits; any sticky part is below half an ulp
+    r.m = (sh == 0) ? mag.lo : (mag.lo << (-sh));
+    r.e = e + sh;
+    return r;
+  }
+  uint64_t m, below;
+  bool rbit;
+  if (sh == 64) {
+    m = mag.hi;
+    rbit = (mag.lo >> 63) & 1;
+    below = mag.lo & 0x7fffffffffffffffull;
+  } else {
+    m = (mag.hi << (64 - sh)) | (mag.lo >> sh);
+    rbit = (mag.lo >> (sh - 1)) & 1;
+    below = (sh == 1) ? 0 : (mag.lo & ((1ull << (sh - 1)) - 1));
+  }
+  int ee = e + sh;
+  if (rbit && (sticky || below != 0 || (m & 1))) {
+    m += 1;
+    if (m == 0) { m = 1ull << 63; ee += 1; }
+  }
+  r.m = m;
+  r.e = ee;
+  return r;
+}
+
+WB_HD X80 x80_mul(X80 a, X80 b)
+{
+  U128 p = mul64(a.m, b.m);
+  return x80_round(p, a.e + b.e, a.s ^ b.s, false);
+}
+
+// floor((hi:lo) / d) and remainder, for d >= 2^63 and hi < d (bitwise).
+WB_HD uint64_t div128(uint64_t hi, uint64_t lo, uint64_t d, uint64_t *rem)
+{
+  uint64_t q = 0, r = hi;
+  for (int i = 63; i >= 0; i--) {
+    uint64_t carry = r >> 63;
+    r = (r << 1) | ((lo >> i) & 1);
+    if (carry || r >= d) {
+      r -= d;
+      q |= 1ull << i;
+    }
+  }
+  *rem = r;
+  return q;
+}
+
+WB_HD X80 x80_div(X80 a, X80 b)
+{
+  // q = floor(a.m * 2^64 / b.m) in [2^63, 2^65); two more bits from rem.
+  X80 z;
+  if (a.m == 0) { z.m = 0; z.e = 0; z.s = a.s ^ b.s; return z; }
+  uint64_t qhi = (a.m >= b.m) ? 1 : 0;
+  uint64_t h = a.m - (qhi ? b.m : 0);
+  uint64_t rem;
+  uint64_t qlo = div128(h, 0, b.m, &rem);
+  // extend by two quotient bits
+  uint64_t ext = 0;
+  for (int k = 0; k < 2; k++) {
+    uint64_t carry = rem >> 63;
+    rem <<= 1;
+    ext <<= 1;
+    if (carry || rem >= b.m) { rem -= b.m; ext |= 1; }
+  }
+  U128 mag;
+  mag.hi = (qhi << 2) | (qlo >> 62);
+  mag.lo = (qlo << 2) | ext;
+  return x80_round(mag, a.e - b.e - 64 - 2, a.s ^ b.s, rem != 0);
+}
+
+WB_HD X80 x80_add(X80 a, X80 b)
+{
+  if (a.m == 0) return b;
+  if (b.m == 0) return a;
+  // order by magnitude (both normalized: exponent, then significand)
+  if ((b.e > a.e) || (b.e == a.e && b.m > a.m)) { X80 t = a; a = b; b = t; }
+  int d = a.e - b.e;
+  // 128-bit window: a.m at bits 127..64 (unit 2^(a.e-64)); b shifted in
+  uint64_t shi, slo;
+  bool sticky = false;
+  if (d == 0) { shi = b.m; slo = 0; }
+  else if (d < 64) { shi = b.m >> d; slo = b.m << (64 - d); }
+  else if (d == 64) { shi = 0; slo = b.m; }
+  else if (d < 128) { shi = 0; slo = b.m >> (d - 64); sticky = (b.m << (128 - d)) != 0; }
+  else { shi = 0; slo = 0; sticky = true; }
+  U128 r;
+  int e = a.e - 64;
+  if (a.s == b.s) {
+    r.lo = slo;
+    r.hi = a.m + shi;
+    if (r.hi < a.m) {  // carry out of bit 127
+      sticky = sticky || (r.lo & 1);
+      r.lo = (r.lo >> 1) | (r.hi << 63);
+      r.hi = (r.hi >> 1) | (1ull << 63);
+      e += 1;
+    }
+    return x80_round(r, e, a.s, sticky);
+  }
+  // a.m*2^64 - (shi:slo) - (sticky ? something in (0,1) : 0)
+  r.lo = 0 - slo;
+  r.hi = a.m - shi - (slo != 0 ? 1 : 0);
+  if (sticky) {
+    if (r.lo == 0) r.hi -= 1;
+    r.lo -= 1;
+  }
+  return x80_round(r, e, a.s, sticky);
+}
+
+WB_HD X80 x80_neg(X80 a) { a.s ^= 1; return a; }
+
+// lrintl (round to nearest, ties to even) for |value| < 2^62
+WB_HD int64_t x80_lrint(X80 a)
+{
+  if (a.m == 0) return 0;
+  uint64_t ip;
+  if (a.e >= 0) {
+    ip = a.m << a.e;
+  } else {
+    int sh = -a.e;
+    if (sh > 64) return 0;
+    uint64_t frac, half;
+    if (sh == 64) { ip = 0; frac = a.m; half = 1ull << 63; }
+    else { ip = a.m >> sh; frac = a.m & ((1ull << sh) - 1); half = 1ull << (sh - 1); }
+    if (frac > half || (frac == half && (ip & 1))) ip += 1;
+  }
+  int64_t v = (int64_t)ip;
+  return a.s ? -v : v;
+}
+
+// ---------------------------------------------------------------------------
+// Double-double helpers (FMA is exact here by construction)
+// ---------------------------------------------------------------------------
+WB_HD void two_prod(double a, double b, double &p, double &e)
+{
+  p = a * b;
+  e = __builtin_fma(a, b, -p);
+}
+WB_HD void two_sum(double a, double b, double &s, double &e)
+{
+  s = a + b;
+  double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// Constants of the last atan2i step: K = 0x40000000/M_PIl as a long double
+// (64-bit significand), split exactly into two doubles, and the cubic
+// coefficient of angle.cpp:151.
+struct AtanConst {
+  double k_hi, k_lo;     // K = k_hi + k_lo exactly
+  uint64_t k_m;          // K = k_m * 2^k_e (x87 value)
+  int k_e;
+};
+constexpr double ATAN_C3 = 1.1392738508503886e8;
+
+// Exact x87 evaluation of lrintl(K*y/x - c3) (angle.cpp:151).
+WB_HD int64_t atan_last_exact(double y, double x, double c3, const AtanConst &K)
+{
+  X80 k;
+  k.m = K.k_m; k.e = K.k_e; k.s = 0;
+  X80 p = x80_mul(k, x80_from_double(y));
+  X80 q = x80_div(p, x80_from_double(x));
+  X80 r = x80_add(q, x80_neg(x80_from_double(c3)));
+  return x80_lrint(r);
+}
+
+// atan2i(y, x), angle.cpp:117-155.  tab_tan[511], tab_cos[512], tab_sin[512]
+// are the host-built tables of fillTanTables (angle.cpp:305-320).
+template <typename TabPtr>
+WB_HD int atan2i(double y, double x, TabPtr tab_tan, TabPtr tab_cos, TabPtr tab_sin,
+                 const AtanConst &K, int *exact_used = nullptr, bool force_exact = false)
+{
+  int a = 0;
+  double t;
+  if (x < 0) {
+    a += (y > 0) ? DEG180 : -DEG180;
+    y = -y;
+    x = -x;
+  }
+  if (y > x) {
+    a += DEG90;
+    t = x; x = y; y = -t;
+  }
+  if (-y > x) {
+    a -= DEG90;
+    t = x; x = -y; y = t;
+  }
+  t = y / x;
+#pragma unroll
+  for (int step = DEG45 / 2; step > DEG45 / 1024; step /= 2) {
+    int k = (((a + DEG45) & 0x1ff00000) >> 20) - 1;
+    a += (t > tab_tan[k]) ? step : -step;
+  }
+  {
+    int k = 511 - (((a + DEG45) & 0x1ff00000) >> 20);
+    double c = tab_cos[k], s = tab_sin[k];
+    t = x * c - y * s;
+    y = y * c + x * s;
+    x = t;
+  }
+  if (x == 0 && y == 0) return 0;
+  double u = y / x;
+  double c3 = ATAN_C3 * (u * u * u);
+  // double-double value of K*y/x - c3
+  double p1, p2;
+  two_prod(K.k_hi, y, p1, p2);
+  p2 += K.k_lo * y;
+  double q1 = p1 / x;
+  double rr = __builtin_fma(-q1, x, p1) + p2;
+  double q2 = rr / x;
+  double v_hi, v_lo;
+  two_sum(q1, -c3, v_hi, v_lo);
+  v_lo += q2;
+  double n = __builtin_rint(v_hi);
+  double f = (v_hi - n) + v_lo;  // v_hi - n is exact
+  double af = dabs(f);
+  int64_t r;
+  if (!force_exact && af < 0.5 - 0x1p-25) {
+    r = (int64_t)n;
+  } else if (!force_exact && af > 0.5 + 0x1p-25) {
+    r = (int64_t)n + (f > 0 ? 1 : -1);
+  } else {
+    r = atan_last_exact(y, x, c3, K);
+    if (exact_used) *exact_used += 1;
+  }
+  return (int)((int64_t)a + r);
+}
+
+// ---------------------------------------------------------------------------
+// Shapes (shape.cpp).  Kinds follow the C ABI: 0 cylinder, 1 sphere,
+// 2 hyperboloid.
+// ---------------------------------------------------------------------------
+enum { SHAPE_CYLINDER = 0, SHAPE_SPHERE = 1, SHAPE_HYPERBOLOID = 2 };
+
+struct Shape {
+  int kind;
+  double cx, cy, cz;  // centre (hyperboloid: vertex + (0,0,por))
+  double r;           // cylinder / sphere radius
+  double por2, slope; // hyperboloid
+};
+
+// Hyperboloid(v, r, s), shape.cpp:119-125
+WB_HD Shape make_hyperboloid(double vx, double vy, double vz, double r, double s)
+{
+  Shape h;
+  double por = r * (s * s);
+  h.kind = SHAPE_HYPERBOLOID;
+  h.slope = s;
+  h.por2 = por * por;
+  h.cx = vx + 0;
+  h.cy = vy + 0;
+  h.cz = vz + por;
+  h.r = 0;
+  return h;
+}
+WB_HD Shape make_cylinder(double cx, double cy, double r)
+{
+  Shape c;
+  c.kind = SHAPE_CYLINDER; c.cx = cx; c.cy = cy; c.cz = 0; c.r = r; c.por2 = 0; c.slope = 0;
+  return c;
+}
+WB_HD Shape make_sphere(double cx, double cy, double cz, double r)
+{
+  Shape c;
+  c.kind = SHAPE_SPHERE; c.cx = cx; c.cy = cy; c.cz = cz; c.r = r; c.por2 = 0; c.slope = 0;
+  return c;
+}
+
+// Cylinder::in, shape.cpp:214-218: hypot(c - q) <= r.  The fast path decides
+// from dx^2+dy^2 when it is more than 2^-46 (relative) away from r^2, where
+// hypot's rounding cannot change the answer.
+WB_HD bool cyl_in(const Shape &s, double qx, double qy)
+{
+  double dx = s.cx - qx, dy = s.cy - qy;
+  double q2 = dx * dx + dy * dy, r2 = s.r * s.r;
+  double m = 0x1p-46 * r2;
+  if (q2 < r2 - m) return true;
+  if (q2 > r2 + m) return false;
+  return hypot(dx, dy) <= s.r;
+}
+
+// Sphere::in, shape.cpp:172-175: hypot(hypot(dx,dy),dz) <= r.
+WB_HD bool sph_in(const Shape &s, double qx, double qy, double qz)
+{
+  double dx = qx - s.cx, dy = qy - s.cy, dz = qz - s.cz;
+  double q2 = dx * dx + dy * dy + dz * dz, r2 = s.r * s.r;
+  double m = 0x1p-45 * r2;
+  if (q2 < r2 - m) return true;
+  if (q2 > r2 + m) return false;
+  return hypot(hypot(dx, dy), dz) <= s.r;
+}
+
+// Hyperboloid::in, shape.cpp:127-135.
+WB_HD bool hyp_in(const Shape &s, double qx, double qy, double qz)
+{
+  double zd = s.cz - qz;
+  if (s.slope > 0) {
+    if (!(zd > 0)) return false;
+  } else {
+    if (!(zd < 0)) return false;
+  }
+  double dx = s.cx - qx, dy = s.cy - qy;
+  double a = zd * zd;
+  double b = (dx * dx + dy * dy) * (s.slope * s.slope);
+  double v = a - b;
+  double m = 0x1p-44 * (a + b);
+  if (v - s.por2 > m) return true;
+  if (v - s.por2 < -m) return false;
+  double xyd = hypot(dx, dy);
+  double w = xyd * s.slope;
+  return a - w * w >= s.por2;
+}
+
+WB_HD bool shape_in(const Shape &s, double qx, double qy, double qz)
+{
+  switch (s.kind) {
+  case SHAPE_CYLINDER: return cyl_in(s, qx, qy);
+  case SHAPE_SPHERE: return sph_in(s, qx, qy, qz);
+  default: return hyp_in(s, qx, qy, qz);
+  }
+}
+
+// Shape::in(Cube&): all eight corners (shape.cpp:45-61)
+WB_HD bool shape_in_cube(const Shape &s, double cx, double cy, double cz, double side)
+{
+  double h = side / 2;
+  double x0 = cx - h, x1 = cx + h, y0 = cy - h, y1 = cy + h, z0 = cz - h, z1 = cz + h;
+  for (int n = 0; n < 8; n++) {
+    double qx = (n & 1) ? x1 : x0, qy = (n & 2) ? y1 : y0, qz = (n & 4) ? z1 : z0;
+    if (!shape_in(s, qx, qy, qz)) return false;
+  }
+  return true;
+}
+
+WB_HD double clamp_axis(double sc, double cc, double side)
+{
+  if (dabs(sc - cc) < side / 2) return sc;
+  if (sc > cc) return cc + side / 2;
+  return cc - side / 2;
+}
+
+// Shape::intersect(cube) = in(closestPoint(cube)), shape.cpp:63-66, 137-238
+WB_HD bool shape_intersect(const Shape &s, double cx, double cy, double cz, double side)
+{
+  double qx = clamp_axis(s.cx, cx, side), qy = clamp_axis(s.cy, cy, side), qz = cz;
+  if (s.kind == SHAPE_SPHERE) qz = clamp_axis(s.cz, cz, side);
+  else if (s.kind == SHAPE_HYPERBOLOID) {
+    if (s.slope > 0) qz -= side / 2;
+    if (s.slope < 0) qz += side / 2;
+  }
+  return shape_in(s, qx, qy, qz);
+}
+
+// Conservative box test for pruning: false only if no point of the box
+// [lo,hi] can satisfy shape_in.  Clamping each axis to the box gives, per
+// axis, a difference no larger in magnitude than any point of the box, and
+// every predicate above is monotone in those magnitudes (and in z for the
+// hyperboloid), so a false here implies false for every point in the box.
+WB_HD bool shape_may_hit_box(const Shape &s, double x0, double x1, double y0, double y1,
+                             double z0, double z1)
+{
+  double qx = s.cx < x0 ? x0 : (s.cx > x1 ? x1 : s.cx);
+  double qy = s.cy < y0 ? y0 : (s.cy > y1 ? y1 : s.cy);
+  double qz;
+  if (s.kind == SHAPE_SPHERE) qz = s.cz < z0 ? z0 : (s.cz > z1 ? z1 : s.cz);
+  else if (s.kind == SHAPE_HYPERBOLOID) qz = s.slope > 0 ? z0 : z1;
+  else qz = z0;
+  return shape_in(s, qx, qy, qz);
+}
+
+}  // namespace wb
