@@ -60,13 +60,24 @@ def test_tree_small_sizes(ctx):
         for key in ("xi", "yi", "zi"):
             cloud[key] = cloud[key][:k]
         x, y, z = decode(cloud)
-        cloud["bbmin"] = np.array([x.min(), y.min(), z.min()])
-        cloud["bbmax"] = np.array([x.max(), y.max(), z.max()])
+        pad = 1.0 if k == 1 else 0.0   # a single point gives sizeFit side 0
+        cloud["bbmin"] = np.array([x.min(), y.min(), z.min()]) - pad
+        cloud["bbmax"] = np.array([x.max(), y.max(), z.max()]) + pad
         tree, ref, _ = build_both(ctx, cloud)
         g, r = tree.leaves(), ref.leaves()
         assert np.array_equal(g["count"], r["count"]), n
         assert np.array_equal(g["center"], r["center"]), n
         assert np.array_equal(tree.order().astype(np.int64), ref.order()), n
+
+
+def test_degenerate_root_rejected(ctx):
+    # Octree::sizeFit gives side 0 for a single location; the reference then
+    # asserts side>0 on the first findBlock (octree.cpp:208).  The ABI
+    # returns WB_EINVAL instead.
+    c, s = W.Octree.size_fit([1.0, 2.0, 3.0], [1.0, 2.0, 3.0])
+    assert s == 0.0
+    with pytest.raises(W.WbError, match="root side"):
+        W.Octree.build(ctx, [1000], [2000], [3000], [0.001] * 3, [0.0] * 3, c, s)
 
 
 def random_shapes(rng, x, y, z, kind, m):
