@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark: classified points/s on the 20M-point tile, one MI355X.
+
+A step is one pass of the hot path over the tile: octree build from the
+int32 LAS coordinates already resident in HBM (Morton keys, radix sort,
+canonical split, SoA permute) followed by SCAN, POSTSCAN and CLASSIFY of
+every flowsnake tile, ending with the class bytes in HBM.
+
+Prints one JSON line (rank 0).  --gpus above 1 is refused: the build runs on
+one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("classified points/s on 20M-pt tile (1 GPU, %HBM roofline); p50/p99 tile latency")
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "tile20m": ("tile20m: 900 m x 900 m, 25 pulses/m^2 (~20.3M points), seed 2000, "
+                "6-sinusoid ground + 5 embankments (20-45 deg) + 2% outliers to +-50 m, "
+                "box buildings and cone trees (1-3 returns), int32 at 0.001 m"),
+    "tile1m": ("tile1m: 200 m x 200 m, 25 pulses/m^2 (~1.2M points), seed 1000, "
+               "6-sinusoid ground, 40 box buildings, 300 cone trees, int32 at 0.001 m"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def percentile(v, q):
+    v = sorted(v)
+    if not v:
+        return None
+    k = (len(v) - 1) * q / 100.0
+    f = math.floor(k)
+    c = min(f + 1, len(v) - 1)
+    return v[f] + (v[c] - v[f]) * (k - f)
+
+
+def cpu_baseline(cloud, window):
+    """The oracle (single-threaded C restatement of the reference pass) on the
+    points of a window x window m square at the tile centre."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    x = cloud["offset"][0] + cloud["scale"][0] * cloud["xi"].astype(np.float64)
+    y = cloud["offset"][1] + cloud["scale"][1] * cloud["yi"].astype(np.float64)
+    cx = 0.5 * (cloud["bbmin"][0] + cloud["bbmax"][0])
+    cy = 0.5 * (cloud["bbmin"][1] + cloud["bbmax"][1])
+    m = (np.abs(x - cx) < window / 2) & (np.abs(y - cy) < window / 2)
+    sub = {k: cloud[k][m] for k in ("xi", "yi", "zi")}
+    for k in ("scale", "offset"):
+        sub[k] = cloud[k]
+    z = cloud["offset"][2] + cloud["scale"][2] * sub["zi"].astype(np.float64)
+    sub["bbmin"] = np.array([x[m].min(), y[m].min(), z.min()])
+    sub["bbmax"] = np.array([x[m].max(), y[m].max(), z.max()])
+    n = int(m.sum())
+    t0 = time.perf_counter()
+    O.run(sub)
+    dt = time.perf_counter() - t0
+    return dict(value=n / dt, unit="points/s", cores=1, kind="port",
+                sample=f"oracle full pass (build, scan, postscan, classify) over the {n} points "
+                       f"of a {window:g} m x {window:g} m window at the tile centre, "
+                       f"{dt:.1f} s single-threaded")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="tile20m", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-window", type=float, default=50.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        if int(os.environ.get("RANK", "0")) == 0:
+            print("bench.py runs on one GPU only: the classification of one tile is not "
+                  "sharded across GPUs in this build (see DESIGN.md, 'Sharding').", flush=True)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import wolkenbase_amd as W
+    from wolkenbase_amd import synth
+
+    t0 = time.perf_counter()
+    cloud = synth.preset(args.workload)
+    n = len(cloud["xi"])
+    log(f"[bench] generated {args.workload}: {n} points in {time.perf_counter() - t0:.1f} s")
+
+    dev = torch.device("cuda:0")
+    xi = torch.from_numpy(cloud["xi"]).to(dev)
+    yi = torch.from_numpy(cloud["yi"]).to(dev)
+    zi = torch.from_numpy(cloud["zi"]).to(dev)
+    cls = torch.empty(n, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    ctx = W.Context(0)
+    center, side = W.Octree.size_fit(cloud["bbmin"], cloud["bbmax"])
+    params = W.default_params()
+
+    def step():
+        tree = W.Octree.build(ctx, xi, yi, zi, cloud["scale"], cloud["offset"], center, side)
+        _, _, st = W.classify(tree, cloud["bbmin"], cloud["bbmax"], params, want_tiles=False,
+                              out=cls)
+        tree.free()
+        return st
+
+    for w in range(args.warmup):
+        st = step()
+        log(f"[bench] warmup {w + 1}/{args.warmup}: build {st['ms_build']:.2f} ms, "
+            f"classify kernel {st['ms_classify_kernel']:.2f} ms")
+    torch.cuda.synchronize()
+    ctx.sync()
+    lat, stats = [], []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        st = step()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+        stats.append(st)
+        log(f"[bench] step {k + 1}/{args.steps}: {lat[-1]:.2f} ms")
+    ctx.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    value = n * args.steps / elapsed
+
+    def avg(f):
+        return sum(s[f] for s in stats) / len(stats)
+
+    st = stats[-1]
+    kern = {
+        "k_classify": avg("ms_classify_kernel"),
+        "radix_sort": avg("ms_sort"),
+    }
+    dominant = max(kern, key=kern.get)
+    passes = int(st["sort_passes"])
+    # algorithmic bytes per launch (SURVEY 8(d) decomposition): the ground
+    # test reads each point once (12 B as int32) and writes its class byte;
+    # the sort moves (u64 key, u32 index) records, 24 B per point per pass,
+    # plus the 8-B key read of each digit histogram.
+    alg = {"k_classify": 13.0 * n, "radix_sort": (24.0 + 8.0) * passes * n}
+    achieved = alg[dominant] / (kern[dominant] * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dominant}_{args.workload}.txt")
+    if os.path.exists(tfile):
+        for line in open(tfile):
+            if line.startswith("hbm_bytes_per_launch"):
+                traffic = float(line.split("=")[1])
+    roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 6), traffic=traffic, kernel=dominant,
+                kernel_ms=round(kern[dominant], 3),
+                algorithmic_bytes_per_launch=alg[dominant])
+    D = int(st["maxdepth"])
+    P = math.ceil(3 * D / 8)
+    B = 12 + 12 + 8 + 24 * P + 24 + 12 * 1.2097 + 12 + 1
+    pipeline_frac = B * value / (HBM_PEAK_GBS * 1e9)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS[args.workload], "n_points": n,
+                   "tile_size_m": 1.0, "max_slope": 1.0, "thickness": 0.0, "min_hs_m": 0.1,
+                   "input": "int32 x/y/z resident in HBM; class bytes left in HBM"},
+        "p50_ms": round(percentile(lat, 50), 3),
+        "p99_ms": round(percentile(lat, 99), 3),
+        "roofline": roof,
+        "pipeline_roofline": {"bytes_per_point": B, "maxdepth": D, "passes_formula": P,
+                              "frac_of_8TBs": round(pipeline_frac, 6)},
+        "breakdown_ms": {k: round(avg(k), 3) for k in
+                         ("ms_build", "ms_sort", "ms_tiles", "ms_scan", "ms_classify",
+                          "ms_classify_kernel")},
+        "classes": {"ground": int(st["n_class2"]), "not_ground": int(st["n_class1"]),
+                    "unclassified": int(st["n_unclassified"])},
+        "exact_paths": {"gap_points": int(st["exact_gap_points"]),
+                        "heavy_points": int(st["heavy_points"]),
+                        "atan_x87_steps": int(st["atan_exact_calls"]),
+                        "sector_borderline": int(st["sector_borderline"])},
+    }
+    if not args.no_cpu_baseline:
+        log(f"[bench] cpu baseline on a {args.cpu_window:g} m window ...")
+        out["cpu_baseline"] = cpu_baseline(cloud, args.cpu_window)
+    else:
+        out["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

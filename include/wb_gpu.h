@@ -85,9 +85,16 @@ typedef struct {
   uint64_t sector_borderline; /* scan sector angles within 1e-12 of a boundary */
   uint64_t exact_gap_points;  /* points decided by the exact (slow) gap pass */
   uint64_t atan_exact_calls;  /* atan2i last steps emulated in 64-bit significands */
+  uint64_t heavy_points;      /* points finished by the wave-per-point kernel */
   uint32_t sort_passes, maxdepth;
   uint64_t nleaves, nnodes;
-  double ms_build, ms_tiles, ms_scan, ms_classify, ms_total;
+  double ms_build;            /* last wb_tree_build of this tree (device time) */
+  double ms_sort;             /* of which the radix sort */
+  double ms_tiles;            /* tile geometry, membership, tile lists */
+  double ms_scan;             /* scan + postscan */
+  double ms_classify;         /* ground test, exact pass, class scatter */
+  double ms_classify_kernel;  /* the ground-test kernel alone */
+  double ms_total;            /* host wall time of wb_classify */
 } wb_stats;
 
 /* --- context ------------------------------------------------------------ */

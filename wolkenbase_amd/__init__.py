@@ -52,10 +52,12 @@ class Stats(C.Structure):
                 ("n_unclassified", C.c_uint64), ("ntiles", C.c_uint64),
                 ("nonempty_tiles", C.c_uint64), ("sector_borderline", C.c_uint64),
                 ("exact_gap_points", C.c_uint64), ("atan_exact_calls", C.c_uint64),
+                ("heavy_points", C.c_uint64),
                 ("sort_passes", C.c_uint32), ("maxdepth", C.c_uint32),
                 ("nleaves", C.c_uint64), ("nnodes", C.c_uint64),
-                ("ms_build", C.c_double), ("ms_tiles", C.c_double), ("ms_scan", C.c_double),
-                ("ms_classify", C.c_double), ("ms_total", C.c_double)]
+                ("ms_build", C.c_double), ("ms_sort", C.c_double), ("ms_tiles", C.c_double),
+                ("ms_scan", C.c_double), ("ms_classify", C.c_double),
+                ("ms_classify_kernel", C.c_double), ("ms_total", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

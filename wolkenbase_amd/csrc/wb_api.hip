@@ -144,6 +144,7 @@ int wb_tree_build(wb_ctx *c, const int32_t *x, const int32_t *y, const int32_t *
   }
   t->side = root_side;
   t->sort_passes = 0;
+  t->ms_build = t->ms_sort = 0;
   if (xf->unit != 1.0) {
     delete t;
     return fail(c, "only unit == 1 is supported (metres)", WB_ERANGE);
@@ -337,7 +338,7 @@ int wb_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   bool host_out = !(flags & WB_DEVICE_INPUT);
   if (host_out) dcls = (uint8_t *)wb_alloc(c, t->n + 16);
   ClassifyOut co;
-  wb_run_classify(t, bbmin, bbmax, p, dcls, tiles_out != nullptr || stats != nullptr, &co);
+  wb_run_classify(t, bbmin, bbmax, p, dcls, tiles_out != nullptr, &co);
   if (host_out) {
     if (t->n) WB_CHECK(hipMemcpyAsync(class_out, dcls, t->n, hipMemcpyDeviceToHost, c->stream));
     WB_CHECK(hipStreamSynchronize(c->stream));
@@ -365,16 +366,18 @@ int wb_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
     stats->n_class1 = co.hist[1];
     stats->n_class2 = co.hist[2];
     stats->ntiles = ntiles;
-    uint64_t ne = 0;
-    for (auto &r : co.tiles) ne += r.npoints != 0;
-    stats->nonempty_tiles = ne;
+    stats->nonempty_tiles = co.nonempty;
     stats->sector_borderline = co.borderline;
     stats->exact_gap_points = co.nslow;
     stats->atan_exact_calls = co.atan_exact;
+    stats->heavy_points = co.nheavy;
     stats->sort_passes = t->sort_passes;
     stats->maxdepth = t->maxdepth;
     stats->nleaves = t->nleaves;
     stats->nnodes = t->nnodes;
+    stats->ms_build = t->ms_build;
+    stats->ms_sort = t->ms_sort;
+    stats->ms_classify_kernel = co.ms_classify_kernel;
     stats->ms_tiles = co.ms_tiles;
     stats->ms_scan = co.ms_scan;
     stats->ms_classify = co.ms_classify;

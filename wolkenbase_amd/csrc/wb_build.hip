@@ -548,6 +548,8 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
 
   t->key = wb_talloc<uint64_t>(t, n);
   t->sidx = wb_talloc<uint32_t>(t, n);
+  hipEvent_t evb[4];
+  for (int k = 0; k < 4; k++) WB_CHECK(hipEventCreate(&evb[k]));
   int passes = t->sort_passes > 0 ? t->sort_passes : default_passes(t->side);
   const uint32_t node_cap = (uint32_t)std::min<uint64_t>(n / 64 + 1024, 0x7fffffff);
   t->node_cube = wb_talloc<double>(t, 4 * (size_t)node_cap);
@@ -561,9 +563,12 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
   Active *nxt = (Active *)wb_alloc(c, sizeof(Active) * node_cap);
   for (;;) {
     if (n > 0) {
+      WB_CHECK(hipEventRecord(evb[0], c->stream));
       hipLaunchKernelGGL(k_keygen, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, dx, dy, dz,
                          n, f, t->key, t->sidx);
+      WB_CHECK(hipEventRecord(evb[1], c->stream));
       wb_radix_sort_pairs(c, t->key, t->sidx, n, passes);
+      WB_CHECK(hipEventRecord(evb[2], c->stream));
     }
     // root node 0
     double root[4] = {t->center[0], t->center[1], t->center[2], t->side};
@@ -652,5 +657,13 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
                        c->stream, t->px, t->py, t->pz, n, t->chunk_box);
     hipLaunchKernelGGL(k_leaf_canon, dim3(nl), dim3(256), 0, c->stream, t->leaf_start, nl,
                        t->sidx, t->canon, t->crank);
+    WB_CHECK(hipEventRecord(evb[3], c->stream));
+    WB_CHECK(hipEventSynchronize(evb[3]));
+    float a = 0, b = 0;
+    hipEventElapsedTime(&a, evb[0], evb[3]);
+    hipEventElapsedTime(&b, evb[1], evb[2]);
+    t->ms_build = a;
+    t->ms_sort = b;
   }
+  for (int k = 0; k < 4; k++) hipEventDestroy(evb[k]);
 }

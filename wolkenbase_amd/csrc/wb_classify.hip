@@ -26,6 +26,9 @@
 //    exact: clamping the shape centre into the box makes each coordinate
 //    difference no larger than for any point inside, and every predicate is
 //    monotone in those differences.
+#include <stdio.h>
+#include <stdlib.h>
+
 #include "wb_flowsnake.h"
 #include "wb_internal.h"
 #include "../../include/wb_gpu.h"
@@ -394,7 +397,7 @@ __global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_
                                              const int32_t *tey, const uint32_t *tile_off,
                                              const uint32_t *tile_cnt, const uint32_t *lists,
                                              double min_hs, TileRec *rec, uint32_t *err,
-                                             unsigned long long *borderline)
+                                             unsigned long long *ctr)
 {
   __shared__ uint32_t cp[SCAN_CAP];
   __shared__ double vx[SCAN_CAP], vy[SCAN_CAP], vz[SCAN_CAP];
@@ -493,7 +496,7 @@ __global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_
     double th = atan2(vy[j], vx[j]) * 3 / M_PI;
     long sec = lrint(th);
     double fr = th - floor(th);
-    if (wb::dabs(fr - 0.5) < 1e-12) atomicAdd(borderline, 1ull);
+    if (wb::dabs(fr - 0.5) < 1e-12) atomicAdd(&ctr[0], 1ull);
     if (sec < 0) sec += 6;
     sec = (sec % 6) + 1;
     if (wb::hypot(vx[j], vy[j]) < r / kSqrt7) sec = 0;
@@ -519,6 +522,7 @@ __global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_
     rr.hs = rr.hs_scan;
     rr.height = top - bottom;
     rec[t] = rr;
+    atomicAdd(&ctr[5], 1ull);  // non-empty tiles
   }
 }
 
@@ -564,9 +568,11 @@ struct ClassifyArgs {
   const double *tab;     // tan[511], cos[512], sin[512] packed
   wb::AtanConst K;
   uint8_t *cls;          // per sorted slot
-  uint32_t *slow;        // slots needing the exact gap pass
-  uint32_t *nslow;
+  uint32_t *nslow;       // points decided by the exact gap pass
+  uint32_t *heavy;       // points handed to the wave-per-point kernel
+  uint32_t *nheavy;
   unsigned long long *atan_exact;
+  unsigned long long *counters;  // 16 work counters (only with the counting kernel)
 };
 
 __device__ inline uint64_t rotr64(uint64_t x, int a) { return (x >> a) | (x << (64 - a)); }
@@ -588,125 +594,232 @@ __device__ inline int gap_verdict(uint64_t occupied)
 
 __device__ inline int dir_bin_pos(int a) { return (int)(((uint32_t)a + 0x40000000u) & 0x7fffffffu); }
 
-template <class OnDir>
-__device__ inline void enumerate_down(const ClassifyArgs &A, uint32_t i, const double *tabs,
-                                      OnDir &&on_dir)
+// ---------------------------------------------------------------------------
+// Wave-cooperative findBlocks walk.  The 64 lanes of a wave hold 64
+// consecutive points of the sorted order (spatial neighbours), each with its
+// own hyperboloid.  The walk itself is uniform: a node is entered when any
+// active lane's hyperboloid intersects it.  Each lane keeps a bit per depth
+// saying whether every ancestor so far intersected *its* hyperboloid, so a
+// lane processes exactly the leaves the reference's findBlocks returns for
+// its own shape.  Leaf points are read with wave-uniform addresses.
+//   SAMPLE: only the first point of each 32-point group is considered (a
+//   subset of D: a pass on it is a pass on D).
+// on_dir(dy, dx) is called for every q in pointsIn(H) with xy(q) != xy(p);
+// after_leaf() returns the lane's "still active" flag.
+// ---------------------------------------------------------------------------
+// per-lane work counters: leaves entered, groups tested, groups scanned,
+// points tested, and walk steps (uniform)
+struct WalkCount {
+  uint32_t leaves, groups, groups_scanned, points, steps;
+};
+
+template <bool SAMPLE, bool COUNT, class OnDir, class AfterLeaf>
+__device__ inline void wave_walk(const TreeView &T, const wb::Shape &H, double x, double y,
+                                 bool active, OnDir &&on_dir, AfterLeaf &&after_leaf,
+                                 WalkCount &wc)
 {
-  const TreeView &T = A.T;
-  const double x = T.px[i], y = T.py[i], z = T.pz[i];
-  int w = INT32_MIN;
-  uint32_t nm = A.nmemb[i];
-  for (uint32_t k = 0; k < nm; k++) w = max(w, A.memb[MAX_MEMB * i + k]);
-  const double hs = A.rec[w - A.g.lo].hs;
-  wb::Shape H = wb::make_hyperboloid(x - 0, y - 0, z - A.thickness, hs, A.max_slope);
-  for_leaves(T, H, [&](int leaf) {
-    const double *q = T.leaf_cube + 4 * leaf;
-    const bool all = wb::shape_in_cube(H, q[0], q[1], q[2], q[3]);
-    const uint32_t b = T.leaf_start[leaf], e = T.leaf_start[leaf + 1];
-    for (uint32_t c0 = b; c0 < e;) {
-      uint32_t ch = c0 / WB_CHUNK;
-      uint32_t ce = min(e, (ch + 1) * WB_CHUNK);
-      if (!all) {
-        const double *bx = T.chunk_box + 6 * (uint64_t)ch;
-        if (!wb::shape_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5])) {
-          c0 = ce;
-          continue;
-        }
+  if (__ballot(active) == 0 || T.nleaves == 0) return;
+  uint32_t okm = (active && wb::shape_intersect(H, T.node_cube[0], T.node_cube[1],
+                                                 T.node_cube[2], T.node_cube[3])) ? 1u : 0u;
+  if (__ballot(okm != 0) == 0) return;
+  int node = 0, depth = 0;
+  Iter it = {0, 0};
+  for (;;) {
+    if (COUNT) wc.steps++;
+    int c = it.get(depth);
+    if (c == 8) {
+      if (depth == 0) return;
+      node = T.node_parent[node];
+      okm &= ~(1u << depth);
+      depth--;
+      continue;
+    }
+    it.set(depth, c + 1);
+    const bool on = active && ((okm >> depth) & 1u);
+    if (__ballot(on) == 0) {  // nobody below this node any more
+      it.set(depth, 8);
+      continue;
+    }
+    int ch = T.node_child[8 * node + c];
+    if (ch == -1) continue;
+    if (ch >= 0) {
+      const double *q = T.node_cube + 4 * ch;
+      bool h = on && wb::shape_intersect(H, q[0], q[1], q[2], q[3]);
+      if (__ballot(h)) {
+        node = ch;
+        depth++;
+        if (h) okm |= 1u << depth;
+        it.set(depth, 0);
       }
-      for (uint32_t j = c0; j < ce; j++) {
-        double qx = T.px[j], qy = T.py[j];
-        if (!all && !wb::hyp_in(H, qx, qy, T.pz[j])) continue;
-        if (qx == x && qy == y) continue;  // dist(xy(p), xy(q)) == 0
-        if (on_dir(qy - y, qx - x)) return true;
+      continue;
+    }
+    const int leaf = -2 - ch;
+    const double *lc = T.leaf_cube + 4 * leaf;
+    const bool hit = on && wb::shape_intersect(H, lc[0], lc[1], lc[2], lc[3]);
+    if (__ballot(hit) == 0) continue;
+    const bool all = hit && wb::shape_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
+    const uint32_t b = T.leaf_start[leaf], e = T.leaf_start[leaf + 1];
+    if (COUNT && hit) wc.leaves++;
+    for (uint32_t c0 = b; c0 < e;) {
+      const uint32_t chk = c0 / WB_CHUNK;
+      const uint32_t ce = min(e, (chk + 1) * WB_CHUNK);
+      bool chit = hit;
+      if (chit && !all) {
+        const double *bx = T.chunk_box + 6 * (uint64_t)chk;
+        chit = wb::shape_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+      }
+      if (COUNT && hit) wc.groups++;
+      if (chit) wc.groups_scanned++;
+      if (__ballot(chit)) {
+        const uint32_t jend = SAMPLE ? c0 + 1 : ce;
+        if (COUNT && chit) wc.points += jend - c0;
+        for (uint32_t j = c0; j < jend; j++) {
+          const double qx = T.px[j], qy = T.py[j];
+          if (chit && (all || wb::hyp_in(H, qx, qy, T.pz[j])) && (qx != x || qy != y))
+            on_dir(qy - y, qx - x);
+        }
       }
       c0 = ce;
     }
-    return false;
-  });
+    active = after_leaf();
+    if (__ballot(active) == 0) return;
+  }
 }
 
+// a lane whose sampled pass scans more than HEAVY_B_GROUPS 32-point groups
+// (about 32x as many point tests in the full pass), or whose sampled pass
+// alone exceeds HEAVY_A_GROUPS, is finished by k_classify_heavy
+#define HEAVY_A_GROUPS 2048u
+#define HEAVY_B_GROUPS 256u
+
+template <bool COUNT>
 __global__ void __launch_bounds__(256) k_classify(ClassifyArgs A)
 {
+  WalkCount wa = {0, 0, 0, 0, 0}, wb_ = {0, 0, 0, 0, 0}, wcc = {0, 0, 0, 0, 0};
+  uint32_t ndir = 0;
+  const uint64_t t_start = COUNT ? wall_clock64() : 0;
   __shared__ double tabs[511 + 512 + 512];
   for (int k = threadIdx.x; k < 511 + 512 + 512; k += blockDim.x) tabs[k] = A.tab[k];
   __syncthreads();
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.T.n) return;
-  if (A.nmemb[i] == 0) {
-    A.cls[i] = 0;
-    return;
-  }
+  const TreeView &T = A.T;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool inrange = i < T.n;
   const double *ttan = tabs, *tcos = tabs + 511, *tsin = tabs + 1023;
+  // lane state: 0 undecided, 1 not ground, 2 ground, 3 no tile, 4 ambiguous
+  int state = 3;
+  double x = 0, y = 0, z = 0;
+  wb::Shape H = wb::make_cylinder(0, 0, 0);
+  if (inrange && A.nmemb[i] > 0) {
+    int w = INT32_MIN;
+    for (uint32_t k = 0; k < A.nmemb[i]; k++) w = max(w, A.memb[MAX_MEMB * i + k]);
+    x = T.px[i]; y = T.py[i]; z = T.pz[i];
+    H = wb::make_hyperboloid(x - 0, y - 0, z - A.thickness, A.rec[w - A.g.lo].hs, A.max_slope);
+    state = 0;
+  }
   uint64_t occ = 0, checked = 0;
-  int verdict = 0, nexact = 0;
-  enumerate_down(A, (uint32_t)i, tabs, [&](double dy, double dx) {
+  int nexact = 0;
+  auto add_dir = [&](double dy, double dx) {
+    if (COUNT) ndir++;
     int a = wb::atan2i(dy, dx, ttan, tcos, tsin, A.K, &nexact);
     occ |= 1ull << (dir_bin_pos(a) >> 25);
-    if (occ != checked) {
+  };
+  auto after = [&]() {
+    if (state == 0 && occ != checked) {
       checked = occ;
-      if (gap_verdict(occ) == 1) { verdict = 1; return true; }
+      if (gap_verdict(occ) == 1) state = 1;
     }
-    return false;
-  });
-  if (verdict == 0) verdict = gap_verdict(occ);
+    if (state == 0 && wa.groups_scanned > HEAVY_A_GROUPS) state = 5;
+    return state == 0;
+  };
+  // phase A: one point per 32-point group; phase B: every point
+  const bool wasA = state == 0;
+  wave_walk<true, COUNT>(T, H, x, y, state == 0, add_dir, after, wa);
+  const uint32_t dirA = ndir;
+  // points whose candidate set is large go to the wave-per-point kernel
+  if (state == 0 && wa.groups_scanned > HEAVY_B_GROUPS) state = 5;
+  {
+    const uint64_t hb = __ballot(state == 5);
+    if (hb) {
+      const int leader = __ffsll((long long)hb) - 1;
+      uint32_t base = 0;
+      if ((int)__lane_id() == leader) base = atomicAdd(A.nheavy, (uint32_t)__popcll(hb));
+      base = __shfl(base, leader, 64);
+      if (state == 5) A.heavy[base + __popcll(hb & ((1ull << __lane_id()) - 1))] = (uint32_t)i;
+    }
+  }
+  const bool inB = state == 0;
+  wave_walk<false, COUNT>(T, H, x, y, state == 0, add_dir, after, wb_);
+  if (state == 0) {
+    int v = gap_verdict(occ);
+    state = v == 0 ? 4 : v;
+  }
+  // phase C: a longest empty run of 24 or 25 bins; record the largest
+  // direction before and the smallest after each such run and compare the
+  // exact gaps with DEG144
+  if (__ballot(state == 4)) {
+    int before[2] = {-1, -1}, afterb[2] = {-1, -1}, nr = 0;
+    int hi[2] = {-1, -1}, lo[2] = {0x7fffffff, 0x7fffffff};
+    if (state == 4) {
+      for (int st = 0; st < 64; st++) {
+        if ((occ >> st) & 1) continue;
+        if (!((occ >> ((st + 63) & 63)) & 1)) continue;
+        int len = 0;
+        while (len < 64 && !((occ >> ((st + len) & 63)) & 1)) len++;
+        if (len >= 24 && nr < 2) {
+          before[nr] = (st + 63) & 63;
+          afterb[nr] = (st + len) & 63;
+          nr++;
+        }
+      }
+    }
+    auto track = [&](double dy, double dx) {
+      int a = wb::atan2i(dy, dx, ttan, tcos, tsin, A.K, &nexact);
+      int pos = dir_bin_pos(a), bin = pos >> 25;
+      for (int k = 0; k < 2; k++) {
+        if (bin == before[k] && pos > hi[k]) hi[k] = pos;
+        if (bin == afterb[k] && pos < lo[k]) lo[k] = pos;
+      }
+    };
+    const bool amb = state == 4;
+    wave_walk<false, COUNT>(T, H, x, y, amb, track, [&]() { return amb; }, wcc);
+    if (amb) {
+      bool pass = true;
+      for (int k = 0; k < nr; k++) {
+        int gap = (int)(((uint32_t)lo[k] - (uint32_t)hi[k]) & 0x7fffffffu);
+        if (gap >= wb::DEG144) pass = false;
+      }
+      state = pass ? 1 : 2;
+      atomicAdd(A.nslow, 1u);
+    }
+  }
   if (nexact) atomicAdd(A.atan_exact, (unsigned long long)nexact);
-  if (verdict == 0) {
-    uint32_t s = atomicAdd(A.nslow, 1u);
-    A.slow[s] = (uint32_t)i;
-    A.cls[i] = 0;
-    return;
-  }
-  A.cls[i] = verdict == 1 ? 1 : 2;
-}
-
-// exact pass for points whose longest empty run is 24 or 25 bins: record
-// the largest direction in the bin before each such run and the smallest in
-// the bin after it, and compare the exact gaps with DEG144
-__global__ void __launch_bounds__(64) k_classify_exact(ClassifyArgs A, uint32_t nslow)
-{
-  __shared__ double tabs[511 + 512 + 512];
-  for (int k = threadIdx.x; k < 511 + 512 + 512; k += blockDim.x) tabs[k] = A.tab[k];
-  __syncthreads();
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslow) return;
-  uint32_t i = A.slow[s];
-  const double *ttan = tabs, *tcos = tabs + 511, *tsin = tabs + 1023;
-  // pass 1: occupancy
-  uint64_t occ = 0;
-  enumerate_down(A, i, tabs, [&](double dy, double dx) {
-    int a = wb::atan2i(dy, dx, ttan, tcos, tsin, A.K);
-    occ |= 1ull << (dir_bin_pos(a) >> 25);
-    return false;
-  });
-  // runs of 24 or 25 empty bins
-  int before[2], after[2], nr = 0;
-  for (int st = 0; st < 64; st++) {
-    if ((occ >> st) & 1) continue;
-    if (!((occ >> ((st + 63) & 63)) & 1)) continue;  // not a run start
-    int len = 0;
-    while (len < 64 && !((occ >> ((st + len) & 63)) & 1)) len++;
-    if (len >= 24 && nr < 2) {
-      before[nr] = (st + 63) & 63;
-      after[nr] = (st + len) & 63;
-      nr++;
+  if (inrange && state != 5) A.cls[i] = state == 1 ? 1 : (state == 2 ? 2 : 0);
+  if (COUNT) {
+    // 0 lanes in A, 1 lanes in B, 2-5 phase A leaves/groups/scanned/points,
+    // 6-9 phase B, 10 dirs A, 11 dirs B, 12 phase C points, 13 steps A (per
+    // wave), 14 steps B (per wave), 15 waves
+    uint64_t v[16] = {wasA, inB, wa.leaves, wa.groups, wa.groups_scanned, wa.points,
+                      wb_.leaves, wb_.groups, wb_.groups_scanned, wb_.points, dirA,
+                      ndir - dirA, wcc.points, 0, 0, 0};
+    for (int k = 0; k < 13; k++) {
+      uint64_t t = v[k];
+      for (int sft = 1; sft < 64; sft <<= 1) t += __shfl_xor(t, sft, 64);
+      v[k] = t;
+    }
+    if (__lane_id() == 0) {
+      for (int k = 0; k < 13; k++) atomicAdd(&A.counters[k], (unsigned long long)v[k]);
+      atomicAdd(&A.counters[13], (unsigned long long)wa.steps);
+      atomicAdd(&A.counters[14], (unsigned long long)wb_.steps);
+      atomicAdd(&A.counters[15], 1ull);
+      uint64_t dt = wall_clock64() - t_start;
+      atomicAdd(&A.counters[16], (unsigned long long)dt);
+      atomicMax(&A.counters[17], (unsigned long long)dt);
+      // histogram of wave times in ticks: <1e3, <1e4, <1e5, <1e6, >=1e6
+      int hb = dt < 1000 ? 0 : dt < 10000 ? 1 : dt < 100000 ? 2 : dt < 1000000 ? 3 : 4;
+      atomicAdd(&A.counters[18 + hb], 1ull);
+      atomicAdd(&A.counters[23 + hb], (unsigned long long)dt);
     }
   }
-  int hi[2] = {-1, -1}, lo[2] = {0x7fffffff, 0x7fffffff};
-  enumerate_down(A, i, tabs, [&](double dy, double dx) {
-    int a = wb::atan2i(dy, dx, ttan, tcos, tsin, A.K);
-    int pos = dir_bin_pos(a), bin = pos >> 25;
-    for (int k = 0; k < nr; k++) {
-      if (bin == before[k] && pos > hi[k]) hi[k] = pos;
-      if (bin == after[k] && pos < lo[k]) lo[k] = pos;
-    }
-    return false;
-  });
-  bool pass = true;
-  for (int k = 0; k < nr; k++) {
-    int gap = (int)(((uint32_t)lo[k] - (uint32_t)hi[k]) & 0x7fffffffu);
-    if (gap >= wb::DEG144) pass = false;
-  }
-  A.cls[i] = pass ? 1 : 2;
 }
 
 __global__ void k_scatter_class(uint64_t n, const uint8_t *cls, const uint32_t *sidx, uint8_t *out)
@@ -716,15 +829,160 @@ __global__ void k_scatter_class(uint64_t n, const uint8_t *cls, const uint32_t *
   out[sidx[i]] = cls[i];
 }
 
+__device__ inline uint64_t wave_or(uint64_t v)
+{
+  for (int s = 1; s < 64; s <<= 1) v |= __shfl_xor(v, s, 64);
+  return v;
+}
+
+// One heavy point per 64-lane block: the walk is uniform (one shape) and the
+// lanes split each leaf's points.  Same three phases as k_classify: sampled
+// groups, all points, then the exact border pass for a 24/25-bin run.
+__global__ void __launch_bounds__(64) k_classify_heavy(ClassifyArgs A, const uint32_t *heavy,
+                                                       uint32_t nheavy)
+{
+  __shared__ double tabs[511 + 512 + 512];
+  for (int k = threadIdx.x; k < 511 + 512 + 512; k += blockDim.x) tabs[k] = A.tab[k];
+  __syncthreads();
+  if (blockIdx.x >= nheavy) return;
+  const TreeView &T = A.T;
+  const uint32_t i = heavy[blockIdx.x];
+  const int lane = threadIdx.x;
+  const double *ttan = tabs, *tcos = tabs + 511, *tsin = tabs + 1023;
+  int w = INT32_MIN;
+  for (uint32_t k = 0; k < A.nmemb[i]; k++) w = max(w, A.memb[MAX_MEMB * i + k]);
+  const double x = T.px[i], y = T.py[i], z = T.pz[i];
+  const wb::Shape H = wb::make_hyperboloid(x - 0, y - 0, z - A.thickness, A.rec[w - A.g.lo].hs,
+                                           A.max_slope);
+  int nexact = 0;
+  // mode 0: sampled, 1: all points, 2: exact border pass
+  auto walk = [&](int mode, auto &&on_dir, auto &&after_leaf) {
+    if (!wb::shape_intersect(H, T.node_cube[0], T.node_cube[1], T.node_cube[2], T.node_cube[3]))
+      return;
+    int node = 0, depth = 0;
+    Iter it = {0, 0};
+    for (;;) {
+      int c = it.get(depth);
+      if (c == 8) {
+        if (depth == 0) return;
+        node = T.node_parent[node];
+        depth--;
+        continue;
+      }
+      it.set(depth, c + 1);
+      int ch = T.node_child[8 * node + c];
+      if (ch == -1) continue;
+      if (ch >= 0) {
+        const double *q = T.node_cube + 4 * ch;
+        if (wb::shape_intersect(H, q[0], q[1], q[2], q[3])) {
+          node = ch;
+          depth++;
+          it.set(depth, 0);
+        }
+        continue;
+      }
+      const int leaf = -2 - ch;
+      const double *lc = T.leaf_cube + 4 * leaf;
+      if (!wb::shape_intersect(H, lc[0], lc[1], lc[2], lc[3])) continue;
+      const bool all = wb::shape_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
+      const uint32_t b = T.leaf_start[leaf], e = T.leaf_start[leaf + 1];
+      if (mode == 0) {
+        for (uint32_t k = b / WB_CHUNK + lane; k * WB_CHUNK < e; k += 64) {
+          const uint32_t j = max(b, k * WB_CHUNK);
+          if (!all) {
+            const double *bx = T.chunk_box + 6 * (uint64_t)k;
+            if (!wb::shape_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5])) continue;
+          }
+          const double qx = T.px[j], qy = T.py[j];
+          if ((all || wb::hyp_in(H, qx, qy, T.pz[j])) && (qx != x || qy != y)) on_dir(qy - y, qx - x);
+        }
+      } else {
+        for (uint32_t j = b + lane; j < e; j += 64) {
+          if (!all) {
+            const double *bx = T.chunk_box + 6 * (uint64_t)(j / WB_CHUNK);
+            if (!wb::shape_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5])) continue;
+          }
+          const double qx = T.px[j], qy = T.py[j];
+          if ((all || wb::hyp_in(H, qx, qy, T.pz[j])) && (qx != x || qy != y)) on_dir(qy - y, qx - x);
+        }
+      }
+      if (after_leaf()) return;
+    }
+  };
+  uint64_t occ = 0, checked = 0;
+  int verdict = 0;
+  auto add_dir = [&](double dy, double dx) {
+    int a = wb::atan2i(dy, dx, ttan, tcos, tsin, A.K, &nexact);
+    occ |= 1ull << (dir_bin_pos(a) >> 25);
+  };
+  auto after = [&]() {
+    uint64_t all = wave_or(occ);
+    if (all != checked) {
+      checked = all;
+      if (gap_verdict(all) == 1) verdict = 1;
+    }
+    return verdict == 1;
+  };
+  walk(0, add_dir, after);
+  if (verdict == 0) walk(1, add_dir, after);
+  uint64_t occw = wave_or(occ);
+  if (verdict == 0) verdict = gap_verdict(occw);
+  if (verdict == 0) {
+    int before[2] = {-1, -1}, afterb[2] = {-1, -1}, nr = 0;
+    for (int st = 0; st < 64; st++) {
+      if ((occw >> st) & 1) continue;
+      if (!((occw >> ((st + 63) & 63)) & 1)) continue;
+      int len = 0;
+      while (len < 64 && !((occw >> ((st + len) & 63)) & 1)) len++;
+      if (len >= 24 && nr < 2) {
+        before[nr] = (st + 63) & 63;
+        afterb[nr] = (st + len) & 63;
+        nr++;
+      }
+    }
+    int hi[2] = {-1, -1}, lo[2] = {0x7fffffff, 0x7fffffff};
+    auto track = [&](double dy, double dx) {
+      int a = wb::atan2i(dy, dx, ttan, tcos, tsin, A.K, &nexact);
+      int pos = dir_bin_pos(a), bin = pos >> 25;
+      for (int k = 0; k < 2; k++) {
+        if (bin == before[k] && pos > hi[k]) hi[k] = pos;
+        if (bin == afterb[k] && pos < lo[k]) lo[k] = pos;
+      }
+    };
+    walk(2, track, [&]() { return false; });
+    bool pass = true;
+    for (int k = 0; k < nr; k++) {
+      int h = hi[k], l = lo[k];
+      for (int s = 1; s < 64; s <<= 1) {
+        h = max(h, __shfl_xor(h, s, 64));
+        l = min(l, __shfl_xor(l, s, 64));
+      }
+      int gap = (int)(((uint32_t)l - (uint32_t)h) & 0x7fffffffu);
+      if (gap >= wb::DEG144) pass = false;
+    }
+    verdict = pass ? 1 : 2;
+    if (lane == 0) atomicAdd(A.nslow, 1u);
+  }
+  for (int s = 1; s < 64; s <<= 1) nexact += __shfl_xor(nexact, s, 64);
+  if (lane == 0) {
+    A.cls[i] = (uint8_t)verdict;
+    if (nexact) atomicAdd(A.atan_exact, (unsigned long long)nexact);
+  }
+}
+
+// class histogram (wave-aggregated) and, when tile records are wanted, the
+// number of ground points each deciding tile classified
 __global__ void k_count_ground(uint64_t n, const uint8_t *cls, const int32_t *memb,
                                const uint32_t *nmemb, int lo, int32_t *nground,
-                               unsigned long long *hist3)
+                               unsigned long long *hist3, int per_tile)
 {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint8_t c = cls[i];
-  atomicAdd(&hist3[c < 3 ? c : 0], 1ull);
-  if (c == 2) {
+  uint8_t c = i < n ? cls[i] : 255;
+  for (int k = 0; k < 3; k++) {
+    uint64_t b = __ballot(c == k);
+    if (__lane_id() == 0 && b) atomicAdd(&hist3[k], (unsigned long long)__popcll(b));
+  }
+  if (per_tile && c == 2) {
     int w = INT32_MIN;
     for (uint32_t k = 0; k < nmemb[i]; k++) w = max(w, memb[MAX_MEMB * i + k]);
     atomicAdd(&nground[w - lo], 1);
@@ -761,8 +1019,8 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   g.scx = ss.cx; g.scy = ss.cy; g.spacing = ss.spacing; g.r = ss.radius;
   g.lo = ss.lo; g.hi = ss.hi;
   g.ntiles = (uint32_t)((int64_t)ss.hi - ss.lo + 1);
-  hipEvent_t ev[4];
-  for (int k = 0; k < 4; k++) WB_CHECK(hipEventCreate(&ev[k]));
+  hipEvent_t ev[6];
+  for (int k = 0; k < 6; k++) WB_CHECK(hipEventCreate(&ev[k]));
   WB_CHECK(hipEventRecord(ev[0], st));
   int32_t *tex = (int32_t *)wb_alloc(c, 4ull * g.ntiles), *tey = (int32_t *)wb_alloc(c, 4ull * g.ntiles);
   uint32_t *tile_cnt = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
@@ -796,22 +1054,41 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   WB_CHECK(hipEventRecord(ev[2], st));
   // classify
   uint8_t *cls = (uint8_t *)wb_alloc(c, n + 16);
-  uint32_t *slow = (uint32_t *)wb_alloc(c, 4 * n + 4);
   ClassifyArgs A;
   A.T = T; A.g = g; A.memb = memb; A.nmemb = nmemb; A.rec = rec;
   A.thickness = p->thickness; A.max_slope = p->max_slope;
-  A.tab = c->d_tan; A.K = c->kconst; A.cls = cls; A.slow = slow; A.nslow = nslow;
+  A.tab = c->d_tan; A.K = c->kconst; A.cls = cls; A.nslow = nslow;
   A.atan_exact = ctr + 1;
-  if (n > 0) hipLaunchKernelGGL(k_classify, dim3(wb_blocks(n, 256)), dim3(256), 0, st, A);
+  WB_CHECK(hipEventRecord(ev[4], st));
+  static const bool count_work = getenv("WB_CLASSIFY_COUNTERS") != nullptr;
+  unsigned long long *dcount = nullptr;
+  if (count_work) {
+    dcount = (unsigned long long *)wb_alloc(c, 32 * 8);
+    WB_CHECK(hipMemsetAsync(dcount, 0, 32 * 8, st));
+  }
+  A.counters = dcount;
+  uint32_t *heavy = (uint32_t *)wb_alloc(c, 4 * n + 4);
+  A.heavy = heavy;
+  A.nheavy = nslow + 1;
+  if (n > 0) {
+    if (count_work)
+      hipLaunchKernelGGL(k_classify<true>, dim3(wb_blocks(n, 256)), dim3(256), 0, st, A);
+    else
+      hipLaunchKernelGGL(k_classify<false>, dim3(wb_blocks(n, 256)), dim3(256), 0, st, A);
+  }
+  uint32_t h_nheavy = 0;
+  WB_CHECK(hipMemcpyAsync(&h_nheavy, nslow + 1, 4, hipMemcpyDeviceToHost, st));
+  WB_CHECK(hipStreamSynchronize(st));
+  if (h_nheavy)
+    hipLaunchKernelGGL(k_classify_heavy, dim3(h_nheavy), dim3(64), 0, st, A, heavy, h_nheavy);
+  WB_CHECK(hipEventRecord(ev[5], st));
   uint32_t h_nslow = 0;
   WB_CHECK(hipMemcpyAsync(&h_nslow, nslow, 4, hipMemcpyDeviceToHost, st));
-  WB_CHECK(hipStreamSynchronize(st));
-  if (h_nslow) hipLaunchKernelGGL(k_classify_exact, dim3(wb_blocks(h_nslow, 64)), dim3(64), 0, st, A, h_nslow);
   int32_t *nground = (int32_t *)tile_fill;  // reuse
   WB_CHECK(hipMemsetAsync(nground, 0, 4ull * g.ntiles, st));
   if (n > 0) {
     hipLaunchKernelGGL(k_count_ground, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, cls, memb,
-                       nmemb, g.lo, nground, ctr + 2);
+                       nmemb, g.lo, nground, ctr + 2, want_tiles ? 1 : 0);
     hipLaunchKernelGGL(k_scatter_class, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, cls, t->sidx,
                        d_class_out);
   }
@@ -835,12 +1112,29 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   hipEventElapsedTime(&ms1, ev[0], ev[1]);
   hipEventElapsedTime(&ms2, ev[1], ev[2]);
   hipEventElapsedTime(&ms3, ev[2], ev[3]);
-  for (int k = 0; k < 4; k++) hipEventDestroy(ev[k]);
+  float ms4 = 0;
+  hipEventElapsedTime(&ms4, ev[4], ev[5]);
+  out->ms_classify_kernel = ms4;
+  for (int k = 0; k < 6; k++) hipEventDestroy(ev[k]);
   unsigned long long *hc = (unsigned long long *)(h_err + 2);
   out->borderline = hc[0];
   out->atan_exact = hc[1];
   out->hist[0] = hc[2]; out->hist[1] = hc[3]; out->hist[2] = hc[4];
   out->nslow = h_nslow;
+  if (dcount) {
+    unsigned long long hcnt[32];
+    WB_CHECK(hipMemcpy(hcnt, dcount, sizeof hcnt, hipMemcpyDeviceToHost));
+    fprintf(stderr, "[wb counters] lanesA %llu lanesB %llu | A: leaves %llu groups %llu scanned %llu points %llu | B: leaves %llu groups %llu scanned %llu points %llu | dirs A %llu B %llu | C points %llu | steps A %llu B %llu waves %llu\n",
+            hcnt[0], hcnt[1], hcnt[2], hcnt[3], hcnt[4], hcnt[5], hcnt[6], hcnt[7], hcnt[8], hcnt[9],
+            hcnt[10], hcnt[11], hcnt[12], hcnt[13], hcnt[14], hcnt[15]);
+    int rate = 0;
+    hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, c->device);
+    fprintf(stderr, "[wb counters] wave time: sum %llu max %llu ticks (rate %d kHz); hist counts %llu %llu %llu %llu %llu ; hist tick sums %llu %llu %llu %llu %llu\n",
+            hcnt[16], hcnt[17], rate, hcnt[18], hcnt[19], hcnt[20], hcnt[21], hcnt[22], hcnt[23],
+            hcnt[24], hcnt[25], hcnt[26], hcnt[27]);
+    wb_free(c, dcount, 32 * 8);
+  }
+  out->nonempty = hc[5];
   out->ms_tiles = ms1; out->ms_scan = ms2; out->ms_classify = ms3;
   wb_free(c, tex, 4ull * g.ntiles);
   wb_free(c, tey, 4ull * g.ntiles);
@@ -852,7 +1146,8 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   wb_free(c, tile_off, 4ull * g.ntiles + 4);
   wb_free(c, lists, 4 * total + 4);
   wb_free(c, cls, n + 16);
-  wb_free(c, slow, 4 * n + 4);
+  wb_free(c, heavy, 4 * n + 4);
+  out->nheavy = h_nheavy;
   if (h_err[0] & 1u) { wb_free(c, err, 64); throw std::string("a point lies in more than 4 tile cylinders"); }
   if (h_err[0] & 2u) { wb_free(c, err, 64); throw std::string("a tile cylinder holds more than 1024 points"); }
   wb_free(c, err, 64);

@@ -49,6 +49,7 @@ struct wb_tree {
   double scale[3], offset[3];
   int sort_passes, maxdepth;
   uint32_t nleaves, nnodes;
+  float ms_build, ms_sort;  // device time of the last build (events)
   // device arrays
   uint64_t *key;
   uint32_t *sidx;
@@ -93,8 +94,8 @@ struct TileRec {
 struct ClassifyOut {
   std::vector<TileRec> tiles;
   std::vector<int32_t> tex, tey;
-  uint64_t borderline, atan_exact, nslow, hist[3];
-  double ms_tiles, ms_scan, ms_classify;
+  uint64_t borderline, atan_exact, nslow, hist[3], nonempty, nheavy;
+  double ms_tiles, ms_scan, ms_classify, ms_classify_kernel;
 };
 
 // ---- implemented in wb_classify.hip ----
