@@ -94,6 +94,7 @@ typedef struct {
   double ms_scan;             /* scan + postscan */
   double ms_classify;         /* ground test, exact pass, class scatter */
   double ms_classify_kernel;  /* the ground-test kernel alone */
+  double ms_heavy_kernel;     /* the wave-per-point kernel for large candidate sets */
   double ms_total;            /* host wall time of wb_classify */
 } wb_stats;
 

@@ -76,6 +76,9 @@ void wbo_fill_tables(void)
   tables_ready = 1;
 }
 
+double wbo_sin_bin(int a) { return bsin(a); }
+double wbo_cos_bin(int a) { return bcos(a); }
+
 void wbo_get_tables(double *tan511, double *cos512, double *sin512)
 {
   if (!tables_ready) wbo_fill_tables();
@@ -678,10 +681,15 @@ static int i_to_flowsnake(int n)
 
 /* baseFlow: centred base-7 digits -> Eisenstein integer in base 2-w
  * (flowsnake.cpp:67-73, 112-131).  (x,y) means x + y*w. */
-void wbo_to_flowsnake(int n, int *ex, int *ey)
+static void base_flow(int m, int *ex, int *ey);
+
+void wbo_to_flowsnake(int n, int *ex, int *ey) { base_flow(i_to_flowsnake(n), ex, ey); }
+void wbo_base_flow(int m, int *ex, int *ey) { base_flow(m, ex, ey); }
+int wbo_i_to_flowsnake(int n) { return i_to_flowsnake(n); }
+
+static void base_flow(int m, int *ex, int *ey)
 {
   long long rx = 0, ry = 0, px = 1, py = 0;
-  int m = i_to_flowsnake(n);
   while (m) {
     int d = m % 7, dy, dx;
     if (d > 3) d -= 7;

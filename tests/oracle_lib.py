@@ -103,6 +103,14 @@ def lib():
         L.wbo_classify_points.argtypes = [C.c_void_p, _lp, C.c_long, _dp,
                                           C.POINTER(Params), _u8p]
         L.wbo_get_tables.argtypes = [_dp, _dp, _dp]
+        L.wbo_sin_bin.restype = C.c_double
+        L.wbo_sin_bin.argtypes = [C.c_int]
+        L.wbo_cos_bin.restype = C.c_double
+        L.wbo_cos_bin.argtypes = [C.c_int]
+        L.wbo_base_flow.argtypes = [C.c_int, _ip, _ip]
+        L.wbo_i_to_flowsnake.restype = C.c_int
+        L.wbo_i_to_flowsnake.argtypes = [C.c_int]
+        L.wbo_set_exhaustive.argtypes = [C.c_int]
         L.wbo_fill_tables()
         _lib = L
     return _lib
@@ -202,6 +210,16 @@ def to_flowsnake(n):
     ex, ey = C.c_int(), C.c_int()
     lib().wbo_to_flowsnake(int(n), C.byref(ex), C.byref(ey))
     return ex.value, ey.value
+
+
+def base_flow(m):
+    ex, ey = C.c_int(), C.c_int()
+    lib().wbo_base_flow(int(m), C.byref(ex), C.byref(ey))
+    return ex.value, ey.value
+
+
+def set_exhaustive(on):
+    lib().wbo_set_exhaustive(1 if on else 0)
 
 
 def base_seven(ex, ey):

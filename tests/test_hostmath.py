@@ -1,0 +1,189 @@
+"""The device math (wb_math.h, wb_flowsnake.h, wb_host.cpp) built for the
+host, pinned against glibc and the oracle without a GPU."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import wolkenbase_amd
+
+HM = os.path.join(os.path.dirname(wolkenbase_amd.__file__), "libwb_hostmath.so")
+D = C.POINTER(C.c_double)
+
+
+@pytest.fixture(scope="module")
+def hm():
+    L = C.CDLL(HM)
+    L.wbh_hypot.restype = C.c_double
+    L.wbh_hypot.argtypes = [C.c_double, C.c_double]
+    L.wbh_atan2i.restype = C.c_int
+    L.wbh_atan2i.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_int)]
+    L.wbh_atan2i_forced.restype = C.c_int
+    L.wbh_atan2i_forced.argtypes = [C.c_double, C.c_double]
+    L.wbh_last_exact.restype = C.c_long
+    L.wbh_last_exact.argtypes = [C.c_double, C.c_double]
+    L.wbh_last_native.restype = C.c_long
+    L.wbh_last_native.argtypes = [C.c_double, C.c_double]
+    L.wbh_x80_op_check.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+    L.wbh_x80_lrint_check.argtypes = [C.c_double, C.c_double]
+    for f in ("wbh_shape_in",):
+        getattr(L, f).argtypes = [C.c_int, D, D]
+    L.wbh_shape_in_cube.argtypes = [C.c_int, D, D, C.c_double]
+    L.wbh_shape_intersect.argtypes = [C.c_int, D, D, C.c_double]
+    L.wbh_to_flowsnake.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.wbh_from_flowsnake.restype = C.c_int
+    L.wbh_from_flowsnake.argtypes = [C.c_int] * 4
+    L.wbh_snake_setup.argtypes = [D, D, C.c_double, D, C.POINTER(C.c_int)]
+    L.wbh_size_fit.argtypes = [D, C.c_int, D, D]
+    return L
+
+
+def glibc_hypot():
+    m = C.CDLL("libm.so.6")
+    m.hypot.restype = C.c_double
+    m.hypot.argtypes = [C.c_double, C.c_double]
+    return m.hypot
+
+
+def test_hypot_matches_glibc(hm):
+    gh = glibc_hypot()
+    rng = np.random.default_rng(11)
+    n = 200000
+    a = np.concatenate([
+        (rng.integers(-10**6, 10**6, n) * 0.001 + 123456.0) - (rng.integers(-10**6, 10**6, n) * 0.001 + 123456.0),
+        rng.random(n) * 100,
+        np.ldexp(rng.random(n), rng.integers(-1060, 1060, n)),
+        rng.integers(0, 2**62, n).astype(np.uint64).view(np.float64)])
+    b = np.concatenate([
+        rng.integers(-3000, 3000, n) * 0.001,
+        rng.random(n) * 100,
+        np.ldexp(rng.random(n), rng.integers(-1060, 1060, n)),
+        rng.integers(0, 2**62, n).astype(np.uint64).view(np.float64)])
+    bad = 0
+    for x, y in zip(a, b):
+        if not (np.isfinite(x) and np.isfinite(y)):
+            continue
+        u, v = hm.wbh_hypot(x, y), gh(x, y)
+        if np.float64(u).view(np.uint64) != np.float64(v).view(np.uint64):
+            bad += 1
+    assert bad == 0
+
+
+def test_hypot_special(hm):
+    inf, nan = float("inf"), float("nan")
+    assert hm.wbh_hypot(inf, nan) == inf
+    assert np.isnan(hm.wbh_hypot(nan, 1.0))
+    assert hm.wbh_hypot(0.0, -0.0) == 0.0
+    assert hm.wbh_hypot(3.0, 4.0) == 5.0
+
+
+def test_atan2i_matches_oracle(hm):
+    L = O.lib()
+    rng = np.random.default_rng(12)
+    n = 100000
+    ys = np.concatenate([rng.normal(size=n), rng.integers(-4000, 4000, n) * 0.001, np.zeros(20)])
+    xs = np.concatenate([rng.normal(size=n), rng.integers(-4000, 4000, n) * 0.001,
+                         rng.normal(size=20)])
+    used = C.c_int(0)
+    bad = [(y, x) for y, x in zip(ys, xs) if hm.wbh_atan2i(y, x, C.byref(used)) != L.wbo_atan2i(y, x)]
+    assert not bad, bad[:5]
+
+
+def test_atan2i_exact_path_matches_oracle(hm):
+    # every last step through the 64-bit-significand emulation
+    L = O.lib()
+    rng = np.random.default_rng(13)
+    for y, x in zip(rng.normal(size=30000), rng.normal(size=30000)):
+        assert hm.wbh_atan2i_forced(y, x) == L.wbo_atan2i(y, x)
+
+
+def test_x87_emulation_ops(hm):
+    rng = np.random.default_rng(14)
+    for _ in range(40000):
+        a = rng.normal() * 2.0 ** rng.integers(-30, 30)
+        b = rng.normal() * 2.0 ** rng.integers(-30, 30)
+        c = rng.normal() * 2.0 ** rng.integers(-5, 5)
+        for op in range(3):
+            assert hm.wbh_x80_op_check(op, a, b, c), (op, a, b, c)
+        assert hm.wbh_x80_lrint_check(a if abs(a) < 1e15 else 1.0, c)
+    for k in range(-3000, 3000):               # exact ties round to even
+        assert hm.wbh_x80_lrint_check(k + 0.5, 1.0)
+
+
+def test_last_step_exact_vs_native(hm):
+    rng = np.random.default_rng(15)
+    for _ in range(50000):
+        x = abs(rng.normal()) + 1e-3
+        y = x * rng.uniform(-0.004, 0.004)     # the residual range after the table search
+        assert hm.wbh_last_exact(y, x) == hm.wbh_last_native(y, x)
+
+
+def test_shapes_match_oracle(hm):
+    rng = np.random.default_rng(16)
+    for it in range(30000):
+        kind = it % 3
+        c = rng.normal(size=3) * 3
+        if kind == 0:
+            p = np.array([c[0], c[1], abs(rng.normal()) * 2, 0, 0])
+        elif kind == 1:
+            p = np.array([c[0], c[1], c[2], abs(rng.normal()) * 2, 0])
+        else:
+            p = np.array([c[0], c[1], c[2], abs(rng.normal()), rng.choice([1.0, -1.0, 0.5])])
+        q = rng.normal(size=3) * 3
+        cc = np.round(rng.normal(size=3) * 4) / 4
+        side = 2.0 ** rng.integers(-3, 3)
+        pp, qq, ccc = (np.ascontiguousarray(v) for v in (p, q, cc))
+        got = (hm.wbh_shape_in(kind, pp.ctypes.data_as(D), qq.ctypes.data_as(D)),
+               hm.wbh_shape_in_cube(kind, pp.ctypes.data_as(D), ccc.ctypes.data_as(D), side),
+               hm.wbh_shape_intersect(kind, pp.ctypes.data_as(D), ccc.ctypes.data_as(D), side))
+        want = (O.shape_in(kind, p[:4 if kind else 3], q), O.shape_in_cube(kind, p[:4 if kind else 3], cc, side),
+                O.shape_intersect(kind, p[:4 if kind else 3], cc, side)) if kind < 2 else \
+               (O.shape_in(kind, p, q), O.shape_in_cube(kind, p, cc, side), O.shape_intersect(kind, p, cc, side))
+        assert got == want, (kind, p, q, cc, side)
+
+
+def test_shape_boundary_cases_take_exact_path(hm):
+    # points placed on the cylinder boundary to within an ulp exercise the
+    # hypot fallback of the fast filter
+    rng = np.random.default_rng(17)
+    for _ in range(20000):
+        r = rng.uniform(0.1, 5)
+        th = rng.uniform(0, 2 * np.pi)
+        q = np.array([r * np.cos(th), r * np.sin(th), 0.0])
+        p = np.array([0.0, 0.0, r, 0, 0])
+        got = hm.wbh_shape_in(0, p.ctypes.data_as(D), q.ctypes.data_as(D))
+        assert got == O.shape_in(0, p[:3], q)
+
+
+def test_flowsnake_round_trip(hm):
+    lo = [0, -4, -18, -214, -900, -10504, -44118]
+    hi = [0, 2, 30, 128, 1500, 6302, 73530]
+    for k in range(7):
+        for n in range(lo[k], hi[k] + 1, max(1, (hi[k] - lo[k]) // 3000)):
+            ex, ey = C.c_int(), C.c_int()
+            hm.wbh_to_flowsnake(n, C.byref(ex), C.byref(ey))
+            assert (ex.value, ey.value) == O.to_flowsnake(n)
+            assert hm.wbh_from_flowsnake(ex.value, ey.value, lo[k], hi[k]) == n
+    # a lattice point outside the island has no counter
+    assert hm.wbh_from_flowsnake(10**6, 0, lo[6], hi[6]) == -2**31
+
+
+def test_snake_setup_and_size_fit_match_oracle(hm):
+    rng = np.random.default_rng(18)
+    for _ in range(200):
+        lo = rng.normal(size=3) * 1e5 + np.array([5e5, 4.4e6, 0])
+        hi = lo + rng.uniform(1, 3000, 3)
+        out4 = np.zeros(4)
+        out3 = (C.c_int * 3)()
+        hm.wbh_snake_setup(lo.ctypes.data_as(D), hi.ctypes.data_as(D), 1.0,
+                           out4.ctypes.data_as(D), out3)
+        sn = O.snake_setup(lo, hi, 1.0)
+        assert (out4[0], out4[1], out4[2]) == (sn.cx, sn.cy, sn.spacing)
+        assert tuple(out3) == (sn.lo, sn.hi, sn.index)
+        pts = np.ascontiguousarray(np.concatenate([lo, hi]))
+        c, s = np.zeros(3), np.zeros(1)
+        hm.wbh_size_fit(pts.ctypes.data_as(D), 2, c.ctypes.data_as(D), s.ctypes.data_as(D))
+        oc, os_ = O.sizefit(lo, hi)
+        assert np.array_equal(c, oc) and s[0] == os_

@@ -57,7 +57,8 @@ class Stats(C.Structure):
                 ("nleaves", C.c_uint64), ("nnodes", C.c_uint64),
                 ("ms_build", C.c_double), ("ms_sort", C.c_double), ("ms_tiles", C.c_double),
                 ("ms_scan", C.c_double), ("ms_classify", C.c_double),
-                ("ms_classify_kernel", C.c_double), ("ms_total", C.c_double)]
+                ("ms_classify_kernel", C.c_double), ("ms_heavy_kernel", C.c_double),
+                ("ms_total", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

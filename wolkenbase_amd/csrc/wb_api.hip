@@ -378,6 +378,7 @@ int wb_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
     stats->ms_build = t->ms_build;
     stats->ms_sort = t->ms_sort;
     stats->ms_classify_kernel = co.ms_classify_kernel;
+    stats->ms_heavy_kernel = co.ms_heavy_kernel;
     stats->ms_tiles = co.ms_tiles;
     stats->ms_scan = co.ms_scan;
     stats->ms_classify = co.ms_classify;

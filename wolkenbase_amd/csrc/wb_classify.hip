@@ -181,7 +181,21 @@ struct TileGeo {
   double scx, scy, spacing, r;
   int lo, hi;
   uint32_t ntiles;
+  // dense Eisenstein-address -> counter index map over the island's box
+  int gx0, gy0, gw, gh;
+  const int32_t *grid;
 };
+
+// counter of the tile at Eisenstein (ex, ey), false if it is not a tile
+__device__ inline bool tile_at(const TileGeo &g, int ex, int ey, int *cnt)
+{
+  int gx = ex - g.gx0, gy = ey - g.gy0;
+  if (gx < 0 || gy < 0 || gx >= g.gw || gy >= g.gh) return false;
+  int k = g.grid[(int64_t)gy * g.gw + gx];
+  if (k < 0) return false;
+  *cnt = g.lo + k;
+  return true;
+}
 
 // Flowsnake::cyl, flowsnake.cpp:263-271
 __device__ inline void tile_center(const TileGeo &g, int ex, int ey, double *cx, double *cy)
@@ -193,14 +207,42 @@ __device__ inline void tile_center(const TileGeo &g, int ex, int ey, double *cx,
   *cy = zi + g.scy;
 }
 
-__global__ void k_tile_geo(TileGeo g, int32_t *tex, int32_t *tey)
+__global__ void k_tile_geo(TileGeo g, int32_t *tex, int32_t *tey, int *bounds)
+{
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  int ex = 0, ey = 0;
+  bool ok = k < g.ntiles;
+  if (ok) {
+    wb::to_flowsnake_dev((int64_t)g.lo + k, &ex, &ey);
+    tex[k] = ex;
+    tey[k] = ey;
+  }
+  int x0 = ok ? ex : INT32_MAX, x1 = ok ? ex : INT32_MIN, y0 = ok ? ey : INT32_MAX, y1 = ok ? ey : INT32_MIN;
+  for (int s = 1; s < 64; s <<= 1) {
+    x0 = min(x0, __shfl_xor(x0, s, 64));
+    x1 = max(x1, __shfl_xor(x1, s, 64));
+    y0 = min(y0, __shfl_xor(y0, s, 64));
+    y1 = max(y1, __shfl_xor(y1, s, 64));
+  }
+  if (__lane_id() == 0) {
+    atomicMin(&bounds[0], x0);
+    atomicMax(&bounds[1], x1);
+    atomicMin(&bounds[2], y0);
+    atomicMax(&bounds[3], y1);
+  }
+}
+
+__global__ void k_fill_i32(int32_t *p, uint64_t n, int32_t v)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_tile_grid(TileGeo g, const int32_t *tex, const int32_t *tey, int32_t *grid)
 {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= g.ntiles) return;
-  int ex, ey;
-  wb::to_flowsnake_dev((int64_t)g.lo + k, &ex, &ey);
-  tex[k] = ex;
-  tey[k] = ey;
+  grid[(int64_t)(tey[k] - g.gy0) * g.gw + (tex[k] - g.gx0)] = (int32_t)k;
 }
 
 #define MAX_MEMB 4
@@ -234,7 +276,7 @@ __global__ void k_membership(TreeView T, TileGeo g, int32_t *memb, uint32_t *nme
       double dx = cx - x, dy = cy - y;
       if (dx * dx + dy * dy > lim * lim) continue;
       int cnt;
-      if (!wb::from_flowsnake_dev(ex, ey, g.lo, g.hi, &cnt)) continue;
+      if (!tile_at(g, ex, ey, &cnt)) continue;
       wb::Shape cyl = wb::make_cylinder(cx, cy, g.r);
       bool in = wb::shape_in_cube(cyl, lc[0], lc[1], lc[2], lc[3]) || wb::cyl_in(cyl, x, y);
       if (!in) continue;
@@ -400,7 +442,6 @@ __global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_
                                              unsigned long long *ctr)
 {
   __shared__ uint32_t cp[SCAN_CAP];
-  __shared__ double vx[SCAN_CAP], vy[SCAN_CAP], vz[SCAN_CAP];
   __shared__ int histo[7];
   const uint32_t t = blockIdx.x;
   if (t >= g.ntiles) return;
@@ -439,39 +480,47 @@ __global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_
   double cx, cy;
   tile_center(g, tex[t], tey[t], &cx, &cy);
   const double r = g.r;
-  for (uint32_t j = lane; j < m; j += WAVE) {
-    uint32_t s = T.canon[cp[j]];
-    vx[j] = T.px[s] - cx;   // location - xyz(cyl centre, 0)
-    vy[j] = T.py[s] - cy;
-    vz[j] = T.pz[s] - 0;
+  // the tile's points relative to the cylinder axis, in canonical order
+  // (location - xyz(cyl centre, 0)); read through the canonical map
+  auto VX = [&](uint32_t j) { return T.px[T.canon[cp[j]]] - cx; };
+  auto VY = [&](uint32_t j) { return T.py[T.canon[cp[j]]] - cy; };
+  auto VZ = [&](uint32_t j) { return T.pz[T.canon[cp[j]]] - 0; };
+  // normal equations of [x y 1] z (transmult and operator*, matrix.cpp:181-215)
+  double sxx = wave_pairwise(m, [&](uint32_t j) { double v = VX(j); return v * v; });
+  double syx = wave_pairwise(m, [&](uint32_t j) { return VY(j) * VX(j); });
+  double syy = wave_pairwise(m, [&](uint32_t j) { double v = VY(j); return v * v; });
+  double s1x = wave_pairwise(m, [&](uint32_t j) { return 1.0 * VX(j); });
+  double s1y = wave_pairwise(m, [&](uint32_t j) { return 1.0 * VY(j); });
+  double s11 = wave_pairwise(m, [&](uint32_t j) { (void)j; return 1.0 * 1.0; });
+  double sxz = wave_pairwise(m, [&](uint32_t j) { return VX(j) * VZ(j); });
+  double syz = wave_pairwise(m, [&](uint32_t j) { return VY(j) * VZ(j); });
+  double s1z = wave_pairwise(m, [&](uint32_t j) { return 1.0 * VZ(j); });
+  // the 3x3 solve runs once, on lane 0, with the matrix in LDS (row swaps
+  // index it at run time)
+  __shared__ double gA[9], gB[3], gS[2];
+  if (lane == 0) {
+    gA[0] = sxx; gA[1] = syx; gA[2] = s1x;
+    gA[3] = syx; gA[4] = syy; gA[5] = s1y;
+    gA[6] = s1x; gA[7] = s1y; gA[8] = s11;
+    gB[0] = sxz; gB[1] = syz; gB[2] = s1z;
+    gausselim3(gA, gB);
+    for (int i = 0; i < 3; i++)
+      if (gA[3 * i + i] == 0) gB[i] = __builtin_nan("");
+    double a = gB[0], b = gB[1];
+    double len = wb::hypot(a, b);
+    if (len > 1) { a /= len; b /= len; }
+    if (wb::disnan(a) || wb::disnan(b)) { a = 0; b = 0; }
+    gS[0] = a;
+    gS[1] = b;
   }
   __syncthreads();
-  // normal equations of [x y 1] z (transmult and operator*, matrix.cpp:181-215)
-  double sxx = wave_pairwise(m, [&](uint32_t j) { return vx[j] * vx[j]; });
-  double syx = wave_pairwise(m, [&](uint32_t j) { return vy[j] * vx[j]; });
-  double syy = wave_pairwise(m, [&](uint32_t j) { return vy[j] * vy[j]; });
-  double s1x = wave_pairwise(m, [&](uint32_t j) { return 1.0 * vx[j]; });
-  double s1y = wave_pairwise(m, [&](uint32_t j) { return 1.0 * vy[j]; });
-  double s11 = wave_pairwise(m, [&](uint32_t j) { (void)j; return 1.0 * 1.0; });
-  double sxz = wave_pairwise(m, [&](uint32_t j) { return vx[j] * vz[j]; });
-  double syz = wave_pairwise(m, [&](uint32_t j) { return vy[j] * vz[j]; });
-  double s1z = wave_pairwise(m, [&](uint32_t j) { return 1.0 * vz[j]; });
-  double A[9] = {sxx, syx, s1x, syx, syy, s1y, s1x, s1y, s11};
-  double B[3] = {sxz, syz, s1z};
-  gausselim3(A, B);
-  for (int i = 0; i < 3; i++)
-    if (A[3 * i + i] == 0) B[i] = __builtin_nan("");
-  double sx = B[0], sy = B[1];
-  {
-    double len = wb::hypot(sx, sy);
-    if (len > 1) { sx /= len; sy /= len; }
-  }
-  if (wb::disnan(sx) || wb::disnan(sy)) { sx = 0; sy = 0; }
-  // untilt; bottom, bottom2 (the running minimum before its last update), top
+  const double sx = gS[0], sy = gS[1];
+  // untilted height; bottom, bottom2 (the running minimum before its last
+  // update), top
+  auto U = [&](uint32_t j) { return VZ(j) - (sy * VY(j) + sx * VX(j)); };
   double bottom = __builtin_inf(), top = -__builtin_inf();
   for (uint32_t j = lane; j < m; j += WAVE) {
-    double u = vz[j] - (sy * vy[j] + sx * vx[j]);
-    vz[j] = u;
+    double u = U(j);
     bottom = fmin(bottom, u);
     top = fmax(top, u);
   }
@@ -479,27 +528,27 @@ __global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_
     bottom = fmin(bottom, __shfl_xor(bottom, s, WAVE));
     top = fmax(top, __shfl_xor(top, s, WAVE));
   }
-  __syncthreads();
   uint32_t first = 0xffffffffu;
   for (uint32_t j = lane; j < m; j += WAVE)
-    if (vz[j] == bottom && j < first) first = j;
+    if (U(j) == bottom && j < first) first = j;
   for (int s = 1; s < WAVE; s <<= 1) first = min(first, (uint32_t)__shfl_xor((int)first, s, WAVE));
   double bottom2 = __builtin_inf();
-  for (uint32_t j = lane; j < first; j += WAVE) bottom2 = fmin(bottom2, vz[j]);
+  for (uint32_t j = lane; j < first; j += WAVE) bottom2 = fmin(bottom2, U(j));
   for (int s = 1; s < WAVE; s <<= 1) bottom2 = fmin(bottom2, __shfl_xor(bottom2, s, WAVE));
   if (wb::disinf(bottom2)) bottom2 = bottom;
   // 7-sector histogram of the bottom layer
   uint32_t nbottom = 0;
   for (uint32_t j = lane; j < m; j += WAVE) {
-    if (!(vz[j] < bottom2 + 2 * r)) continue;
+    if (!(U(j) < bottom2 + 2 * r)) continue;
     nbottom++;
-    double th = atan2(vy[j], vx[j]) * 3 / M_PI;
+    const double vx = VX(j), vy = VY(j);
+    double th = atan2(vy, vx) * 3 / M_PI;
     long sec = lrint(th);
     double fr = th - floor(th);
     if (wb::dabs(fr - 0.5) < 1e-12) atomicAdd(&ctr[0], 1ull);
     if (sec < 0) sec += 6;
     sec = (sec % 6) + 1;
-    if (wb::hypot(vx[j], vy[j]) < r / kSqrt7) sec = 0;
+    if (wb::hypot(vx, vy) < r / kSqrt7) sec = 0;
     atomicAdd(&histo[sec], 1);
   }
   for (int s = 1; s < WAVE; s <<= 1) nbottom += __shfl_xor((int)nbottom, s, WAVE);
@@ -541,7 +590,7 @@ __global__ void k_postscan(TileGeo g, const int32_t *tex, const int32_t *tey, Ti
     ring = nontree = 0;
     for (int j = 0; j < 6 && (me.tree_flags & 1); j++) {
       int o;
-      if (!wb::from_flowsnake_dev(tex[t] + r1x[j] * i, tey[t] + r1y[j] * i, g.lo, g.hi, &o)) continue;
+      if (!tile_at(g, tex[t] + r1x[j] * i, tey[t] + r1y[j] * i, &o)) continue;
       const TileRec &q = rec[o - g.lo];
       if (q.npoints) {
         ring++;
@@ -693,7 +742,7 @@ __device__ inline void wave_walk(const TreeView &T, const wb::Shape &H, double x
 #define HEAVY_B_GROUPS 256u
 
 template <bool COUNT>
-__global__ void __launch_bounds__(256) k_classify(ClassifyArgs A)
+__global__ void __launch_bounds__(256, 6) k_classify(ClassifyArgs A)
 {
   WalkCount wa = {0, 0, 0, 0, 0}, wb_ = {0, 0, 0, 0, 0}, wcc = {0, 0, 0, 0, 0};
   uint32_t ndir = 0;
@@ -970,22 +1019,42 @@ __global__ void __launch_bounds__(64) k_classify_heavy(ClassifyArgs A, const uin
   }
 }
 
-// class histogram (wave-aggregated) and, when tile records are wanted, the
-// number of ground points each deciding tile classified
-__global__ void k_count_ground(uint64_t n, const uint8_t *cls, const int32_t *memb,
-                               const uint32_t *nmemb, int lo, int32_t *nground,
-                               unsigned long long *hist3, int per_tile)
+// class histogram (per-block partials, summed by k_sum_hist) and, when tile
+// records are wanted, the number of ground points each deciding tile
+// classified
+__global__ void __launch_bounds__(256) k_count_ground(uint64_t n, const uint8_t *cls,
+                                                       const int32_t *memb, const uint32_t *nmemb,
+                                                       int lo, int32_t *nground, uint32_t *partial,
+                                                       int per_tile)
 {
+  __shared__ uint32_t h[3];
+  if (threadIdx.x < 3) h[threadIdx.x] = 0;
+  __syncthreads();
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint8_t c = i < n ? cls[i] : 255;
   for (int k = 0; k < 3; k++) {
     uint64_t b = __ballot(c == k);
-    if (__lane_id() == 0 && b) atomicAdd(&hist3[k], (unsigned long long)__popcll(b));
+    if (__lane_id() == 0 && b) atomicAdd(&h[k], (uint32_t)__popcll(b));
   }
   if (per_tile && c == 2) {
     int w = INT32_MIN;
     for (uint32_t k = 0; k < nmemb[i]; k++) w = max(w, memb[MAX_MEMB * i + k]);
     atomicAdd(&nground[w - lo], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) partial[3 * (uint64_t)blockIdx.x + threadIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(256) k_sum_hist(const uint32_t *partial, uint32_t nblocks,
+                                                   unsigned long long *hist3)
+{
+  uint64_t s[3] = {0, 0, 0};
+  for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x)
+    for (int k = 0; k < 3; k++) s[k] += partial[3 * (uint64_t)b + k];
+  for (int k = 0; k < 3; k++) {
+    uint64_t v = s[k];
+    for (int sh = 1; sh < 64; sh <<= 1) v += __shfl_xor(v, sh, 64);
+    if (__lane_id() == 0) atomicAdd(&hist3[k], (unsigned long long)v);
   }
 }
 
@@ -1019,8 +1088,8 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   g.scx = ss.cx; g.scy = ss.cy; g.spacing = ss.spacing; g.r = ss.radius;
   g.lo = ss.lo; g.hi = ss.hi;
   g.ntiles = (uint32_t)((int64_t)ss.hi - ss.lo + 1);
-  hipEvent_t ev[6];
-  for (int k = 0; k < 6; k++) WB_CHECK(hipEventCreate(&ev[k]));
+  hipEvent_t ev[8];
+  for (int k = 0; k < 8; k++) WB_CHECK(hipEventCreate(&ev[k]));
   WB_CHECK(hipEventRecord(ev[0], st));
   int32_t *tex = (int32_t *)wb_alloc(c, 4ull * g.ntiles), *tey = (int32_t *)wb_alloc(c, 4ull * g.ntiles);
   uint32_t *tile_cnt = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
@@ -1035,7 +1104,22 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   WB_CHECK(hipMemsetAsync(tile_cnt, 0, 4ull * g.ntiles, st));
   WB_CHECK(hipMemsetAsync(tile_fill, 0, 4ull * g.ntiles, st));
   TreeView T = view(t);
-  hipLaunchKernelGGL(k_tile_geo, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey);
+  int *dbounds = (int *)wb_alloc(c, 16);
+  int hb[4] = {INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN};
+  WB_CHECK(hipMemcpyAsync(dbounds, hb, 16, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_tile_geo, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey, dbounds);
+  WB_CHECK(hipMemcpyAsync(hb, dbounds, 16, hipMemcpyDeviceToHost, st));
+  WB_CHECK(hipStreamSynchronize(st));
+  wb_free(c, dbounds, 16);
+  g.gx0 = hb[0];
+  g.gy0 = hb[2];
+  g.gw = hb[1] - hb[0] + 1;
+  g.gh = hb[3] - hb[2] + 1;
+  const uint64_t gcells = (uint64_t)g.gw * (uint64_t)g.gh;
+  int32_t *grid = (int32_t *)wb_alloc(c, 4 * gcells);
+  hipLaunchKernelGGL(k_fill_i32, dim3(wb_blocks(gcells, 256)), dim3(256), 0, st, grid, gcells, -1);
+  g.grid = grid;
+  hipLaunchKernelGGL(k_tile_grid, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey, grid);
   if (n > 0)
     hipLaunchKernelGGL(k_membership, dim3(wb_blocks(n, 256)), dim3(256), 0, st, T, g, memb, nmemb,
                        tile_cnt, err);
@@ -1076,19 +1160,25 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
     else
       hipLaunchKernelGGL(k_classify<false>, dim3(wb_blocks(n, 256)), dim3(256), 0, st, A);
   }
+  WB_CHECK(hipEventRecord(ev[5], st));
   uint32_t h_nheavy = 0;
   WB_CHECK(hipMemcpyAsync(&h_nheavy, nslow + 1, 4, hipMemcpyDeviceToHost, st));
   WB_CHECK(hipStreamSynchronize(st));
+  WB_CHECK(hipEventRecord(ev[6], st));
   if (h_nheavy)
     hipLaunchKernelGGL(k_classify_heavy, dim3(h_nheavy), dim3(64), 0, st, A, heavy, h_nheavy);
-  WB_CHECK(hipEventRecord(ev[5], st));
+  WB_CHECK(hipEventRecord(ev[7], st));
   uint32_t h_nslow = 0;
   WB_CHECK(hipMemcpyAsync(&h_nslow, nslow, 4, hipMemcpyDeviceToHost, st));
   int32_t *nground = (int32_t *)tile_fill;  // reuse
   WB_CHECK(hipMemsetAsync(nground, 0, 4ull * g.ntiles, st));
   if (n > 0) {
-    hipLaunchKernelGGL(k_count_ground, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, cls, memb,
-                       nmemb, g.lo, nground, ctr + 2, want_tiles ? 1 : 0);
+    const unsigned nbk = wb_blocks(n, 256);
+    uint32_t *partial = (uint32_t *)wb_alloc(c, 12ull * nbk);
+    hipLaunchKernelGGL(k_count_ground, dim3(nbk), dim3(256), 0, st, n, cls, memb, nmemb, g.lo,
+                       nground, partial, want_tiles ? 1 : 0);
+    hipLaunchKernelGGL(k_sum_hist, dim3(1), dim3(256), 0, st, partial, nbk, ctr + 2);
+    wb_free(c, partial, 12ull * nbk);
     hipLaunchKernelGGL(k_scatter_class, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, cls, t->sidx,
                        d_class_out);
   }
@@ -1115,7 +1205,10 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   float ms4 = 0;
   hipEventElapsedTime(&ms4, ev[4], ev[5]);
   out->ms_classify_kernel = ms4;
-  for (int k = 0; k < 6; k++) hipEventDestroy(ev[k]);
+  float ms5 = 0;
+  hipEventElapsedTime(&ms5, ev[6], ev[7]);
+  out->ms_heavy_kernel = ms5;
+  for (int k = 0; k < 8; k++) hipEventDestroy(ev[k]);
   unsigned long long *hc = (unsigned long long *)(h_err + 2);
   out->borderline = hc[0];
   out->atan_exact = hc[1];
@@ -1136,6 +1229,7 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   }
   out->nonempty = hc[5];
   out->ms_tiles = ms1; out->ms_scan = ms2; out->ms_classify = ms3;
+  wb_free(c, grid, 4 * gcells);
   wb_free(c, tex, 4ull * g.ntiles);
   wb_free(c, tey, 4ull * g.ntiles);
   wb_free(c, tile_cnt, 4ull * g.ntiles + 4);

@@ -95,7 +95,7 @@ struct ClassifyOut {
   std::vector<TileRec> tiles;
   std::vector<int32_t> tex, tey;
   uint64_t borderline, atan_exact, nslow, hist[3], nonempty, nheavy;
-  double ms_tiles, ms_scan, ms_classify, ms_classify_kernel;
+  double ms_tiles, ms_scan, ms_classify, ms_classify_kernel, ms_heavy_kernel;
 };
 
 // ---- implemented in wb_classify.hip ----
