@@ -99,6 +99,33 @@ def test_atan2i_exact_path_matches_oracle(hm):
         assert hm.wbh_atan2i_forced(y, x) == L.wbo_atan2i(y, x)
 
 
+def exact_bin(a):
+    return (((a + 0x40000000) & 0xffffffff) & 0x7fffffff) >> 25
+
+
+def test_dir_bin_matches_exact_bin(hm):
+    hm.wbh_dir_bin.restype = C.c_int
+    hm.wbh_dir_bin.argtypes = [C.c_double, C.c_double]
+    L = O.lib()
+    rng = np.random.default_rng(19)
+    n = 100000
+    # random, LAS-like, axis-aligned, and directions placed on bin edges
+    edges = (np.arange(64) * 2 ** 25 - 2 ** 30)[rng.integers(0, 64, n)] + rng.integers(-70000, 70000, n)
+    ang = edges / 2 ** 31 * 2 * np.pi
+    r = rng.uniform(0.001, 500, n)
+    ys = np.concatenate([rng.normal(size=n), rng.integers(-4000, 4000, n) * 0.001, r * np.sin(ang),
+                         np.zeros(10), rng.normal(size=10)])
+    xs = np.concatenate([rng.normal(size=n), rng.integers(-4000, 4000, n) * 0.001, r * np.cos(ang),
+                         rng.normal(size=10), np.zeros(10)])
+    bad = 0
+    for y, x in zip(ys, xs):
+        if x == 0 and y == 0:
+            continue
+        if hm.wbh_dir_bin(y, x) != exact_bin(L.wbo_atan2i(y, x)):
+            bad += 1
+    assert bad == 0
+
+
 def test_x87_emulation_ops(hm):
     rng = np.random.default_rng(14)
     for _ in range(40000):

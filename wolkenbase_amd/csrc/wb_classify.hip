@@ -34,6 +34,10 @@
 #include "../../include/wb_gpu.h"
 
 #define WAVE 64
+// sampled pass: every SAMPLE_ALL-th point of leaves inside the hyperboloid
+#ifndef SAMPLE_ALL
+#define SAMPLE_ALL 32u
+#endif
 constexpr double kSqrt34 = 0.86602540378443864676372317;  // eisenstein.h:33
 constexpr double kSqrt7 = 2.6457513110645905905016;       // scan.h:23
 
@@ -720,12 +724,19 @@ __device__ inline void wave_walk(const TreeView &T, const wb::Shape &H, double x
       if (COUNT && hit) wc.groups++;
       if (chit) wc.groups_scanned++;
       if (__ballot(chit)) {
-        const uint32_t jend = SAMPLE ? c0 + 1 : ce;
-        if (COUNT && chit) wc.points += jend - c0;
-        for (uint32_t j = c0; j < jend; j++) {
+        // the sampled pass reads one point per group, and every
+        // SAMPLE_ALL-th point of groups of leaves entirely inside the lane's
+        // hyperboloid (a leaf under the point surrounds it)
+        const uint32_t stride = SAMPLE ? (__ballot(chit && all) ? SAMPLE_ALL : WB_CHUNK) : 1;
+        if (COUNT && chit) wc.points += (ce - c0 + stride - 1) / stride;
+        for (uint32_t j = c0; j < ce; j += stride) {
           const double qx = T.px[j], qy = T.py[j];
-          if (chit && (all || wb::hyp_in(H, qx, qy, T.pz[j])) && (qx != x || qy != y))
-            on_dir(qy - y, qx - x);
+          bool take = false;
+          if (chit) {
+            if (all) take = true;
+            else if (!SAMPLE || j == c0) take = wb::hyp_in(H, qx, qy, T.pz[j]);
+          }
+          if (take && (qx != x || qy != y)) on_dir(qy - y, qx - x);
         }
       }
       c0 = ce;
@@ -741,8 +752,8 @@ __device__ inline void wave_walk(const TreeView &T, const wb::Shape &H, double x
 #define HEAVY_A_GROUPS 2048u
 #define HEAVY_B_GROUPS 256u
 
-template <bool COUNT>
-__global__ void __launch_bounds__(256, 6) k_classify(ClassifyArgs A)
+template <bool COUNT, int MINW>
+__global__ void __launch_bounds__(256, MINW) k_classify(ClassifyArgs A)
 {
   WalkCount wa = {0, 0, 0, 0, 0}, wb_ = {0, 0, 0, 0, 0}, wcc = {0, 0, 0, 0, 0};
   uint32_t ndir = 0;
@@ -769,8 +780,7 @@ __global__ void __launch_bounds__(256, 6) k_classify(ClassifyArgs A)
   int nexact = 0;
   auto add_dir = [&](double dy, double dx) {
     if (COUNT) ndir++;
-    int a = wb::atan2i(dy, dx, ttan, tcos, tsin, A.K, &nexact);
-    occ |= 1ull << (dir_bin_pos(a) >> 25);
+    occ |= 1ull << wb::dir_bin(dy, dx, ttan, tcos, tsin, A.K, &nexact);
   };
   auto after = [&]() {
     if (state == 0 && occ != checked) {
@@ -822,8 +832,9 @@ __global__ void __launch_bounds__(256, 6) k_classify(ClassifyArgs A)
       }
     }
     auto track = [&](double dy, double dx) {
-      int a = wb::atan2i(dy, dx, ttan, tcos, tsin, A.K, &nexact);
-      int pos = dir_bin_pos(a), bin = pos >> 25;
+      const int bin = wb::dir_bin(dy, dx, ttan, tcos, tsin, A.K, &nexact);
+      if (bin != before[0] && bin != afterb[0] && bin != before[1] && bin != afterb[1]) return;
+      const int pos = dir_bin_pos(wb::atan2i(dy, dx, ttan, tcos, tsin, A.K, &nexact));
       for (int k = 0; k < 2; k++) {
         if (bin == before[k] && pos > hi[k]) hi[k] = pos;
         if (bin == afterb[k] && pos < lo[k]) lo[k] = pos;
@@ -961,8 +972,7 @@ __global__ void __launch_bounds__(64) k_classify_heavy(ClassifyArgs A, const uin
   uint64_t occ = 0, checked = 0;
   int verdict = 0;
   auto add_dir = [&](double dy, double dx) {
-    int a = wb::atan2i(dy, dx, ttan, tcos, tsin, A.K, &nexact);
-    occ |= 1ull << (dir_bin_pos(a) >> 25);
+    occ |= 1ull << wb::dir_bin(dy, dx, ttan, tcos, tsin, A.K, &nexact);
   };
   auto after = [&]() {
     uint64_t all = wave_or(occ);
@@ -991,8 +1001,9 @@ __global__ void __launch_bounds__(64) k_classify_heavy(ClassifyArgs A, const uin
     }
     int hi[2] = {-1, -1}, lo[2] = {0x7fffffff, 0x7fffffff};
     auto track = [&](double dy, double dx) {
-      int a = wb::atan2i(dy, dx, ttan, tcos, tsin, A.K, &nexact);
-      int pos = dir_bin_pos(a), bin = pos >> 25;
+      const int bin = wb::dir_bin(dy, dx, ttan, tcos, tsin, A.K, &nexact);
+      if (bin != before[0] && bin != afterb[0] && bin != before[1] && bin != afterb[1]) return;
+      const int pos = dir_bin_pos(wb::atan2i(dy, dx, ttan, tcos, tsin, A.K, &nexact));
       for (int k = 0; k < 2; k++) {
         if (bin == before[k] && pos > hi[k]) hi[k] = pos;
         if (bin == afterb[k] && pos < lo[k]) lo[k] = pos;
@@ -1156,9 +1167,15 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   A.nheavy = nslow + 1;
   if (n > 0) {
     if (count_work)
-      hipLaunchKernelGGL(k_classify<true>, dim3(wb_blocks(n, 256)), dim3(256), 0, st, A);
+      hipLaunchKernelGGL((k_classify<true, 6>), dim3(wb_blocks(n, 256)), dim3(256), 0, st, A);
     else
-      hipLaunchKernelGGL(k_classify<false>, dim3(wb_blocks(n, 256)), dim3(256), 0, st, A);
+    {
+      static const int occ8 = getenv("WB_CLASSIFY_OCC8") != nullptr;
+      if (occ8)
+        hipLaunchKernelGGL((k_classify<false, 8>), dim3(wb_blocks(n, 256)), dim3(256), 0, st, A);
+      else
+        hipLaunchKernelGGL((k_classify<false, 6>), dim3(wb_blocks(n, 256)), dim3(256), 0, st, A);
+    }
   }
   WB_CHECK(hipEventRecord(ev[5], st));
   uint32_t h_nheavy = 0;

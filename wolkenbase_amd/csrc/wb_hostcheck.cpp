@@ -31,6 +31,13 @@ int wbh_atan2i(double y, double x, int *exact_used)
                     g_k, exact_used);
 }
 
+int wbh_dir_bin(double dy, double dx)
+{
+  ready();
+  return wb::dir_bin(dy, dx, (const double *)g_tan, (const double *)g_cos, (const double *)g_sin,
+                     g_k);
+}
+
 int wbh_atan2i_forced(double y, double x)
 {
   ready();

@@ -378,6 +378,37 @@ WB_HD int atan2i(double y, double x, TabPtr tab_tan, TabPtr tab_cos, TabPtr tab_
 }
 
 // ---------------------------------------------------------------------------
+// 64-bin direction index of atan2i(dy, dx): the bin of the circle position
+// (a + 2^30) mod 2^31, 2^25 units per bin.  A float atan2 decides it unless
+// the direction lies within DIR_EPS units of a bin edge.  The float path's
+// error is bounded by the two float conversions of dy and dx (2^-24 relative
+// each, at most 2^-23 rad), the atan2f result (a few ulp of at most pi) and
+// the scaling to binary units -- together well under 2^10 units against
+// DIR_EPS = 2^16 -- and atan2i itself is within one unit of the true angle,
+// so outside the edge band both give the same bin.  Inside it, or for
+// operands a float cannot carry, atan2i decides.
+// ---------------------------------------------------------------------------
+constexpr int DIR_EPS = 1 << 16;
+
+template <typename TabPtr>
+WB_HD int dir_bin(double dy, double dx, TabPtr tab_tan, TabPtr tab_cos, TabPtr tab_sin,
+                  const AtanConst &K, int *exact_used = nullptr)
+{
+  const double m = dabs(dy) > dabs(dx) ? dabs(dy) : dabs(dx);
+  if (m > 1e-30 && m < 1e30) {
+    const float t = atan2f((float)dy, (float)dx);          // [-pi, pi]
+    double pos = (double)t * (1073741824.0 / 3.14159265358979323846) + 1073741824.0;
+    if (pos >= 2147483648.0) pos -= 2147483648.0;
+    if (pos < 0) pos += 2147483648.0;
+    const int64_t ip = (int64_t)pos;
+    const int64_t frac = ip & ((1 << 25) - 1);
+    if (frac > DIR_EPS && frac < (1 << 25) - DIR_EPS) return (int)((ip >> 25) & 63);
+  }
+  const int a = atan2i(dy, dx, tab_tan, tab_cos, tab_sin, K, exact_used);
+  return (int)((((uint32_t)a + 0x40000000u) & 0x7fffffffu) >> 25);
+}
+
+// ---------------------------------------------------------------------------
 // Shapes (shape.cpp).  Kinds follow the C ABI: 0 cylinder, 1 sphere,
 // 2 hyperboloid.
 // ---------------------------------------------------------------------------
