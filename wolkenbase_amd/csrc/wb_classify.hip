@@ -34,9 +34,9 @@
 #include "../../include/wb_gpu.h"
 
 #define WAVE 64
-// sampled pass: every SAMPLE_ALL-th point of leaves inside the hyperboloid
-#ifndef SAMPLE_ALL
-#define SAMPLE_ALL 32u
+// waves per SIMD the ground-test kernel is compiled for
+#ifndef WB_GROUND_MINW
+#define WB_GROUND_MINW 6
 #endif
 constexpr double kSqrt34 = 0.86602540378443864676372317;  // eisenstein.h:33
 constexpr double kSqrt7 = 2.6457513110645905905016;       // scan.h:23
@@ -647,125 +647,124 @@ __device__ inline int gap_verdict(uint64_t occupied)
 
 __device__ inline int dir_bin_pos(int a) { return (int)(((uint32_t)a + 0x40000000u) & 0x7fffffffu); }
 
-// ---------------------------------------------------------------------------
-// Wave-cooperative findBlocks walk.  The 64 lanes of a wave hold 64
-// consecutive points of the sorted order (spatial neighbours), each with its
-// own hyperboloid.  The walk itself is uniform: a node is entered when any
-// active lane's hyperboloid intersects it.  Each lane keeps a bit per depth
-// saying whether every ancestor so far intersected *its* hyperboloid, so a
-// lane processes exactly the leaves the reference's findBlocks returns for
-// its own shape.  Leaf points are read with wave-uniform addresses.
-//   SAMPLE: only the first point of each 32-point group is considered (a
-//   subset of D: a pass on it is a pass on D).
-// on_dir(dy, dx) is called for every q in pointsIn(H) with xy(q) != xy(p);
-// after_leaf() returns the lane's "still active" flag.
-// ---------------------------------------------------------------------------
-// per-lane work counters: leaves entered, groups tested, groups scanned,
-// points tested, and walk steps (uniform)
-struct WalkCount {
-  uint32_t leaves, groups, groups_scanned, points, steps;
-};
-
-template <bool SAMPLE, bool COUNT, class OnDir, class AfterLeaf>
-__device__ inline void wave_walk(const TreeView &T, const wb::Shape &H, double x, double y,
-                                 bool active, OnDir &&on_dir, AfterLeaf &&after_leaf,
-                                 WalkCount &wc)
+__device__ inline uint64_t wave_or(uint64_t v)
 {
-  if (__ballot(active) == 0 || T.nleaves == 0) return;
-  uint32_t okm = (active && wb::shape_intersect(H, T.node_cube[0], T.node_cube[1],
-                                                 T.node_cube[2], T.node_cube[3])) ? 1u : 0u;
-  if (__ballot(okm != 0) == 0) return;
-  int node = 0, depth = 0;
-  Iter it = {0, 0};
-  for (;;) {
-    if (COUNT) wc.steps++;
-    int c = it.get(depth);
-    if (c == 8) {
-      if (depth == 0) return;
-      node = T.node_parent[node];
-      okm &= ~(1u << depth);
-      depth--;
-      continue;
-    }
-    it.set(depth, c + 1);
-    const bool on = active && ((okm >> depth) & 1u);
-    if (__ballot(on) == 0) {  // nobody below this node any more
-      it.set(depth, 8);
-      continue;
-    }
-    int ch = T.node_child[8 * node + c];
-    if (ch == -1) continue;
-    if (ch >= 0) {
-      const double *q = T.node_cube + 4 * ch;
-      bool h = on && wb::shape_intersect(H, q[0], q[1], q[2], q[3]);
-      if (__ballot(h)) {
-        node = ch;
-        depth++;
-        if (h) okm |= 1u << depth;
-        it.set(depth, 0);
-      }
-      continue;
-    }
-    const int leaf = -2 - ch;
-    const double *lc = T.leaf_cube + 4 * leaf;
-    const bool hit = on && wb::shape_intersect(H, lc[0], lc[1], lc[2], lc[3]);
-    if (__ballot(hit) == 0) continue;
-    const bool all = hit && wb::shape_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
-    const uint32_t b = T.leaf_start[leaf], e = T.leaf_start[leaf + 1];
-    if (COUNT && hit) wc.leaves++;
-    for (uint32_t c0 = b; c0 < e;) {
-      const uint32_t chk = c0 / WB_CHUNK;
-      const uint32_t ce = min(e, (chk + 1) * WB_CHUNK);
-      bool chit = hit;
-      if (chit && !all) {
-        const double *bx = T.chunk_box + 6 * (uint64_t)chk;
-        chit = wb::shape_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
-      }
-      if (COUNT && hit) wc.groups++;
-      if (chit) wc.groups_scanned++;
-      if (__ballot(chit)) {
-        // the sampled pass reads one point per group, and every
-        // SAMPLE_ALL-th point of groups of leaves entirely inside the lane's
-        // hyperboloid (a leaf under the point surrounds it)
-        const uint32_t stride = SAMPLE ? (__ballot(chit && all) ? SAMPLE_ALL : WB_CHUNK) : 1;
-        if (COUNT && chit) wc.points += (ce - c0 + stride - 1) / stride;
-        for (uint32_t j = c0; j < ce; j += stride) {
-          const double qx = T.px[j], qy = T.py[j];
-          bool take = false;
-          if (chit) {
-            if (all) take = true;
-            else if (!SAMPLE || j == c0) take = wb::hyp_in(H, qx, qy, T.pz[j]);
-          }
-          if (take && (qx != x || qy != y)) on_dir(qy - y, qx - x);
-        }
-      }
-      c0 = ce;
-    }
-    active = after_leaf();
-    if (__ballot(active) == 0) return;
-  }
+  for (int s = 1; s < 64; s <<= 1) v |= __shfl_xor(v, s, 64);
+  return v;
 }
 
-// a lane whose sampled pass scans more than HEAVY_B_GROUPS 32-point groups
-// (about 32x as many point tests in the full pass), or whose sampled pass
-// alone exceeds HEAVY_A_GROUPS, is finished by k_classify_heavy
-#define HEAVY_A_GROUPS 2048u
-#define HEAVY_B_GROUPS 256u
-
-template <bool COUNT, int MINW>
-__global__ void __launch_bounds__(256, MINW) k_classify(ClassifyArgs A)
+__device__ inline int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline uint64_t rfl64(uint64_t v)
 {
-  WalkCount wa = {0, 0, 0, 0, 0}, wb_ = {0, 0, 0, 0, 0}, wcc = {0, 0, 0, 0, 0};
-  uint32_t ndir = 0;
-  const uint64_t t_start = COUNT ? wall_clock64() : 0;
-  __shared__ double tabs[511 + 512 + 512];
-  for (int k = threadIdx.x; k < 511 + 512 + 512; k += blockDim.x) tabs[k] = A.tab[k];
-  __syncthreads();
+  return ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
+         (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+}
+
+// ---------------------------------------------------------------------------
+// Ground-test walk: findBlocks (octree.cpp:234-251) for 64 hyperboloids at
+// once, with an explicit stack.  The 64 lanes of a wave hold 64 consecutive
+// points of the sorted order (spatial neighbours), each with its own
+// hyperboloid.  A stack entry is a node or leaf code and the mask of lanes
+// for which it and every ancestor intersect the lane's hyperboloid, so each
+// lane sees exactly the leaves findBlocks returns for its own shape.
+// Expanding a node tests its eight children at once; their cubes follow from
+// the parent's by Octree::cube (octree.cpp:349-358), the operations the
+// build used, so no child cube is loaded.
+// Visiting order: the children of a node are taken starting with the xy
+// quadrant that holds the wave's first point, then its x, y and diagonal
+// neighbours, the lower half (where a downward hyperboloid lies) first.
+// Leaves near the point in every direction come early, which lets the early
+// exit of the ground test come early.  The order changes no result: a pass
+// on a subset of D is a pass on D, and a fail or an exact gap needs all of D.
+// ---------------------------------------------------------------------------
+// Entries pending at one time: at most 7 siblings per level on the path
+// (internal nodes sit at levels 0..WB_MAX_LEVEL) plus 8 children.
+#define WB_STK (7 * (WB_MAX_LEVEL + 1) + 8 + 2)
+struct WaveStack {
+  int code[WB_STK];
+  uint32_t lo[WB_STK], hi[WB_STK];
+};
+
+// rank of child k in the visiting order for quadrant o and lower-first flag
+__device__ inline int child_rank(int k, int o, int zf) { return ((k ^ o) & 3) | ((k ^ zf) & 4); }
+
+// Pushes the children of internal node `code` that intersect the
+// hyperboloid of a lane in `m` (this lane takes part iff `on`), in reverse
+// visiting order.  Returns the new stack depth.
+__device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int code, bool on,
+                                  const wb::Shape &H, double rx, double ry, int zf)
+{
+  const double *nc = T.node_cube + 4 * (size_t)code;
+  const double cx = nc[0], cy = nc[1], cz = nc[2], side = nc[3];
+  const int *chp = T.node_child + 8 * (size_t)code;
+  const int o = rfl((rx >= cx ? 1 : 0) | (ry >= cy ? 2 : 0));
+  int ck[8];
+  uint64_t mk[8];
+  uint32_t pushed = 0;  // bit t: the child of rank t is pushed
+#pragma unroll
+  for (int k = 0; k < 8; k++) ck[k] = rfl(chp[k]);
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    mk[k] = 0;
+    if (ck[k] != -1) {
+      const double ncx = cx + (double)(2 * (k & 1) - 1) * side / 4;
+      const double ncy = cy + (double)(2 * ((k & 2) / 2) - 1) * side / 4;
+      const double ncz = cz + (double)(2 * ((k & 4) / 4) - 1) * side / 4;
+      const bool h = on && wb::shape_intersect(H, ncx, ncy, ncz, side / 2);
+      mk[k] = __ballot(h);
+      if (mk[k]) pushed |= 1u << child_rank(k, o, zf);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    if (mk[k]) {
+      const int pos = sp + __popc(pushed >> (child_rank(k, o, zf) + 1));
+      S.code[pos] = ck[k];
+      S.lo[pos] = (uint32_t)mk[k];
+      S.hi[pos] = (uint32_t)(mk[k] >> 32);
+    }
+  }
+  return sp + __popc(pushed);
+}
+
+// per-wave work counters of the instrumented kernel
+enum {
+  CT_LANES_A, CT_LANES_B, CT_LANES_C, CT_LEAF_A, CT_GROUP_A, CT_DIR_A, CT_LEAF_B, CT_POINT_B,
+  CT_DIR_B, CT_EXP_A, CT_WLEAF_A, CT_EXP_B, CT_WLEAF_B, CT_WALK_C, CT_HEAVY, CT_WAVES,
+  CT_TSUM, CT_TMAX, CT_THIST, CT_TSUMS = CT_THIST + 5, CT_N = CT_TSUMS + 5
+};
+
+// a lane whose sampled pass enters leaves with more than HEAVY_B_GROUPS
+// 32-point groups, or whose sampled pass alone passes HEAVY_A_GROUPS, is
+// finished by k_ground_heavy
+#ifndef HEAVY_A_GROUPS
+#define HEAVY_A_GROUPS 2048u
+#endif
+#ifndef HEAVY_B_GROUPS
+#define HEAVY_B_GROUPS 256u
+#endif
+
+// The ground test of classify.cpp:128-165 for one point per lane, in three
+// walks over the octree:
+//   A  the first point of each 32-point group of every leaf (a subset of D;
+//      only a pass is decided here),
+//   B  every point, for lanes A left open,
+//   C  for a longest empty run of 24 or 25 bins, the exact extreme
+//      directions in the two bins bordering the run.
+template <bool COUNT, int MINW>
+__global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
+{
+  __shared__ WaveStack stacks[256 / WAVE];
+  WaveStack &S = stacks[threadIdx.x / WAVE];
   const TreeView &T = A.T;
+  const uint32_t lane = __lane_id();
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool inrange = i < T.n;
-  const double *ttan = tabs, *tcos = tabs + 511, *tsin = tabs + 1023;
-  // lane state: 0 undecided, 1 not ground, 2 ground, 3 no tile, 4 ambiguous
+  const double *ttan = A.tab, *tcos = A.tab + 511, *tsin = A.tab + 1023;
+  const uint64_t t_start = COUNT ? wall_clock64() : 0;
+  uint32_t cnt[CT_WAVES] = {};
+  // lane state: 0 undecided, 1 not ground, 2 ground, 3 no tile, 4 ambiguous,
+  // 5 heavy
   int state = 3;
   double x = 0, y = 0, z = 0;
   wb::Shape H = wb::make_cylinder(0, 0, 0);
@@ -776,76 +775,151 @@ __global__ void __launch_bounds__(256, MINW) k_classify(ClassifyArgs A)
     H = wb::make_hyperboloid(x - 0, y - 0, z - A.thickness, A.rec[w - A.g.lo].hs, A.max_slope);
     state = 0;
   }
+  const int zf = A.max_slope > 0 ? 0 : 4;
   uint64_t occ = 0, checked = 0;
   int nexact = 0;
-  auto add_dir = [&](double dy, double dx) {
-    if (COUNT) ndir++;
-    occ |= 1ull << wb::dir_bin(dy, dx, ttan, tcos, tsin, A.K, &nexact);
-  };
-  auto after = [&]() {
-    if (state == 0 && occ != checked) {
-      checked = occ;
-      if (gap_verdict(occ) == 1) state = 1;
-    }
-    if (state == 0 && wa.groups_scanned > HEAVY_A_GROUPS) state = 5;
-    return state == 0;
-  };
-  // phase A: one point per 32-point group; phase B: every point
-  const bool wasA = state == 0;
-  wave_walk<true, COUNT>(T, H, x, y, state == 0, add_dir, after, wa);
-  const uint32_t dirA = ndir;
-  // points whose candidate set is large go to the wave-per-point kernel
-  if (state == 0 && wa.groups_scanned > HEAVY_B_GROUPS) state = 5;
-  {
-    const uint64_t hb = __ballot(state == 5);
-    if (hb) {
-      const int leader = __ffsll((long long)hb) - 1;
-      uint32_t base = 0;
-      if ((int)__lane_id() == leader) base = atomicAdd(A.nheavy, (uint32_t)__popcll(hb));
-      base = __shfl(base, leader, 64);
-      if (state == 5) A.heavy[base + __popcll(hb & ((1ull << __lane_id()) - 1))] = (uint32_t)i;
-    }
-  }
-  const bool inB = state == 0;
-  wave_walk<false, COUNT>(T, H, x, y, state == 0, add_dir, after, wb_);
-  if (state == 0) {
-    int v = gap_verdict(occ);
-    state = v == 0 ? 4 : v;
-  }
-  // phase C: a longest empty run of 24 or 25 bins; record the largest
-  // direction before and the smallest after each such run and compare the
-  // exact gaps with DEG144
-  if (__ballot(state == 4)) {
-    int before[2] = {-1, -1}, afterb[2] = {-1, -1}, nr = 0;
-    int hi[2] = {-1, -1}, lo[2] = {0x7fffffff, 0x7fffffff};
-    if (state == 4) {
-      for (int st = 0; st < 64; st++) {
-        if ((occ >> st) & 1) continue;
-        if (!((occ >> ((st + 63) & 63)) & 1)) continue;
-        int len = 0;
-        while (len < 64 && !((occ >> ((st + len) & 63)) & 1)) len++;
-        if (len >= 24 && nr < 2) {
-          before[nr] = (st + 63) & 63;
-          afterb[nr] = (st + len) & 63;
-          nr++;
+  uint32_t groups_a = 0;
+  int before[2] = {-1, -1}, afterb[2] = {-1, -1}, nr = 0;
+  int hi[2] = {-1, -1}, lo[2] = {0x7fffffff, 0x7fffffff};
+  for (int phase = 0; phase < 3; phase++) {
+    if (phase == 1) {
+      if (state == 0 && groups_a > HEAVY_B_GROUPS) state = 5;
+      const uint64_t hb = __ballot(state == 5);
+      if (hb) {
+        const int leader = __ffsll((long long)hb) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(A.nheavy, (uint32_t)__popcll(hb));
+        base = __shfl(base, leader, 64);
+        if (state == 5) A.heavy[base + __popcll(hb & ((1ull << lane) - 1))] = (uint32_t)i;
+      }
+    } else if (phase == 2) {
+      if (state == 0) {
+        const int v = gap_verdict(occ);
+        state = v == 0 ? 4 : v;
+      }
+      if (state == 4) {
+        for (int st = 0; st < 64; st++) {
+          if ((occ >> st) & 1) continue;
+          if (!((occ >> ((st + 63) & 63)) & 1)) continue;
+          int len = 0;
+          while (len < 64 && !((occ >> ((st + len) & 63)) & 1)) len++;
+          if (len >= 24 && nr < 2) {
+            before[nr] = (st + 63) & 63;
+            afterb[nr] = (st + len) & 63;
+            nr++;
+          }
         }
       }
     }
-    auto track = [&](double dy, double dx) {
-      const int bin = wb::dir_bin(dy, dx, ttan, tcos, tsin, A.K, &nexact);
-      if (bin != before[0] && bin != afterb[0] && bin != before[1] && bin != afterb[1]) return;
-      const int pos = dir_bin_pos(wb::atan2i(dy, dx, ttan, tcos, tsin, A.K, &nexact));
-      for (int k = 0; k < 2; k++) {
-        if (bin == before[k] && pos > hi[k]) hi[k] = pos;
-        if (bin == afterb[k] && pos < lo[k]) lo[k] = pos;
+    const bool act = phase < 2 ? state == 0 : state == 4;
+    uint64_t alive = __ballot(act);
+    if (alive == 0) continue;
+    if (COUNT && act) {
+      if (phase == 0) cnt[CT_LANES_A]++;
+      else if (phase == 1) cnt[CT_LANES_B]++;
+      else cnt[CT_LANES_C]++;
+    }
+    // the wave's first point orders the children
+    const int ld = __ffsll((long long)alive) - 1;
+    const double rx = __shfl(x, ld, 64), ry = __shfl(y, ld, 64);
+    int sp = 0;
+    {
+      const double *rc = T.node_cube;
+      const uint64_t m = __ballot(act && wb::shape_intersect(H, rc[0], rc[1], rc[2], rc[3]));
+      if (m) {
+        S.code[0] = 0;
+        S.lo[0] = (uint32_t)m;
+        S.hi[0] = (uint32_t)(m >> 32);
+        sp = 1;
       }
-    };
-    const bool amb = state == 4;
-    wave_walk<false, COUNT>(T, H, x, y, amb, track, [&]() { return amb; }, wcc);
-    if (amb) {
+    }
+    while (sp > 0) {
+      sp--;
+      const int code = rfl(S.code[sp]);
+      const uint64_t m = (((uint64_t)(uint32_t)rfl((int)S.hi[sp]) << 32) |
+                          (uint32_t)rfl((int)S.lo[sp])) & alive;
+      if (m == 0) continue;
+      const bool on = (m >> lane) & 1;
+      if (code >= 0) {
+        if (COUNT) {
+          if (phase == 0) cnt[CT_EXP_A]++;
+          else if (phase == 1) cnt[CT_EXP_B]++;
+          else cnt[CT_WALK_C]++;
+        }
+        sp = expand_node(T, S, sp, code, on, H, rx, ry, zf);
+        continue;
+      }
+      const int leaf = -2 - code;
+      const uint32_t b = (uint32_t)rfl((int)T.leaf_start[leaf]);
+      const uint32_t e = (uint32_t)rfl((int)T.leaf_start[leaf + 1]);
+      if (phase == 0) {
+        if (COUNT) {
+          cnt[CT_WLEAF_A]++;
+          if (on) cnt[CT_LEAF_A]++;
+        }
+        if (on) groups_a += (e - 1) / WB_CHUNK - b / WB_CHUNK + 1;
+        for (uint32_t j = b; j < e; j = (j / WB_CHUNK + 1) * WB_CHUNK) {
+          const double qx = T.px[j], qy = T.py[j], qz = T.pz[j];
+          if (on && wb::hyp_in(H, qx, qy, qz) && (qx != x || qy != y)) {
+            if (COUNT) cnt[CT_DIR_A]++;
+            occ |= 1ull << wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
+          }
+        }
+        if (COUNT && on) cnt[CT_GROUP_A] += (e - 1) / WB_CHUNK - b / WB_CHUNK + 1;
+      } else {
+        if (COUNT) {
+          if (phase == 1) cnt[CT_WLEAF_B]++;
+          else cnt[CT_WALK_C]++;
+          if (phase == 1 && on) cnt[CT_LEAF_B]++;
+        }
+        const double *lc = T.leaf_cube + 4 * (size_t)leaf;
+        const bool all = on && wb::shape_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
+        for (uint32_t c0 = b; c0 < e;) {
+          const uint32_t chk = c0 / WB_CHUNK;
+          const uint32_t ce = min(e, (chk + 1) * WB_CHUNK);
+          bool chit = on;
+          if (chit && !all) {
+            const double *bx = T.chunk_box + 6 * (uint64_t)chk;
+            chit = wb::shape_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+          }
+          if (__ballot(chit)) {
+            if (COUNT && phase == 1 && chit) cnt[CT_POINT_B] += ce - c0;
+            for (uint32_t j = c0; j < ce; j++) {
+              const double qx = T.px[j], qy = T.py[j];
+              if (chit && (all || wb::hyp_in(H, qx, qy, T.pz[j])) && (qx != x || qy != y)) {
+                const int bin = wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
+                if (phase == 1) {
+                  if (COUNT) cnt[CT_DIR_B]++;
+                  occ |= 1ull << bin;
+                } else if (bin == before[0] || bin == afterb[0] || bin == before[1] ||
+                           bin == afterb[1]) {
+                  const int pos =
+                      dir_bin_pos(wb::atan2i_ext(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact));
+                  for (int k = 0; k < 2; k++) {
+                    if (bin == before[k] && pos > hi[k]) hi[k] = pos;
+                    if (bin == afterb[k] && pos < lo[k]) lo[k] = pos;
+                  }
+                }
+              }
+            }
+          }
+          c0 = ce;
+        }
+      }
+      if (phase < 2) {
+        if (state == 0 && occ != checked) {
+          checked = occ;
+          if (gap_verdict(occ) == 1) state = 1;
+        }
+        if (phase == 0 && state == 0 && groups_a > HEAVY_A_GROUPS) state = 5;
+        alive = __ballot(state == 0);
+        if (alive == 0) break;
+      }
+    }
+    if (phase == 2 && state == 4) {
       bool pass = true;
       for (int k = 0; k < nr; k++) {
-        int gap = (int)(((uint32_t)lo[k] - (uint32_t)hi[k]) & 0x7fffffffu);
+        const int gap = (int)(((uint32_t)lo[k] - (uint32_t)hi[k]) & 0x7fffffffu);
         if (gap >= wb::DEG144) pass = false;
       }
       state = pass ? 1 : 2;
@@ -855,161 +929,154 @@ __global__ void __launch_bounds__(256, MINW) k_classify(ClassifyArgs A)
   if (nexact) atomicAdd(A.atan_exact, (unsigned long long)nexact);
   if (inrange && state != 5) A.cls[i] = state == 1 ? 1 : (state == 2 ? 2 : 0);
   if (COUNT) {
-    // 0 lanes in A, 1 lanes in B, 2-5 phase A leaves/groups/scanned/points,
-    // 6-9 phase B, 10 dirs A, 11 dirs B, 12 phase C points, 13 steps A (per
-    // wave), 14 steps B (per wave), 15 waves
-    uint64_t v[16] = {wasA, inB, wa.leaves, wa.groups, wa.groups_scanned, wa.points,
-                      wb_.leaves, wb_.groups, wb_.groups_scanned, wb_.points, dirA,
-                      ndir - dirA, wcc.points, 0, 0, 0};
-    for (int k = 0; k < 13; k++) {
-      uint64_t t = v[k];
-      for (int sft = 1; sft < 64; sft <<= 1) t += __shfl_xor(t, sft, 64);
+    if (state == 5) cnt[CT_HEAVY]++;
+    uint64_t v[CT_WAVES];
+    for (int k = 0; k < CT_WAVES; k++) {
+      uint64_t t = cnt[k];
+      // wave-level counters are the same in every lane
+      if (k != CT_EXP_A && k != CT_WLEAF_A && k != CT_EXP_B && k != CT_WLEAF_B && k != CT_WALK_C)
+        for (int sft = 1; sft < 64; sft <<= 1) t += __shfl_xor(t, sft, 64);
       v[k] = t;
     }
-    if (__lane_id() == 0) {
-      for (int k = 0; k < 13; k++) atomicAdd(&A.counters[k], (unsigned long long)v[k]);
-      atomicAdd(&A.counters[13], (unsigned long long)wa.steps);
-      atomicAdd(&A.counters[14], (unsigned long long)wb_.steps);
-      atomicAdd(&A.counters[15], 1ull);
-      uint64_t dt = wall_clock64() - t_start;
-      atomicAdd(&A.counters[16], (unsigned long long)dt);
-      atomicMax(&A.counters[17], (unsigned long long)dt);
-      // histogram of wave times in ticks: <1e3, <1e4, <1e5, <1e6, >=1e6
-      int hb = dt < 1000 ? 0 : dt < 10000 ? 1 : dt < 100000 ? 2 : dt < 1000000 ? 3 : 4;
-      atomicAdd(&A.counters[18 + hb], 1ull);
-      atomicAdd(&A.counters[23 + hb], (unsigned long long)dt);
+    if (lane == 0) {
+      for (int k = 0; k < CT_WAVES; k++) atomicAdd(&A.counters[k], (unsigned long long)v[k]);
+      atomicAdd(&A.counters[CT_WAVES], 1ull);
+      const uint64_t dt = wall_clock64() - t_start;
+      atomicAdd(&A.counters[CT_TSUM], (unsigned long long)dt);
+      atomicMax(&A.counters[CT_TMAX], (unsigned long long)dt);
+      // wave times in ticks: <1e3, <1e4, <1e5, <1e6, >=1e6
+      const int hb = dt < 1000 ? 0 : dt < 10000 ? 1 : dt < 100000 ? 2 : dt < 1000000 ? 3 : 4;
+      atomicAdd(&A.counters[CT_THIST + hb], 1ull);
+      atomicAdd(&A.counters[CT_TSUMS + hb], (unsigned long long)dt);
     }
   }
 }
 
-__global__ void k_scatter_class(uint64_t n, const uint8_t *cls, const uint32_t *sidx, uint8_t *out)
+// One heavy point per 64-lane block.  The shape is the same in every lane,
+// so lanes split the work instead: lanes 0-7 test the eight children of a
+// node, and the lanes share a leaf's sampled group starts (walk A) or its
+// points (walks B and C).  Same three walks and visiting order as k_ground.
+__global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const uint32_t *heavy,
+                                                     uint32_t nheavy)
 {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  out[sidx[i]] = cls[i];
-}
-
-__device__ inline uint64_t wave_or(uint64_t v)
-{
-  for (int s = 1; s < 64; s <<= 1) v |= __shfl_xor(v, s, 64);
-  return v;
-}
-
-// One heavy point per 64-lane block: the walk is uniform (one shape) and the
-// lanes split each leaf's points.  Same three phases as k_classify: sampled
-// groups, all points, then the exact border pass for a 24/25-bin run.
-__global__ void __launch_bounds__(64) k_classify_heavy(ClassifyArgs A, const uint32_t *heavy,
-                                                       uint32_t nheavy)
-{
-  __shared__ double tabs[511 + 512 + 512];
-  for (int k = threadIdx.x; k < 511 + 512 + 512; k += blockDim.x) tabs[k] = A.tab[k];
-  __syncthreads();
+  __shared__ int stk[WB_STK];
   if (blockIdx.x >= nheavy) return;
   const TreeView &T = A.T;
   const uint32_t i = heavy[blockIdx.x];
   const int lane = threadIdx.x;
-  const double *ttan = tabs, *tcos = tabs + 511, *tsin = tabs + 1023;
+  const double *ttan = A.tab, *tcos = A.tab + 511, *tsin = A.tab + 1023;
   int w = INT32_MIN;
   for (uint32_t k = 0; k < A.nmemb[i]; k++) w = max(w, A.memb[MAX_MEMB * i + k]);
   const double x = T.px[i], y = T.py[i], z = T.pz[i];
   const wb::Shape H = wb::make_hyperboloid(x - 0, y - 0, z - A.thickness, A.rec[w - A.g.lo].hs,
                                            A.max_slope);
+  const int zf = A.max_slope > 0 ? 0 : 4;
   int nexact = 0;
-  // mode 0: sampled, 1: all points, 2: exact border pass
-  auto walk = [&](int mode, auto &&on_dir, auto &&after_leaf) {
-    if (!wb::shape_intersect(H, T.node_cube[0], T.node_cube[1], T.node_cube[2], T.node_cube[3]))
-      return;
-    int node = 0, depth = 0;
-    Iter it = {0, 0};
-    for (;;) {
-      int c = it.get(depth);
-      if (c == 8) {
-        if (depth == 0) return;
-        node = T.node_parent[node];
-        depth--;
-        continue;
-      }
-      it.set(depth, c + 1);
-      int ch = T.node_child[8 * node + c];
-      if (ch == -1) continue;
-      if (ch >= 0) {
-        const double *q = T.node_cube + 4 * ch;
-        if (wb::shape_intersect(H, q[0], q[1], q[2], q[3])) {
-          node = ch;
-          depth++;
-          it.set(depth, 0);
+  uint64_t occ = 0, checked = 0;
+  int verdict = 0;  // 1 pass, 2 fail, 4 ambiguous
+  int before[2] = {-1, -1}, afterb[2] = {-1, -1}, nr = 0;
+  int hi[2] = {-1, -1}, lo[2] = {0x7fffffff, 0x7fffffff};
+  for (int phase = 0; phase < 3 && verdict != 1 && verdict != 2; phase++) {
+    if (phase == 2) {
+      const uint64_t occw = wave_or(occ);
+      verdict = gap_verdict(occw);
+      if (verdict != 0) break;
+      verdict = 4;
+      for (int st = 0; st < 64; st++) {
+        if ((occw >> st) & 1) continue;
+        if (!((occw >> ((st + 63) & 63)) & 1)) continue;
+        int len = 0;
+        while (len < 64 && !((occw >> ((st + len) & 63)) & 1)) len++;
+        if (len >= 24 && nr < 2) {
+          before[nr] = (st + 63) & 63;
+          afterb[nr] = (st + len) & 63;
+          nr++;
         }
+      }
+    }
+    int sp = 0;
+    {
+      const double *rc = T.node_cube;
+      if (wb::shape_intersect(H, rc[0], rc[1], rc[2], rc[3])) {
+        if (lane == 0) stk[0] = 0;
+        sp = 1;
+      }
+    }
+    __syncthreads();
+    while (sp > 0) {
+      sp--;
+      const int code = rfl(stk[sp]);
+      __syncthreads();
+      if (code >= 0) {
+        const double *nc = T.node_cube + 4 * (size_t)code;
+        const double cx = nc[0], cy = nc[1], cz = nc[2], side = nc[3];
+        const int o = (x >= cx ? 1 : 0) | (y >= cy ? 2 : 0);
+        const int k = lane & 7;
+        const int ck = T.node_child[8 * (size_t)code + k];
+        bool h = false;
+        if (lane < 8 && ck != -1) {
+          const double ncx = cx + (double)(2 * (k & 1) - 1) * side / 4;
+          const double ncy = cy + (double)(2 * ((k & 2) / 2) - 1) * side / 4;
+          const double ncz = cz + (double)(2 * ((k & 4) / 4) - 1) * side / 4;
+          h = wb::shape_intersect(H, ncx, ncy, ncz, side / 2);
+        }
+        const int t = child_rank(k, o, zf);
+        uint32_t rb = h ? 1u << t : 0u;
+        rb |= __shfl_xor(rb, 1, 64);
+        rb |= __shfl_xor(rb, 2, 64);
+        rb |= __shfl_xor(rb, 4, 64);
+        rb = (uint32_t)rfl((int)rb);
+        if (h) stk[sp + __popc(rb >> (t + 1))] = ck;
+        sp += __popc(rb);
+        __syncthreads();
         continue;
       }
-      const int leaf = -2 - ch;
-      const double *lc = T.leaf_cube + 4 * leaf;
-      if (!wb::shape_intersect(H, lc[0], lc[1], lc[2], lc[3])) continue;
-      const bool all = wb::shape_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
-      const uint32_t b = T.leaf_start[leaf], e = T.leaf_start[leaf + 1];
-      if (mode == 0) {
-        for (uint32_t k = b / WB_CHUNK + lane; k * WB_CHUNK < e; k += 64) {
-          const uint32_t j = max(b, k * WB_CHUNK);
-          if (!all) {
-            const double *bx = T.chunk_box + 6 * (uint64_t)k;
-            if (!wb::shape_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5])) continue;
-          }
+      const int leaf = -2 - code;
+      const uint32_t b = (uint32_t)rfl((int)T.leaf_start[leaf]);
+      const uint32_t e = (uint32_t)rfl((int)T.leaf_start[leaf + 1]);
+      if (phase == 0) {
+        const uint32_t g0 = b / WB_CHUNK, ng = (e - 1) / WB_CHUNK - g0 + 1;
+        for (uint32_t g = lane; g < ng; g += 64) {
+          const uint32_t j = g == 0 ? b : (g0 + g) * WB_CHUNK;
           const double qx = T.px[j], qy = T.py[j];
-          if ((all || wb::hyp_in(H, qx, qy, T.pz[j])) && (qx != x || qy != y)) on_dir(qy - y, qx - x);
+          if (wb::hyp_in(H, qx, qy, T.pz[j]) && (qx != x || qy != y))
+            occ |= 1ull << wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
         }
       } else {
+        const double *lc = T.leaf_cube + 4 * (size_t)leaf;
+        const bool all = wb::shape_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
         for (uint32_t j = b + lane; j < e; j += 64) {
-          if (!all) {
-            const double *bx = T.chunk_box + 6 * (uint64_t)(j / WB_CHUNK);
-            if (!wb::shape_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5])) continue;
-          }
           const double qx = T.px[j], qy = T.py[j];
-          if ((all || wb::hyp_in(H, qx, qy, T.pz[j])) && (qx != x || qy != y)) on_dir(qy - y, qx - x);
+          if ((all || wb::hyp_in(H, qx, qy, T.pz[j])) && (qx != x || qy != y)) {
+            const int bin = wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
+            if (phase == 1) {
+              occ |= 1ull << bin;
+            } else if (bin == before[0] || bin == afterb[0] || bin == before[1] ||
+                       bin == afterb[1]) {
+              const int pos =
+                  dir_bin_pos(wb::atan2i_ext(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact));
+              for (int k = 0; k < 2; k++) {
+                if (bin == before[k] && pos > hi[k]) hi[k] = pos;
+                if (bin == afterb[k] && pos < lo[k]) lo[k] = pos;
+              }
+            }
+          }
         }
       }
-      if (after_leaf()) return;
-    }
-  };
-  uint64_t occ = 0, checked = 0;
-  int verdict = 0;
-  auto add_dir = [&](double dy, double dx) {
-    occ |= 1ull << wb::dir_bin(dy, dx, ttan, tcos, tsin, A.K, &nexact);
-  };
-  auto after = [&]() {
-    uint64_t all = wave_or(occ);
-    if (all != checked) {
-      checked = all;
-      if (gap_verdict(all) == 1) verdict = 1;
-    }
-    return verdict == 1;
-  };
-  walk(0, add_dir, after);
-  if (verdict == 0) walk(1, add_dir, after);
-  uint64_t occw = wave_or(occ);
-  if (verdict == 0) verdict = gap_verdict(occw);
-  if (verdict == 0) {
-    int before[2] = {-1, -1}, afterb[2] = {-1, -1}, nr = 0;
-    for (int st = 0; st < 64; st++) {
-      if ((occw >> st) & 1) continue;
-      if (!((occw >> ((st + 63) & 63)) & 1)) continue;
-      int len = 0;
-      while (len < 64 && !((occw >> ((st + len) & 63)) & 1)) len++;
-      if (len >= 24 && nr < 2) {
-        before[nr] = (st + 63) & 63;
-        afterb[nr] = (st + len) & 63;
-        nr++;
+      if (phase < 2) {
+        const uint64_t occw = wave_or(occ);
+        if (occw != checked) {
+          checked = occw;
+          if (gap_verdict(occw) == 1) {
+            verdict = 1;
+            break;
+          }
+        }
       }
     }
-    int hi[2] = {-1, -1}, lo[2] = {0x7fffffff, 0x7fffffff};
-    auto track = [&](double dy, double dx) {
-      const int bin = wb::dir_bin(dy, dx, ttan, tcos, tsin, A.K, &nexact);
-      if (bin != before[0] && bin != afterb[0] && bin != before[1] && bin != afterb[1]) return;
-      const int pos = dir_bin_pos(wb::atan2i(dy, dx, ttan, tcos, tsin, A.K, &nexact));
-      for (int k = 0; k < 2; k++) {
-        if (bin == before[k] && pos > hi[k]) hi[k] = pos;
-        if (bin == afterb[k] && pos < lo[k]) lo[k] = pos;
-      }
-    };
-    walk(2, track, [&]() { return false; });
+  }
+  if (verdict == 0) verdict = gap_verdict(wave_or(occ));
+  if (verdict == 4) {
     bool pass = true;
     for (int k = 0; k < nr; k++) {
       int h = hi[k], l = lo[k];
@@ -1017,7 +1084,7 @@ __global__ void __launch_bounds__(64) k_classify_heavy(ClassifyArgs A, const uin
         h = max(h, __shfl_xor(h, s, 64));
         l = min(l, __shfl_xor(l, s, 64));
       }
-      int gap = (int)(((uint32_t)l - (uint32_t)h) & 0x7fffffffu);
+      const int gap = (int)(((uint32_t)l - (uint32_t)h) & 0x7fffffffu);
       if (gap >= wb::DEG144) pass = false;
     }
     verdict = pass ? 1 : 2;
@@ -1028,6 +1095,13 @@ __global__ void __launch_bounds__(64) k_classify_heavy(ClassifyArgs A, const uin
     A.cls[i] = (uint8_t)verdict;
     if (nexact) atomicAdd(A.atan_exact, (unsigned long long)nexact);
   }
+}
+
+__global__ void k_scatter_class(uint64_t n, const uint8_t *cls, const uint32_t *sidx, uint8_t *out)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[sidx[i]] = cls[i];
 }
 
 // class histogram (per-block partials, summed by k_sum_hist) and, when tile
@@ -1158,8 +1232,8 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   static const bool count_work = getenv("WB_CLASSIFY_COUNTERS") != nullptr;
   unsigned long long *dcount = nullptr;
   if (count_work) {
-    dcount = (unsigned long long *)wb_alloc(c, 32 * 8);
-    WB_CHECK(hipMemsetAsync(dcount, 0, 32 * 8, st));
+    dcount = (unsigned long long *)wb_alloc(c, CT_N * 8);
+    WB_CHECK(hipMemsetAsync(dcount, 0, CT_N * 8, st));
   }
   A.counters = dcount;
   uint32_t *heavy = (uint32_t *)wb_alloc(c, 4 * n + 4);
@@ -1167,15 +1241,11 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   A.nheavy = nslow + 1;
   if (n > 0) {
     if (count_work)
-      hipLaunchKernelGGL((k_classify<true, 6>), dim3(wb_blocks(n, 256)), dim3(256), 0, st, A);
+      hipLaunchKernelGGL((k_ground<true, WB_GROUND_MINW>), dim3(wb_blocks(n, 256)), dim3(256), 0,
+                         st, A);
     else
-    {
-      static const int occ8 = getenv("WB_CLASSIFY_OCC8") != nullptr;
-      if (occ8)
-        hipLaunchKernelGGL((k_classify<false, 8>), dim3(wb_blocks(n, 256)), dim3(256), 0, st, A);
-      else
-        hipLaunchKernelGGL((k_classify<false, 6>), dim3(wb_blocks(n, 256)), dim3(256), 0, st, A);
-    }
+      hipLaunchKernelGGL((k_ground<false, WB_GROUND_MINW>), dim3(wb_blocks(n, 256)), dim3(256), 0,
+                         st, A);
   }
   WB_CHECK(hipEventRecord(ev[5], st));
   uint32_t h_nheavy = 0;
@@ -1183,7 +1253,7 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   WB_CHECK(hipStreamSynchronize(st));
   WB_CHECK(hipEventRecord(ev[6], st));
   if (h_nheavy)
-    hipLaunchKernelGGL(k_classify_heavy, dim3(h_nheavy), dim3(64), 0, st, A, heavy, h_nheavy);
+    hipLaunchKernelGGL(k_ground_heavy, dim3(h_nheavy), dim3(64), 0, st, A, heavy, h_nheavy);
   WB_CHECK(hipEventRecord(ev[7], st));
   uint32_t h_nslow = 0;
   WB_CHECK(hipMemcpyAsync(&h_nslow, nslow, 4, hipMemcpyDeviceToHost, st));
@@ -1232,17 +1302,25 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   out->hist[0] = hc[2]; out->hist[1] = hc[3]; out->hist[2] = hc[4];
   out->nslow = h_nslow;
   if (dcount) {
-    unsigned long long hcnt[32];
-    WB_CHECK(hipMemcpy(hcnt, dcount, sizeof hcnt, hipMemcpyDeviceToHost));
-    fprintf(stderr, "[wb counters] lanesA %llu lanesB %llu | A: leaves %llu groups %llu scanned %llu points %llu | B: leaves %llu groups %llu scanned %llu points %llu | dirs A %llu B %llu | C points %llu | steps A %llu B %llu waves %llu\n",
-            hcnt[0], hcnt[1], hcnt[2], hcnt[3], hcnt[4], hcnt[5], hcnt[6], hcnt[7], hcnt[8], hcnt[9],
-            hcnt[10], hcnt[11], hcnt[12], hcnt[13], hcnt[14], hcnt[15]);
+    unsigned long long h[CT_N];
+    WB_CHECK(hipMemcpy(h, dcount, sizeof h, hipMemcpyDeviceToHost));
+    fprintf(stderr,
+            "[wb counters] lanes A %llu B %llu C %llu heavy %llu | A: leaf-lanes %llu group-lanes "
+            "%llu dirs %llu | B: leaf-lanes %llu point-lanes %llu dirs %llu | per wave: "
+            "expansions A %llu leaves A %llu expansions B %llu leaves B %llu walk C %llu | "
+            "waves %llu\n",
+            h[CT_LANES_A], h[CT_LANES_B], h[CT_LANES_C], h[CT_HEAVY], h[CT_LEAF_A], h[CT_GROUP_A],
+            h[CT_DIR_A], h[CT_LEAF_B], h[CT_POINT_B], h[CT_DIR_B], h[CT_EXP_A], h[CT_WLEAF_A],
+            h[CT_EXP_B], h[CT_WLEAF_B], h[CT_WALK_C], h[CT_WAVES]);
     int rate = 0;
     hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, c->device);
-    fprintf(stderr, "[wb counters] wave time: sum %llu max %llu ticks (rate %d kHz); hist counts %llu %llu %llu %llu %llu ; hist tick sums %llu %llu %llu %llu %llu\n",
-            hcnt[16], hcnt[17], rate, hcnt[18], hcnt[19], hcnt[20], hcnt[21], hcnt[22], hcnt[23],
-            hcnt[24], hcnt[25], hcnt[26], hcnt[27]);
-    wb_free(c, dcount, 32 * 8);
+    fprintf(stderr,
+            "[wb counters] wave time: sum %llu max %llu ticks (rate %d kHz); hist counts %llu "
+            "%llu %llu %llu %llu ; hist tick sums %llu %llu %llu %llu %llu\n",
+            h[CT_TSUM], h[CT_TMAX], rate, h[CT_THIST], h[CT_THIST + 1], h[CT_THIST + 2],
+            h[CT_THIST + 3], h[CT_THIST + 4], h[CT_TSUMS], h[CT_TSUMS + 1], h[CT_TSUMS + 2],
+            h[CT_TSUMS + 3], h[CT_TSUMS + 4]);
+    wb_free(c, dcount, CT_N * 8);
   }
   out->nonempty = hc[5];
   out->ms_tiles = ms1; out->ms_scan = ms2; out->ms_classify = ms3;

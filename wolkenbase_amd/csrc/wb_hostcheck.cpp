@@ -51,7 +51,7 @@ long wbh_last_exact(double y, double x)
   ready();
   double u = y / x;
   double c3 = wb::ATAN_C3 * (u * u * u);
-  return (long)wb::atan_last_exact(y, x, c3, g_k);
+  return (long)wb::atan_last_exact(y, x, c3, g_k.k_m, g_k.k_e);
 }
 
 long wbh_last_native(double y, double x)

@@ -23,9 +23,12 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define WB_HD __host__ __device__ inline
+// rarely taken exact paths stay out of line so the hot loops stay small
+#define WB_HD_COLD __host__ __device__ inline __attribute__((noinline))
 #else
 #include <math.h>
 #define WB_HD inline
+#define WB_HD_COLD inline __attribute__((noinline))
 #endif
 
 namespace wb {
@@ -305,10 +308,10 @@ struct AtanConst {
 constexpr double ATAN_C3 = 1.1392738508503886e8;
 
 // Exact x87 evaluation of lrintl(K*y/x - c3) (angle.cpp:151).
-WB_HD int64_t atan_last_exact(double y, double x, double c3, const AtanConst &K)
+WB_HD_COLD int64_t atan_last_exact(double y, double x, double c3, uint64_t k_m, int k_e)
 {
   X80 k;
-  k.m = K.k_m; k.e = K.k_e; k.s = 0;
+  k.m = k_m; k.e = k_e; k.s = 0;
   X80 p = x80_mul(k, x80_from_double(y));
   X80 q = x80_div(p, x80_from_double(x));
   X80 r = x80_add(q, x80_neg(x80_from_double(c3)));
@@ -371,7 +374,7 @@ WB_HD int atan2i(double y, double x, TabPtr tab_tan, TabPtr tab_cos, TabPtr tab_
   } else if (!force_exact && af > 0.5 + 0x1p-25) {
     r = (int64_t)n + (f > 0 ? 1 : -1);
   } else {
-    r = atan_last_exact(y, x, c3, K);
+    r = atan_last_exact(y, x, c3, K.k_m, K.k_e);
     if (exact_used) *exact_used += 1;
   }
   return (int)((int64_t)a + r);
@@ -390,9 +393,29 @@ WB_HD int atan2i(double y, double x, TabPtr tab_tan, TabPtr tab_cos, TabPtr tab_
 // ---------------------------------------------------------------------------
 constexpr int DIR_EPS = 1 << 16;
 
-template <typename TabPtr>
-WB_HD int dir_bin(double dy, double dx, TabPtr tab_tan, TabPtr tab_cos, TabPtr tab_sin,
-                  const AtanConst &K, int *exact_used = nullptr)
+// atan2i out of line; returns the angle in the low 32 bits and the number of
+// exact last steps taken (0 or 1) in bit 32
+WB_HD_COLD int64_t atan2i_cold(double y, double x, const double *tab_tan, const double *tab_cos,
+                               const double *tab_sin, double k_hi, double k_lo, uint64_t k_m,
+                               int k_e)
+{
+  AtanConst K;
+  K.k_hi = k_hi; K.k_lo = k_lo; K.k_m = k_m; K.k_e = k_e;
+  int used = 0;
+  const int a = atan2i(y, x, tab_tan, tab_cos, tab_sin, K, &used);
+  return (int64_t)(uint32_t)a | ((int64_t)used << 32);
+}
+
+WB_HD int atan2i_ext(double y, double x, const double *tab_tan, const double *tab_cos,
+                     const double *tab_sin, const AtanConst &K, int *exact_used)
+{
+  const int64_t r = atan2i_cold(y, x, tab_tan, tab_cos, tab_sin, K.k_hi, K.k_lo, K.k_m, K.k_e);
+  if (exact_used) *exact_used += (int)(r >> 32);
+  return (int)(uint32_t)r;
+}
+
+WB_HD int dir_bin(double dy, double dx, const double *tab_tan, const double *tab_cos,
+                  const double *tab_sin, const AtanConst &K, int *exact_used = nullptr)
 {
   const double m = dabs(dy) > dabs(dx) ? dabs(dy) : dabs(dx);
   if (m > 1e-30 && m < 1e30) {
@@ -404,7 +427,7 @@ WB_HD int dir_bin(double dy, double dx, TabPtr tab_tan, TabPtr tab_cos, TabPtr t
     const int64_t frac = ip & ((1 << 25) - 1);
     if (frac > DIR_EPS && frac < (1 << 25) - DIR_EPS) return (int)((ip >> 25) & 63);
   }
-  const int a = atan2i(dy, dx, tab_tan, tab_cos, tab_sin, K, exact_used);
+  const int a = atan2i_ext(dy, dx, tab_tan, tab_cos, tab_sin, K, exact_used);
   return (int)((((uint32_t)a + 0x40000000u) & 0x7fffffffu) >> 25);
 }
 
@@ -448,6 +471,20 @@ WB_HD Shape make_sphere(double cx, double cy, double cz, double r)
   return c;
 }
 
+// the exact forms of the three predicates, for the rare points near the
+// boundary
+WB_HD_COLD bool cyl_in_tail(double dx, double dy, double r) { return hypot(dx, dy) <= r; }
+WB_HD_COLD bool sph_in_tail(double dx, double dy, double dz, double r)
+{
+  return hypot(hypot(dx, dy), dz) <= r;
+}
+WB_HD_COLD bool hyp_in_tail(double dx, double dy, double a, double slope, double por2)
+{
+  double xyd = hypot(dx, dy);
+  double w = xyd * slope;
+  return a - w * w >= por2;
+}
+
 // Cylinder::in, shape.cpp:214-218: hypot(c - q) <= r.  The fast path decides
 // from dx^2+dy^2 when it is more than 2^-46 (relative) away from r^2, where
 // hypot's rounding cannot change the answer.
@@ -458,7 +495,7 @@ WB_HD bool cyl_in(const Shape &s, double qx, double qy)
   double m = 0x1p-46 * r2;
   if (q2 < r2 - m) return true;
   if (q2 > r2 + m) return false;
-  return hypot(dx, dy) <= s.r;
+  return cyl_in_tail(dx, dy, s.r);
 }
 
 // Sphere::in, shape.cpp:172-175: hypot(hypot(dx,dy),dz) <= r.
@@ -469,7 +506,7 @@ WB_HD bool sph_in(const Shape &s, double qx, double qy, double qz)
   double m = 0x1p-45 * r2;
   if (q2 < r2 - m) return true;
   if (q2 > r2 + m) return false;
-  return hypot(hypot(dx, dy), dz) <= s.r;
+  return sph_in_tail(dx, dy, dz, s.r);
 }
 
 // Hyperboloid::in, shape.cpp:127-135.
@@ -488,9 +525,7 @@ WB_HD bool hyp_in(const Shape &s, double qx, double qy, double qz)
   double m = 0x1p-44 * (a + b);
   if (v - s.por2 > m) return true;
   if (v - s.por2 < -m) return false;
-  double xyd = hypot(dx, dy);
-  double w = xyd * s.slope;
-  return a - w * w >= s.por2;
+  return hyp_in_tail(dx, dy, a, s.slope, s.por2);
 }
 
 WB_HD bool shape_in(const Shape &s, double qx, double qy, double qz)
