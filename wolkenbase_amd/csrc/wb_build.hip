@@ -456,7 +456,7 @@ __global__ void k_permute(const int32_t *xi, const int32_t *yi, const int32_t *z
 }
 
 __global__ void k_chunk_box(const double *px, const double *py, const double *pz, uint64_t n,
-                            double *box)
+                            double *box, uint32_t *low)
 {
   // one half-wave (32 lanes) per chunk
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -466,8 +466,17 @@ __global__ void k_chunk_box(const double *px, const double *py, const double *pz
   double x0 = ok ? px[i] : __builtin_inf(), x1 = ok ? px[i] : -__builtin_inf();
   double y0 = ok ? py[i] : __builtin_inf(), y1 = ok ? py[i] : -__builtin_inf();
   double z0 = ok ? pz[i] : __builtin_inf(), z1 = ok ? pz[i] : -__builtin_inf();
+  // lowest point: smallest z, then smallest slot
+  double lz = z0;
+  uint32_t li = (uint32_t)i;
 #pragma unroll
   for (int d = 1; d < WB_CHUNK; d <<= 1) {
+    const double oz = __shfl_xor(lz, d, WAVE);
+    const uint32_t oi = __shfl_xor(li, d, WAVE);
+    if (oz < lz || (oz == lz && oi < li)) {
+      lz = oz;
+      li = oi;
+    }
     x0 = fmin(x0, __shfl_xor(x0, d, WAVE));
     x1 = fmax(x1, __shfl_xor(x1, d, WAVE));
     y0 = fmin(y0, __shfl_xor(y0, d, WAVE));
@@ -479,6 +488,7 @@ __global__ void k_chunk_box(const double *px, const double *py, const double *pz
   if ((g % WB_CHUNK) == 0 && chunk < nchunks) {
     double *b = box + 6 * chunk;
     b[0] = x0; b[1] = x1; b[2] = y0; b[3] = y1; b[4] = z0; b[5] = z1;
+    low[chunk] = li;
   }
 }
 
@@ -648,13 +658,14 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
   t->crank = wb_talloc<uint32_t>(t, n);
   uint64_t nchunks = (n + WB_CHUNK - 1) / WB_CHUNK;
   t->chunk_box = wb_talloc<double>(t, 6 * nchunks + 6);
+  t->chunk_low = wb_talloc<uint32_t>(t, nchunks + 1);
   if (n > 0) {
     hipLaunchKernelGGL(k_fill_pt_leaf, dim3(nl), dim3(256), 0, c->stream, t->leaf_start, nl,
                        t->pt_leaf);
     hipLaunchKernelGGL(k_permute, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, dx, dy, dz,
                        t->sidx, n, f, t->px, t->py, t->pz);
     hipLaunchKernelGGL(k_chunk_box, dim3(wb_blocks(nchunks * WB_CHUNK, 256)), dim3(256), 0,
-                       c->stream, t->px, t->py, t->pz, n, t->chunk_box);
+                       c->stream, t->px, t->py, t->pz, n, t->chunk_box, t->chunk_low);
     hipLaunchKernelGGL(k_leaf_canon, dim3(nl), dim3(256), 0, c->stream, t->leaf_start, nl,
                        t->sidx, t->canon, t->crank);
     WB_CHECK(hipEventRecord(evb[3], c->stream));

@@ -49,6 +49,7 @@ struct TreeView {
   const double *node_cube;
   const int32_t *node_child, *node_parent;
   const double *chunk_box;
+  const uint32_t *chunk_low;
   const uint32_t *pt_leaf, *canon, *crank, *sidx;
   uint32_t nleaves, nnodes;
   uint64_t n;
@@ -60,7 +61,7 @@ static TreeView view(const wb_tree *t)
   v.px = t->px; v.py = t->py; v.pz = t->pz;
   v.leaf_start = t->leaf_start; v.leaf_cube = t->leaf_cube; v.leaf_parent = t->leaf_parent;
   v.node_cube = t->node_cube; v.node_child = t->node_child; v.node_parent = t->node_parent;
-  v.chunk_box = t->chunk_box; v.pt_leaf = t->pt_leaf; v.canon = t->canon; v.crank = t->crank;
+  v.chunk_box = t->chunk_box; v.chunk_low = t->chunk_low; v.pt_leaf = t->pt_leaf; v.canon = t->canon; v.crank = t->crank;
   v.sidx = t->sidx; v.nleaves = t->nleaves; v.nnodes = t->nnodes; v.n = t->n;
   return v;
 }
@@ -611,6 +612,14 @@ __global__ void k_postscan(TileGeo g, const int32_t *tex, const int32_t *tey, Ti
 // ---------------------------------------------------------------------------
 // CLASSIFY
 // ---------------------------------------------------------------------------
+// a point handed to the wave-per-point kernel, with the directions found
+// so far and the walk it continues with
+struct HeavyRec {
+  uint64_t occ;
+  uint32_t i;
+  uint32_t start;  // walk to start from: 0 A, 1 B, 2 the exact border pass
+};
+
 struct ClassifyArgs {
   TreeView T;
   TileGeo g;
@@ -622,7 +631,7 @@ struct ClassifyArgs {
   wb::AtanConst K;
   uint8_t *cls;          // per sorted slot
   uint32_t *nslow;       // points decided by the exact gap pass
-  uint32_t *heavy;       // points handed to the wave-per-point kernel
+  HeavyRec *heavy;       // points handed to the wave-per-point kernel
   uint32_t *nheavy;
   unsigned long long *atan_exact;
   unsigned long long *counters;  // 16 work counters (only with the counting kernel)
@@ -654,6 +663,14 @@ __device__ inline uint64_t wave_or(uint64_t v)
 }
 
 __device__ inline int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// lane l's value of v (l wave-uniform)
+__device__ inline double rdl(double v, uint32_t l)
+{
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, (int)l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), (int)l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
 __device__ inline uint64_t rfl64(uint64_t v)
 {
   return ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
@@ -699,29 +716,29 @@ __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int c
   const int *chp = T.node_child + 8 * (size_t)code;
   const int o = rfl((rx >= cx ? 1 : 0) | (ry >= cy ? 2 : 0));
   int ck[8];
-  uint64_t mk[8];
+  uint32_t hits = 0;    // bit k: child k intersects this lane's hyperboloid
   uint32_t pushed = 0;  // bit t: the child of rank t is pushed
 #pragma unroll
   for (int k = 0; k < 8; k++) ck[k] = rfl(chp[k]);
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    mk[k] = 0;
     if (ck[k] != -1) {
       const double ncx = cx + (double)(2 * (k & 1) - 1) * side / 4;
       const double ncy = cy + (double)(2 * ((k & 2) / 2) - 1) * side / 4;
       const double ncz = cz + (double)(2 * ((k & 4) / 4) - 1) * side / 4;
-      const bool h = on && wb::shape_intersect(H, ncx, ncy, ncz, side / 2);
-      mk[k] = __ballot(h);
-      if (mk[k]) pushed |= 1u << child_rank(k, o, zf);
+      const bool h = on && wb::hyp_intersect(H, ncx, ncy, ncz, side / 2);
+      hits |= (h ? 1u : 0u) << k;
+      if (__ballot(h)) pushed |= 1u << child_rank(k, o, zf);
     }
   }
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    if (mk[k]) {
+    const uint64_t mk = __ballot((hits >> k) & 1);
+    if (mk) {
       const int pos = sp + __popc(pushed >> (child_rank(k, o, zf) + 1));
       S.code[pos] = ck[k];
-      S.lo[pos] = (uint32_t)mk[k];
-      S.hi[pos] = (uint32_t)(mk[k] >> 32);
+      S.lo[pos] = (uint32_t)mk;
+      S.hi[pos] = (uint32_t)(mk >> 32);
     }
   }
   return sp + __popc(pushed);
@@ -730,7 +747,7 @@ __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int c
 // per-wave work counters of the instrumented kernel
 enum {
   CT_LANES_A, CT_LANES_B, CT_LANES_C, CT_LEAF_A, CT_GROUP_A, CT_DIR_A, CT_LEAF_B, CT_POINT_B,
-  CT_DIR_B, CT_EXP_A, CT_WLEAF_A, CT_EXP_B, CT_WLEAF_B, CT_WALK_C, CT_HEAVY, CT_WAVES,
+  CT_DIR_B, CT_EXP_A, CT_WLEAF_A, CT_EXP_B, CT_WLEAF_B, CT_HEAVY, CT_WAVES,
   CT_TSUM, CT_TMAX, CT_THIST, CT_TSUMS = CT_THIST + 5, CT_N = CT_TSUMS + 5
 };
 
@@ -744,13 +761,19 @@ enum {
 #define HEAVY_B_GROUPS 256u
 #endif
 
-// The ground test of classify.cpp:128-165 for one point per lane, in three
+// The ground test of classify.cpp:128-165 for one point per lane, in two
 // walks over the octree:
-//   A  the first point of each 32-point group of every leaf (a subset of D;
+//   A  the lowest point of each 32-point group of every leaf (a subset of D;
 //      only a pass is decided here),
-//   B  every point, for lanes A left open,
-//   C  for a longest empty run of 24 or 25 bins, the exact extreme
-//      directions in the two bins bordering the run.
+//   B  every point, for lanes A left open.
+// Lanes whose walks would be long, and lanes whose directions leave a
+// longest empty run of 24 or 25 bins (decided by the exact directions in
+// the two bins bordering the run), go to k_ground_heavy with the directions
+// found so far.
+// The hyperboloid is Hyperboloid(p - (0,0,thickness), hs, slope)
+// (shape.cpp:119-125) with its axis at x, y instead of x + 0, y + 0: the
+// two differ only in the sign of a zero, which no test below can see (every
+// use is a difference that is squared, compared or passed to hypot).
 template <bool COUNT, int MINW>
 __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
 {
@@ -763,61 +786,40 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
   const double *ttan = A.tab, *tcos = A.tab + 511, *tsin = A.tab + 1023;
   const uint64_t t_start = COUNT ? wall_clock64() : 0;
   uint32_t cnt[CT_WAVES] = {};
-  // lane state: 0 undecided, 1 not ground, 2 ground, 3 no tile, 4 ambiguous,
-  // 5 heavy
+  // lane state: 0 undecided, 1 not ground, 2 ground, 3 no tile; handed to
+  // k_ground_heavy: 5 walk A cut short, 6 after walk A, 7 after walk B
   int state = 3;
-  double x = 0, y = 0, z = 0;
-  wb::Shape H = wb::make_cylinder(0, 0, 0);
+  double x = 0, y = 0, z = 0, hs = 0;
   if (inrange && A.nmemb[i] > 0) {
     int w = INT32_MIN;
     for (uint32_t k = 0; k < A.nmemb[i]; k++) w = max(w, A.memb[MAX_MEMB * i + k]);
     x = T.px[i]; y = T.py[i]; z = T.pz[i];
-    H = wb::make_hyperboloid(x - 0, y - 0, z - A.thickness, A.rec[w - A.g.lo].hs, A.max_slope);
+    hs = A.rec[w - A.g.lo].hs;
     state = 0;
+  }
+  wb::Shape H;
+  {
+    const double sl = A.max_slope, por = hs * (sl * sl);
+    H.kind = wb::SHAPE_HYPERBOLOID;
+    H.cx = x;
+    H.cy = y;
+    H.cz = (z - A.thickness) + por;
+    H.r = 0;
+    H.por2 = por * por;
+    H.slope = sl;
   }
   const int zf = A.max_slope > 0 ? 0 : 4;
   uint64_t occ = 0, checked = 0;
   int nexact = 0;
   uint32_t groups_a = 0;
-  int before[2] = {-1, -1}, afterb[2] = {-1, -1}, nr = 0;
-  int hi[2] = {-1, -1}, lo[2] = {0x7fffffff, 0x7fffffff};
-  for (int phase = 0; phase < 3; phase++) {
-    if (phase == 1) {
-      if (state == 0 && groups_a > HEAVY_B_GROUPS) state = 5;
-      const uint64_t hb = __ballot(state == 5);
-      if (hb) {
-        const int leader = __ffsll((long long)hb) - 1;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(A.nheavy, (uint32_t)__popcll(hb));
-        base = __shfl(base, leader, 64);
-        if (state == 5) A.heavy[base + __popcll(hb & ((1ull << lane) - 1))] = (uint32_t)i;
-      }
-    } else if (phase == 2) {
-      if (state == 0) {
-        const int v = gap_verdict(occ);
-        state = v == 0 ? 4 : v;
-      }
-      if (state == 4) {
-        for (int st = 0; st < 64; st++) {
-          if ((occ >> st) & 1) continue;
-          if (!((occ >> ((st + 63) & 63)) & 1)) continue;
-          int len = 0;
-          while (len < 64 && !((occ >> ((st + len) & 63)) & 1)) len++;
-          if (len >= 24 && nr < 2) {
-            before[nr] = (st + 63) & 63;
-            afterb[nr] = (st + len) & 63;
-            nr++;
-          }
-        }
-      }
-    }
-    const bool act = phase < 2 ? state == 0 : state == 4;
+  for (int phase = 0; phase < 2; phase++) {
+    if (phase == 1 && state == 0 && groups_a > HEAVY_B_GROUPS) state = 6;
+    const bool act = state == 0;
     uint64_t alive = __ballot(act);
-    if (alive == 0) continue;
+    if (alive == 0) break;
     if (COUNT && act) {
       if (phase == 0) cnt[CT_LANES_A]++;
-      else if (phase == 1) cnt[CT_LANES_B]++;
-      else cnt[CT_LANES_C]++;
+      else cnt[CT_LANES_B]++;
     }
     // the wave's first point orders the children
     const int ld = __ffsll((long long)alive) - 1;
@@ -825,7 +827,7 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
     int sp = 0;
     {
       const double *rc = T.node_cube;
-      const uint64_t m = __ballot(act && wb::shape_intersect(H, rc[0], rc[1], rc[2], rc[3]));
+      const uint64_t m = __ballot(act && wb::hyp_intersect(H, rc[0], rc[1], rc[2], rc[3]));
       if (m) {
         S.code[0] = 0;
         S.lo[0] = (uint32_t)m;
@@ -843,8 +845,7 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
       if (code >= 0) {
         if (COUNT) {
           if (phase == 0) cnt[CT_EXP_A]++;
-          else if (phase == 1) cnt[CT_EXP_B]++;
-          else cnt[CT_WALK_C]++;
+          else cnt[CT_EXP_B]++;
         }
         sp = expand_node(T, S, sp, code, on, H, rx, ry, zf);
         continue;
@@ -857,84 +858,101 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
           cnt[CT_WLEAF_A]++;
           if (on) cnt[CT_LEAF_A]++;
         }
-        if (on) groups_a += (e - 1) / WB_CHUNK - b / WB_CHUNK + 1;
-        for (uint32_t j = b; j < e; j = (j / WB_CHUNK + 1) * WB_CHUNK) {
-          const double qx = T.px[j], qy = T.py[j], qz = T.pz[j];
-          if (on && wb::hyp_in(H, qx, qy, qz) && (qx != x || qy != y)) {
-            if (COUNT) cnt[CT_DIR_A]++;
-            occ |= 1ull << wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
+        const uint32_t g0 = b / WB_CHUNK, ng = (e - 1) / WB_CHUNK - g0 + 1;
+        if (on) groups_a += ng;
+        if (COUNT && on) cnt[CT_GROUP_A] += ng;
+        // the lowest point of each group (the likeliest to lie in a downward
+        // hyperboloid), or the group's first slot in the leaf when the
+        // lowest one belongs to the neighbouring leaf; lane g fetches the
+        // sample of group g, then every lane tests every sample
+        for (uint32_t gb = 0; gb < ng; gb += WAVE) {
+          const uint32_t g = gb + lane;
+          double sx = 0, sy = 0, sz = 0;
+          if (g < ng) {
+            uint32_t j = T.chunk_low[g0 + g];
+            if (j < b || j >= e) j = g == 0 ? b : (g0 + g) * WB_CHUNK;
+            sx = T.px[j]; sy = T.py[j]; sz = T.pz[j];
+          }
+          const uint32_t ns = min((uint32_t)WAVE, ng - gb);
+          for (uint32_t s = 0; s < ns; s++) {
+            const double qx = rdl(sx, s), qy = rdl(sy, s), qz = rdl(sz, s);
+            if (on && wb::hyp_in(H, qx, qy, qz) && (qx != x || qy != y)) {
+              if (COUNT) cnt[CT_DIR_A]++;
+              occ |= 1ull << wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
+            }
           }
         }
-        if (COUNT && on) cnt[CT_GROUP_A] += (e - 1) / WB_CHUNK - b / WB_CHUNK + 1;
       } else {
         if (COUNT) {
-          if (phase == 1) cnt[CT_WLEAF_B]++;
-          else cnt[CT_WALK_C]++;
-          if (phase == 1 && on) cnt[CT_LEAF_B]++;
+          cnt[CT_WLEAF_B]++;
+          if (on) cnt[CT_LEAF_B]++;
         }
         const double *lc = T.leaf_cube + 4 * (size_t)leaf;
-        const bool all = on && wb::shape_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
+        const bool all = on && wb::hyp_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
         for (uint32_t c0 = b; c0 < e;) {
           const uint32_t chk = c0 / WB_CHUNK;
           const uint32_t ce = min(e, (chk + 1) * WB_CHUNK);
+          // lane k fetches slot c0 + k while the group's box is tested
+          double sx = 0, sy = 0, sz = 0;
+          if (lane < ce - c0) {
+            sx = T.px[c0 + lane]; sy = T.py[c0 + lane]; sz = T.pz[c0 + lane];
+          }
           bool chit = on;
           if (chit && !all) {
             const double *bx = T.chunk_box + 6 * (uint64_t)chk;
-            chit = wb::shape_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+            chit = wb::hyp_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
           }
           if (__ballot(chit)) {
-            if (COUNT && phase == 1 && chit) cnt[CT_POINT_B] += ce - c0;
-            for (uint32_t j = c0; j < ce; j++) {
-              const double qx = T.px[j], qy = T.py[j];
-              if (chit && (all || wb::hyp_in(H, qx, qy, T.pz[j])) && (qx != x || qy != y)) {
-                const int bin = wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
-                if (phase == 1) {
-                  if (COUNT) cnt[CT_DIR_B]++;
-                  occ |= 1ull << bin;
-                } else if (bin == before[0] || bin == afterb[0] || bin == before[1] ||
-                           bin == afterb[1]) {
-                  const int pos =
-                      dir_bin_pos(wb::atan2i_ext(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact));
-                  for (int k = 0; k < 2; k++) {
-                    if (bin == before[k] && pos > hi[k]) hi[k] = pos;
-                    if (bin == afterb[k] && pos < lo[k]) lo[k] = pos;
-                  }
-                }
+            if (COUNT && chit) cnt[CT_POINT_B] += ce - c0;
+            for (uint32_t s = 0; s < ce - c0; s++) {
+              const double qx = rdl(sx, s), qy = rdl(sy, s), qz = rdl(sz, s);
+              if (chit && (all || wb::hyp_in(H, qx, qy, qz)) && (qx != x || qy != y)) {
+                if (COUNT) cnt[CT_DIR_B]++;
+                occ |= 1ull << wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
               }
             }
           }
           c0 = ce;
         }
       }
-      if (phase < 2) {
-        if (state == 0 && occ != checked) {
-          checked = occ;
-          if (gap_verdict(occ) == 1) state = 1;
-        }
-        if (phase == 0 && state == 0 && groups_a > HEAVY_A_GROUPS) state = 5;
-        alive = __ballot(state == 0);
-        if (alive == 0) break;
+      if (state == 0 && occ != checked) {
+        checked = occ;
+        if (gap_verdict(occ) == 1) state = 1;
       }
+      if (phase == 0 && state == 0 && groups_a > HEAVY_A_GROUPS) state = 5;
+      alive = __ballot(state == 0);
+      if (alive == 0) break;
     }
-    if (phase == 2 && state == 4) {
-      bool pass = true;
-      for (int k = 0; k < nr; k++) {
-        const int gap = (int)(((uint32_t)lo[k] - (uint32_t)hi[k]) & 0x7fffffffu);
-        if (gap >= wb::DEG144) pass = false;
-      }
-      state = pass ? 1 : 2;
-      atomicAdd(A.nslow, 1u);
+  }
+  if (state == 0) {
+    const int v = gap_verdict(occ);
+    state = v == 0 ? 7 : v;
+  }
+  // hand the open lanes over, with the directions found so far
+  const uint64_t hb = __ballot(state >= 5);
+  if (hb) {
+    const int leader = __ffsll((long long)hb) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(A.nheavy, (uint32_t)__popcll(hb));
+    base = __shfl(base, leader, 64);
+    if (state >= 5) {
+      HeavyRec r;
+      r.occ = occ;
+      r.i = (uint32_t)i;
+      r.start = (uint32_t)(state - 5);
+      A.heavy[base + __popcll(hb & ((1ull << lane) - 1))] = r;
     }
   }
   if (nexact) atomicAdd(A.atan_exact, (unsigned long long)nexact);
-  if (inrange && state != 5) A.cls[i] = state == 1 ? 1 : (state == 2 ? 2 : 0);
+  if (inrange && state < 5) A.cls[i] = state == 1 ? 1 : (state == 2 ? 2 : 0);
   if (COUNT) {
-    if (state == 5) cnt[CT_HEAVY]++;
+    if (state == 7) cnt[CT_LANES_C]++;
+    if (state >= 5) cnt[CT_HEAVY]++;
     uint64_t v[CT_WAVES];
     for (int k = 0; k < CT_WAVES; k++) {
       uint64_t t = cnt[k];
       // wave-level counters are the same in every lane
-      if (k != CT_EXP_A && k != CT_WLEAF_A && k != CT_EXP_B && k != CT_WLEAF_B && k != CT_WALK_C)
+      if (k != CT_EXP_A && k != CT_WLEAF_A && k != CT_EXP_B && k != CT_WLEAF_B)
         for (int sft = 1; sft < 64; sft <<= 1) t += __shfl_xor(t, sft, 64);
       v[k] = t;
     }
@@ -954,15 +972,17 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
 
 // One heavy point per 64-lane block.  The shape is the same in every lane,
 // so lanes split the work instead: lanes 0-7 test the eight children of a
-// node, and the lanes share a leaf's sampled group starts (walk A) or its
-// points (walks B and C).  Same three walks and visiting order as k_ground.
-__global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const uint32_t *heavy,
+// node, and the lanes share a leaf's sampled groups (walk A) or its points
+// (walk B, and C, the exact border pass for a longest empty run of 24 or 25
+// bins).  Same visiting order as k_ground.
+__global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const HeavyRec *heavy,
                                                      uint32_t nheavy)
 {
   __shared__ int stk[WB_STK];
   if (blockIdx.x >= nheavy) return;
   const TreeView &T = A.T;
-  const uint32_t i = heavy[blockIdx.x];
+  const HeavyRec hr = heavy[blockIdx.x];
+  const uint32_t i = hr.i;
   const int lane = threadIdx.x;
   const double *ttan = A.tab, *tcos = A.tab + 511, *tsin = A.tab + 1023;
   int w = INT32_MIN;
@@ -972,11 +992,13 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const uint3
                                            A.max_slope);
   const int zf = A.max_slope > 0 ? 0 : 4;
   int nexact = 0;
-  uint64_t occ = 0, checked = 0;
+  // the directions k_ground found are a subset of D; the walks it completed
+  // did not pass and are not repeated
+  uint64_t occ = lane == 0 ? hr.occ : 0, checked = hr.occ;
   int verdict = 0;  // 1 pass, 2 fail, 4 ambiguous
   int before[2] = {-1, -1}, afterb[2] = {-1, -1}, nr = 0;
   int hi[2] = {-1, -1}, lo[2] = {0x7fffffff, 0x7fffffff};
-  for (int phase = 0; phase < 3 && verdict != 1 && verdict != 2; phase++) {
+  for (int phase = rfl((int)hr.start); phase < 3 && verdict != 1 && verdict != 2; phase++) {
     if (phase == 2) {
       const uint64_t occw = wave_or(occ);
       verdict = gap_verdict(occw);
@@ -997,7 +1019,7 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const uint3
     int sp = 0;
     {
       const double *rc = T.node_cube;
-      if (wb::shape_intersect(H, rc[0], rc[1], rc[2], rc[3])) {
+      if (wb::hyp_intersect(H, rc[0], rc[1], rc[2], rc[3])) {
         if (lane == 0) stk[0] = 0;
         sp = 1;
       }
@@ -1018,7 +1040,7 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const uint3
           const double ncx = cx + (double)(2 * (k & 1) - 1) * side / 4;
           const double ncy = cy + (double)(2 * ((k & 2) / 2) - 1) * side / 4;
           const double ncz = cz + (double)(2 * ((k & 4) / 4) - 1) * side / 4;
-          h = wb::shape_intersect(H, ncx, ncy, ncz, side / 2);
+          h = wb::hyp_intersect(H, ncx, ncy, ncz, side / 2);
         }
         const int t = child_rank(k, o, zf);
         uint32_t rb = h ? 1u << t : 0u;
@@ -1037,14 +1059,16 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const uint3
       if (phase == 0) {
         const uint32_t g0 = b / WB_CHUNK, ng = (e - 1) / WB_CHUNK - g0 + 1;
         for (uint32_t g = lane; g < ng; g += 64) {
-          const uint32_t j = g == 0 ? b : (g0 + g) * WB_CHUNK;
+          const uint32_t c0 = g == 0 ? b : (g0 + g) * WB_CHUNK;
+          uint32_t j = T.chunk_low[g0 + g];
+          if (j < b || j >= e) j = c0;
           const double qx = T.px[j], qy = T.py[j];
           if (wb::hyp_in(H, qx, qy, T.pz[j]) && (qx != x || qy != y))
             occ |= 1ull << wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
         }
       } else {
         const double *lc = T.leaf_cube + 4 * (size_t)leaf;
-        const bool all = wb::shape_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
+        const bool all = wb::hyp_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
         for (uint32_t j = b + lane; j < e; j += 64) {
           const double qx = T.px[j], qy = T.py[j];
           if ((all || wb::hyp_in(H, qx, qy, T.pz[j])) && (qx != x || qy != y)) {
@@ -1236,7 +1260,7 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
     WB_CHECK(hipMemsetAsync(dcount, 0, CT_N * 8, st));
   }
   A.counters = dcount;
-  uint32_t *heavy = (uint32_t *)wb_alloc(c, 4 * n + 4);
+  HeavyRec *heavy = (HeavyRec *)wb_alloc(c, sizeof(HeavyRec) * (n + 1));
   A.heavy = heavy;
   A.nheavy = nslow + 1;
   if (n > 0) {
@@ -1305,13 +1329,12 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
     unsigned long long h[CT_N];
     WB_CHECK(hipMemcpy(h, dcount, sizeof h, hipMemcpyDeviceToHost));
     fprintf(stderr,
-            "[wb counters] lanes A %llu B %llu C %llu heavy %llu | A: leaf-lanes %llu group-lanes "
-            "%llu dirs %llu | B: leaf-lanes %llu point-lanes %llu dirs %llu | per wave: "
-            "expansions A %llu leaves A %llu expansions B %llu leaves B %llu walk C %llu | "
-            "waves %llu\n",
+            "[wb counters] lanes A %llu B %llu open after B %llu heavy %llu | A: leaf-lanes %llu "
+            "group-lanes %llu dirs %llu | B: leaf-lanes %llu point-lanes %llu dirs %llu | per "
+            "wave: expansions A %llu leaves A %llu expansions B %llu leaves B %llu | waves %llu\n",
             h[CT_LANES_A], h[CT_LANES_B], h[CT_LANES_C], h[CT_HEAVY], h[CT_LEAF_A], h[CT_GROUP_A],
             h[CT_DIR_A], h[CT_LEAF_B], h[CT_POINT_B], h[CT_DIR_B], h[CT_EXP_A], h[CT_WLEAF_A],
-            h[CT_EXP_B], h[CT_WLEAF_B], h[CT_WALK_C], h[CT_WAVES]);
+            h[CT_EXP_B], h[CT_WLEAF_B], h[CT_WAVES]);
     int rate = 0;
     hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, c->device);
     fprintf(stderr,
@@ -1335,7 +1358,7 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   wb_free(c, tile_off, 4ull * g.ntiles + 4);
   wb_free(c, lists, 4 * total + 4);
   wb_free(c, cls, n + 16);
-  wb_free(c, heavy, 4 * n + 4);
+  wb_free(c, heavy, sizeof(HeavyRec) * (n + 1));
   out->nheavy = h_nheavy;
   if (h_err[0] & 1u) { wb_free(c, err, 64); throw std::string("a point lies in more than 4 tile cylinders"); }
   if (h_err[0] & 2u) { wb_free(c, err, 64); throw std::string("a tile cylinder holds more than 1024 points"); }

@@ -9,6 +9,8 @@
 //                     input order inside a leaf, as the reference stores it)
 //   crank[n]    u32   sorted slot -> canonical position
 //   chunk_box[ceil(n/32)]  f64x6  bounding box of each 32-slot group
+//   chunk_low[ceil(n/32)]  u32    slot of the group's lowest point (first of
+//                                 equal z)
 //   leaf_start[nl+1] u32, leaf_cube[nl] f64x4 (cx,cy,cz,side), leaf_parent[nl]
 //   node_cube[nn] f64x4, node_child[8*nn] i32 (>=0 node, -2-l leaf, -1 empty),
 //   node_parent[nn] i32
@@ -56,6 +58,7 @@ struct wb_tree {
   double *px, *py, *pz;
   uint32_t *pt_leaf, *canon, *crank;
   double *chunk_box;
+  uint32_t *chunk_low;
   uint32_t *leaf_start;
   double *leaf_cube;
   int32_t *leaf_parent;

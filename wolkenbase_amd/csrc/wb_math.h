@@ -585,4 +585,34 @@ WB_HD bool shape_may_hit_box(const Shape &s, double x0, double x1, double y0, do
   return shape_in(s, qx, qy, qz);
 }
 
+// The three tests above for a shape known to be a hyperboloid (the same
+// operations without the kind dispatch).
+WB_HD bool hyp_in_cube(const Shape &s, double cx, double cy, double cz, double side)
+{
+  double h = side / 2;
+  double x0 = cx - h, x1 = cx + h, y0 = cy - h, y1 = cy + h, z0 = cz - h, z1 = cz + h;
+  bool all = true;
+  for (int n = 0; n < 8; n++) {
+    double qx = (n & 1) ? x1 : x0, qy = (n & 2) ? y1 : y0, qz = (n & 4) ? z1 : z0;
+    all = all && hyp_in(s, qx, qy, qz);
+  }
+  return all;
+}
+
+WB_HD bool hyp_intersect(const Shape &s, double cx, double cy, double cz, double side)
+{
+  double qx = clamp_axis(s.cx, cx, side), qy = clamp_axis(s.cy, cy, side), qz = cz;
+  if (s.slope > 0) qz -= side / 2;
+  if (s.slope < 0) qz += side / 2;
+  return hyp_in(s, qx, qy, qz);
+}
+
+WB_HD bool hyp_may_hit_box(const Shape &s, double x0, double x1, double y0, double y1,
+                           double z0, double z1)
+{
+  double qx = s.cx < x0 ? x0 : (s.cx > x1 ? x1 : s.cx);
+  double qy = s.cy < y0 ? y0 : (s.cy > y1 ? y1 : s.cy);
+  return hyp_in(s, qx, qy, s.slope > 0 ? z0 : z1);
+}
+
 }  // namespace wb
