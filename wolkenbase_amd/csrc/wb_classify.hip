@@ -1069,7 +1069,24 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const Heavy
       } else {
         const double *lc = T.leaf_cube + 4 * (size_t)leaf;
         const bool all = wb::hyp_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
-        for (uint32_t j = b + lane; j < e; j += 64) {
+        // groups whose box may meet the hyperboloid, a group per lane
+        const uint32_t g0 = b / WB_CHUNK, ng = (e - 1) / WB_CHUNK - g0 + 1;
+        uint64_t gmask = ~0ull;
+        if (!all && ng <= WAVE) {
+          bool gh = false;
+          if ((uint32_t)lane < ng) {
+            const double *bx = T.chunk_box + 6 * (uint64_t)(g0 + lane);
+            gh = wb::hyp_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+          }
+          gmask = __ballot(gh);
+        }
+        // 64 slots a step from the first group's start: groups ga and ga+1
+        for (uint32_t j0 = g0 * WB_CHUNK; j0 < e; j0 += WAVE) {
+          const uint32_t ga = j0 / WB_CHUNK - g0;
+          if (ga < 63 && ((gmask >> ga) & 3ull) == 0) continue;
+          const uint32_t j = j0 + lane;
+          const uint32_t g = ga + lane / WB_CHUNK;
+          if (j < b || j >= e || (g < 64 && !((gmask >> g) & 1))) continue;
           const double qx = T.px[j], qy = T.py[j];
           if ((all || wb::hyp_in(H, qx, qy, T.pz[j])) && (qx != x || qy != y)) {
             const int bin = wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
