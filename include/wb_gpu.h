@@ -100,6 +100,8 @@ typedef struct {
 
 /* --- context ------------------------------------------------------------ */
 int wb_ctx_create(int device, wb_ctx **out);
+/* also frees every tree built on the context that is still alive: their
+ * handles are invalid afterwards and must not be passed to wb_tree_free */
 void wb_ctx_destroy(wb_ctx *ctx);
 const char *wb_last_error(wb_ctx *ctx);
 /* synchronise the context's stream */

@@ -106,9 +106,12 @@ def test_count_and_points_in(ctx, kind):
         assert np.array_equal(lists[s].astype(np.int64), ref.points_in(kind, P[s])), s
 
 
+# the third cloud is dense enough for tiles of 65-128, 129-256 and more than
+# 256 points (each size class of the scan has its own kernel)
 @pytest.mark.parametrize("cfg", [dict(seed=21, extent=24.0, density=25.0),
                                  dict(seed=22, extent=30.0, density=20.0, embankments=2,
-                                      outliers=0.02)])
+                                      outliers=0.02),
+                                 dict(seed=26, extent=12.0, density=500.0, buildings=1, trees=20)])
 def test_classify_matches_oracle(ctx, cfg):
     cloud = synth.generate(**cfg)
     tree, _, _ = build_both(ctx, cloud)
@@ -120,3 +123,19 @@ def test_classify_matches_oracle(ctx, cfg):
     for f in ("density", "hs_scan", "hs", "height"):
         assert np.array_equal(tiles[f], rtiles[f]), f
     assert int((cls != rcls).sum()) == 0
+    if cfg["seed"] == 26:
+        m = tiles["npoints"]
+        assert ((m > 64) & (m <= 128)).any() and ((m > 128) & (m <= 256)).any() and (m > 256).any()
+
+
+def test_context_close_frees_live_trees():
+    # a tree still alive when its context closes is freed with it; freeing
+    # it afterwards is a no-op, not a use of the closed context
+    c = W.Context(0)
+    cloud = synth.generate(seed=31, extent=10.0, density=25.0)
+    center, side = W.Octree.size_fit(cloud["bbmin"], cloud["bbmax"])
+    tree = W.Octree.build(c, cloud["xi"], cloud["yi"], cloud["zi"], cloud["scale"],
+                          cloud["offset"], center, side)
+    c.close()
+    assert tree.h is None
+    tree.free()

@@ -11,6 +11,7 @@ GPU entry point without the built library raises.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -129,6 +130,7 @@ class Context:
         if rc != 0:
             raise WbError(f"wb_ctx_create({device}) failed with {rc}")
         self.h = h
+        self.trees = weakref.WeakSet()
 
     def check(self, rc, what):
         if rc != 0:
@@ -138,7 +140,10 @@ class Context:
         self.check(lib().wb_ctx_sync(self.h), "wb_ctx_sync")
 
     def close(self):
+        """Frees the trees still alive on this context, then the context."""
         if getattr(self, "h", None):
+            for t in list(getattr(self, "trees", ())):
+                t.free()
             lib().wb_ctx_destroy(self.h)
             self.h = None
 
@@ -216,15 +221,17 @@ class Octree:
                                  WB_DEVICE_INPUT if dev else 0, C.byref(h))
         ctx.check(rc, "wb_tree_build")
         self.h = h
+        ctx.trees.add(self)
         self.n = n
         self.center = cen
         self.side = float(side)
         return self
 
     def free(self):
-        if getattr(self, "h", None):
+        # a closed context has already freed its trees
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
             lib().wb_tree_free(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

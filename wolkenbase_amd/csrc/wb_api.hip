@@ -102,6 +102,12 @@ void wb_ctx_destroy(wb_ctx *c)
   if (!c) return;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
+  // trees still alive go with their context
+  for (wb_tree *t : c->trees) {
+    for (auto &o : t->owned) hipFree(o.first);
+    delete t;
+  }
+  c->trees.clear();
   for (auto &kv : c->cache) hipFree(kv.second);
   hipFree(c->d_tan);
   hipStreamDestroy(c->stream);
@@ -176,6 +182,7 @@ int wb_tree_build(wb_ctx *c, const int32_t *x, const int32_t *y, const int32_t *
   }
   if (tmp) wb_free(c, tmp, 12 * n);
   WB_CHECK(hipStreamSynchronize(c->stream));
+  c->trees.insert(t);
   *out = t;
   return WB_OK;
   API_CATCH(c)
@@ -187,6 +194,7 @@ int wb_tree_free(wb_tree *t)
   wb_ctx *c = t->ctx;
   hipStreamSynchronize(c->stream);
   for (auto &o : t->owned) wb_free(c, o.first, o.second);
+  c->trees.erase(t);
   delete t;
   return WB_OK;
 }

@@ -229,7 +229,18 @@ __global__ void k_tile_geo(TileGeo g, int32_t *tex, int32_t *tey, int *bounds)
     y0 = min(y0, __shfl_xor(y0, s, 64));
     y1 = max(y1, __shfl_xor(y1, s, 64));
   }
+  // one atomic per block and bound
+  __shared__ int wb4[4][256 / 64];
+  const int w = threadIdx.x / 64;
   if (__lane_id() == 0) {
+    wb4[0][w] = x0; wb4[1][w] = x1; wb4[2][w] = y0; wb4[3][w] = y1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)blockDim.x / 64; k++) {
+      x0 = min(x0, wb4[0][k]); x1 = max(x1, wb4[1][k]);
+      y0 = min(y0, wb4[2][k]); y1 = max(y1, wb4[3][k]);
+    }
     atomicMin(&bounds[0], x0);
     atomicMax(&bounds[1], x1);
     atomicMin(&bounds[2], y0);
@@ -251,6 +262,27 @@ __global__ void k_tile_grid(TileGeo g, const int32_t *tex, const int32_t *tey, i
 }
 
 #define MAX_MEMB 4
+
+// cnt[key] += 1 for every active lane with `has`, one atomic per distinct key
+// in the wave; returns the lane's slot (the old count plus its rank among
+// the lanes with the same key)
+__device__ inline uint32_t wave_count_add(uint32_t *cnt, bool has, int key)
+{
+  uint32_t slot = 0;
+  uint64_t todo = __ballot(has);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int kl = __shfl(key, leader, 64);
+    const bool same_key = has && key == kl;
+    const uint64_t same = __ballot(same_key);
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(&cnt[kl], (uint32_t)__popcll(same));
+    base = (uint32_t)__shfl((int)base, leader, 64);
+    if (same_key) slot = base + (uint32_t)__popcll(same & ((1ull << __lane_id()) - 1));
+    todo &= ~same;
+  }
+  return slot;
+}
 
 // pointsIn(cylinder) membership of every point, evaluated from the point's
 // side: the candidates are the lattice tiles near it; a tile holds the point
@@ -301,7 +333,8 @@ __global__ void k_membership(TreeView T, TileGeo g, int32_t *memb, uint32_t *nme
   }
   for (int k = 0; k < MAX_MEMB; k++) memb[MAX_MEMB * i + k] = k < nm ? mem[k] : INT32_MIN;
   nmemb[i] = nm;
-  for (int k = 0; k < nm; k++) atomicAdd(&tile_cnt[mem[k] - g.lo], 1u);
+#pragma unroll
+  for (int k = 0; k < MAX_MEMB; k++) wave_count_add(tile_cnt, k < nm, k < nm ? mem[k] - g.lo : 0);
 }
 
 __global__ void k_tile_lists(uint64_t n, const int32_t *memb, const uint32_t *nmemb, int lo,
@@ -310,11 +343,13 @@ __global__ void k_tile_lists(uint64_t n, const int32_t *memb, const uint32_t *nm
 {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint32_t nm = nmemb[i];
-  for (uint32_t k = 0; k < nm; k++) {
-    int t = memb[MAX_MEMB * i + k] - lo;
-    uint32_t slot = atomicAdd(&tile_fill[t], 1u);
-    lists[tile_off[t] + slot] = crank[i];
+  const uint32_t nm = nmemb[i];
+#pragma unroll
+  for (uint32_t k = 0; k < MAX_MEMB; k++) {
+    const bool has = k < nm;
+    const int t = has ? memb[MAX_MEMB * i + k] - lo : 0;
+    const uint32_t slot = wave_count_add(tile_fill, has, t);
+    if (has) lists[tile_off[t] + slot] = crank[i];
   }
 }
 
@@ -367,105 +402,361 @@ __device__ inline double pw3(double a0, double a1, double a2, int n)
   return (0 + a2) + (a0 + a1);
 }
 
-// 3x3 Gauss-Jordan of matrix.cpp:262-422 on a (3x3) and b (3x1)
-__device__ inline void swap_rows3(double *a, double *b, int r0, int r1)
+// 3x3 Gauss-Jordan of matrix.cpp:262-422 on a (3x3, row-major) and b (3x1).
+// Every row and column index is a compile-time constant, so the system stays
+// in registers; the pivot choices made at run time select among them.
+// rows R0 and R1 exchanged when `c`, by selects (no indexed access)
+template <int R0, int R1>
+__device__ __attribute__((always_inline)) inline void swap_rows_s(double (&a)[9], double (&b)[3],
+                                                                  bool c)
 {
-  for (int c = 0; c < 3; c++) { double t = a[3 * r0 + c]; a[3 * r0 + c] = a[3 * r1 + c]; a[3 * r1 + c] = t; }
-  double t = b[r0]; b[r0] = b[r1]; b[r1] = t;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double x = a[3 * R0 + k], y = a[3 * R1 + k];
+    a[3 * R0 + k] = c ? y : x;
+    a[3 * R1 + k] = c ? x : y;
+  }
+  const double x = b[R0], y = b[R1];
+  b[R0] = c ? y : x;
+  b[R1] = c ? x : y;
 }
 
-__device__ inline void rowop3(double *a, double *b, int row0, int row1, int piv)
+__device__ inline double sel3(double r0, double r1, double r2, int k)
+{
+  return k == 0 ? r0 : (k == 1 ? r1 : r2);
+}
+
+// rowop(row0, row1, piv) of matrix.cpp
+template <int ROW0, int ROW1, int PIV>
+__device__ __attribute__((always_inline)) inline void rowop_s(double (&a)[9], double (&b)[3])
 {
   double slope = 0, minslope = __builtin_inf(), det;
   int flags = 0, pivot;
-  double *r0 = a + 3 * row0, *r1 = a + 3 * row1;
-  if (piv >= 0 && r0[piv] == 0 && r1[piv] == 0) piv = -1;
-  pivot = piv;
-  if (piv >= 0 && r0[piv] == 0) flags = 9;
-  for (int i = 0; piv < 0 && i < 3; i++)
-    if (r0[i] != 0 || r1[i] != 0) {
-      if (wb::dabs(r0[i]) > wb::dabs(r1[i]) || row0 >= row1) {
-        slope = wb::dabs(r1[i] / r0[i]);
-        flags &= ~8;
-      } else {
-        slope = wb::dabs(r0[i] / r1[i]);
-        flags |= 8;
-      }
-      if (slope < minslope) {
-        minslope = slope;
-        flags = (flags >> 3) * 9;
-        pivot = i;
+  bool piv_ok = true;  // piv >= 0
+  if (a[3 * ROW0 + PIV] == 0 && a[3 * ROW1 + PIV] == 0) piv_ok = false;
+  pivot = piv_ok ? PIV : -1;
+  if (piv_ok && a[3 * ROW0 + PIV] == 0) flags = 9;
+  if (!piv_ok) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const double x0 = a[3 * ROW0 + i], x1 = a[3 * ROW1 + i];
+      if (x0 != 0 || x1 != 0) {
+        if (wb::dabs(x0) > wb::dabs(x1) || ROW0 >= ROW1) {
+          slope = wb::dabs(x1 / x0);
+          flags &= ~8;
+        } else {
+          slope = wb::dabs(x0 / x1);
+          flags |= 8;
+        }
+        if (slope < minslope) {
+          minslope = slope;
+          flags = (flags >> 3) * 9;
+          pivot = i;
+        }
       }
     }
-  flags &= 1;
-  if (flags) swap_rows3(a, b, row0, row1);
-  det = (pivot < 0) ? 0 : r0[pivot];
-  if (det != 0 && det != 1) {
-    for (int i = 0; i < 3; i++) r0[i] /= det;
-    b[row0] /= det;
   }
-  if (pivot >= 0) slope = r1[pivot];
-  if (slope != 0 && row0 != row1) {
-    for (int i = 0; i < 3; i++) r1[i] -= r0[i] * slope;
-    b[row1] -= b[row0] * slope;
+  flags &= 1;
+  swap_rows_s<ROW0, ROW1>(a, b, flags != 0);
+  det = (pivot < 0) ? 0 : sel3(a[3 * ROW0], a[3 * ROW0 + 1], a[3 * ROW0 + 2], pivot);
+  if (det != 0 && det != 1) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) a[3 * ROW0 + i] /= det;
+    b[ROW0] /= det;
+  }
+  if (pivot >= 0) slope = sel3(a[3 * ROW1], a[3 * ROW1 + 1], a[3 * ROW1 + 2], pivot);
+  if (slope != 0 && ROW0 != ROW1) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) a[3 * ROW1 + i] -= a[3 * ROW0 + i] * slope;
+    b[ROW1] -= b[ROW0] * slope;
   }
 }
 
-__device__ inline void findpivot3(double *a, double *b, int row, int column)
+// findpivot(row, column) of matrix.cpp
+template <int ROW>
+__device__ __attribute__((always_inline)) inline void findpivot_s(double (&a)[9], double (&b)[3])
 {
   int pivotrow = -1;
   double maxratio = 0;
-  for (; pivotrow < row && column < 3; column++) {
-    for (int i = row; i < 3; i++) {
-      double sq[3] = {0, 0, 0};
-      for (int j = column + 1; j < 3; j++) sq[j - column - 1] = a[3 * i + j] * a[3 * i + j];
-      double piv = a[3 * i + column];
-      double ratio = (piv * piv) / pw3(sq[0], sq[1], sq[2], 3 - column);
-      if (ratio > maxratio) {
-        pivotrow = i;
-        maxratio = ratio;
+  // the column loop runs while pivotrow < ROW; pivotrow, once set, is >= ROW
+#pragma unroll
+  for (int column = ROW; column < 3; column++) {
+    if (pivotrow < ROW) {
+#pragma unroll
+      for (int i = ROW; i < 3; i++) {
+        const double s0 = column + 1 < 3 ? a[3 * i + column + 1] * a[3 * i + column + 1] : 0.0;
+        const double s1 = column + 2 < 3 ? a[3 * i + column + 2] * a[3 * i + column + 2] : 0.0;
+        const double piv = a[3 * i + column];
+        const double ratio = (piv * piv) / pw3(s0, s1, 0.0, 3 - column);
+        if (ratio > maxratio) {
+          pivotrow = i;
+          maxratio = ratio;
+        }
       }
     }
   }
-  if (pivotrow > row) swap_rows3(a, b, pivotrow, row);
+  if constexpr (ROW < 1) swap_rows_s<1, ROW>(a, b, pivotrow == 1);
+  if constexpr (ROW < 2) swap_rows_s<2, ROW>(a, b, pivotrow == 2);
 }
 
-__device__ inline void gausselim3(double *a, double *b)
+__device__ __attribute__((always_inline)) inline void gausselim_s(double (&a)[9], double (&b)[3])
 {
-  for (int i = 0; i < 3; i++) {
-    findpivot3(a, b, i, i);
-    for (int j = 0; j < 3; j++) rowop3(a, b, i, j, i);
-  }
-  for (int i = 2; i >= 0; i--)
-    for (int j = 0; j < i; j++) rowop3(a, b, i, j, i);
+  findpivot_s<0>(a, b);
+  rowop_s<0, 0, 0>(a, b); rowop_s<0, 1, 0>(a, b); rowop_s<0, 2, 0>(a, b);
+  findpivot_s<1>(a, b);
+  rowop_s<1, 0, 1>(a, b); rowop_s<1, 1, 1>(a, b); rowop_s<1, 2, 1>(a, b);
+  findpivot_s<2>(a, b);
+  rowop_s<2, 0, 2>(a, b); rowop_s<2, 1, 2>(a, b); rowop_s<2, 2, 2>(a, b);
+  // back substitution
+  rowop_s<2, 0, 2>(a, b); rowop_s<2, 1, 2>(a, b);
+  rowop_s<1, 0, 1>(a, b);
 }
 
-__global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_t *tex,
-                                             const int32_t *tey, const uint32_t *tile_off,
-                                             const uint32_t *tile_cnt, const uint32_t *lists,
-                                             double min_hs, TileRec *rec, uint32_t *err,
-                                             unsigned long long *ctr)
+// the tilt (sx, sy) of the least-squares plane z = sx*x + sy*y + c through
+// the normal equations (scan.cpp:53-73, leastsquares.cpp); every lane holds the same sums and
+// gets the same answer
+__device__ __attribute__((always_inline)) inline void plane_tilt(double sxx, double syx, double syy, double s1x, double s1y,
+                                  double s11, double sxz, double syz, double s1z, double *sx,
+                                  double *sy)
 {
-  __shared__ uint32_t cp[SCAN_CAP];
-  __shared__ int histo[7];
-  const uint32_t t = blockIdx.x;
-  if (t >= g.ntiles) return;
-  const int lane = threadIdx.x;
-  const uint32_t m = tile_cnt[t];
-  if (m == 0) {
-    if (lane == 0) {
-      TileRec r = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0};
-      rec[t] = r;
+  double a[9] = {sxx, syx, s1x, syx, syy, s1y, s1x, s1y, s11};
+  double b[3] = {sxz, syz, s1z};
+  gausselim_s(a, b);
+  if (a[0] == 0) b[0] = __builtin_nan("");
+  if (a[4] == 0) b[1] = __builtin_nan("");
+  if (a[8] == 0) b[2] = __builtin_nan("");
+  double p = b[0], q = b[1];
+  const double len = wb::hypot(p, q);
+  if (len > 1) { p /= len; q /= len; }
+  if (wb::disnan(p) || wb::disnan(q)) { p = 0; q = 0; }
+  *sx = p;
+  *sy = q;
+}
+
+// Slot s of a K-slot per-lane array, s wave-uniform.
+template <int K, class V>
+__device__ inline V slot_sel(const V (&v)[K], uint32_t s)
+{
+  V r = v[0];
+#pragma unroll
+  for (int k = 1; k < K; k++)
+    if (s == (uint32_t)k) r = v[k];
+  return r;
+}
+
+// wave_pairwise for m <= 64*K values held K per lane (value e in slot e/64
+// of lane e%64): the same block trees and block order.
+template <int K>
+__device__ inline double wave_pairwise_reg(uint32_t m, const double (&v)[K])
+{
+  const int lane = __lane_id();
+  double acc = 0;
+  for (int k = 0; k < 9; k++) {
+    if (!((m >> k) & 1u)) continue;
+    const uint32_t off = (m >> (k + 1)) << (k + 1);
+    const uint32_t sz = 1u << k;
+    double blk;
+    if (sz <= WAVE) {
+      // the block lies inside slot off/64
+      const double vs = slot_sel<K>(v, off / WAVE);
+      double b = __shfl(vs, (int)((off + lane) & (WAVE - 1)), WAVE);
+      b = (uint32_t)lane < sz ? b : 0.0;
+      for (uint32_t s = 1; s < sz; s <<= 1) b = b + __shfl_xor(b, (int)s, WAVE);
+      blk = __shfl(b, 0, WAVE);
+    } else {
+      // chunk sums of 64 (chunk ch = slot off/64 + ch), then a balanced tree
+      const uint32_t nch = sz / WAVE;
+      double mine = 0;
+      for (uint32_t ch = 0; ch < nch; ch++) {
+        double b = slot_sel<K>(v, off / WAVE + ch);
+        for (int s = 1; s < WAVE; s <<= 1) b = b + __shfl_xor(b, s, WAVE);
+        if ((uint32_t)lane == ch) mine = b;
+      }
+      for (uint32_t s = 1; s < nch; s <<= 1) mine = mine + __shfl_xor(mine, (int)s, WAVE);
+      blk = __shfl(mine, 0, WAVE);
     }
-    return;
+    acc += blk;
   }
-  if (m > SCAN_CAP) {
-    if (lane == 0) atomicOr(err, 2u);
-    return;
+  return acc;
+}
+
+// The scan of a tile of at most 64*K points, K per lane (point e in slot
+// e/64 of lane e%64): the same operations and summation order as the
+// general path of k_scan, with the canonical sort done by a bitonic network
+// over lanes and slots and every point read once.
+template <int K>
+__device__ __attribute__((always_inline)) inline void scan_reg(const TreeView &T, const TileGeo &g,
+                                                                int ex, int ey, uint32_t off,
+                         uint32_t m, const uint32_t *lists, double min_hs, TileRec *out,
+                         unsigned long long *ctr)
+{
+  const uint32_t lane = __lane_id();
+  uint32_t c[K];
+#pragma unroll
+  for (int s = 0; s < K; s++) {
+    const uint32_t e = s * WAVE + lane;
+    c[s] = e < m ? lists[off + e] : 0xffffffffu;
   }
+  for (uint32_t k = 2; k <= (uint32_t)(WAVE * K); k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j < WAVE) {
+#pragma unroll
+        for (int s = 0; s < K; s++) {
+          const uint32_t e = s * WAVE + lane;
+          const uint32_t o = (uint32_t)__shfl_xor((int)c[s], (int)j, WAVE);
+          const bool up = (e & k) == 0, low = (e & j) == 0;
+          c[s] = (low == up) ? min(c[s], o) : max(c[s], o);
+        }
+      } else {
+        const uint32_t js = j / WAVE;
+#pragma unroll
+        for (int s = 0; s < K; s++) {
+#pragma unroll
+          for (int s2 = 0; s2 < K; s2++) {
+            if (s2 != (s | (int)js) || (s & (int)js)) continue;
+            const uint32_t e = s * WAVE + lane;
+            const bool up = (e & k) == 0;
+            const uint32_t a = c[s], b = c[s2];
+            if ((a > b) == up) { c[s] = b; c[s2] = a; }
+          }
+        }
+      }
+    }
+  double cx, cy;
+  tile_center(g, ex, ey, &cx, &cy);
+  const double r = g.r;
+  double vx[K], vy[K], vz[K];
+  bool mine[K];
+#pragma unroll
+  for (int s = 0; s < K; s++) {
+    mine[s] = s * WAVE + lane < m;
+    vx[s] = 0; vy[s] = 0; vz[s] = 0;
+    if (mine[s]) {
+      const uint32_t q = T.canon[c[s]];
+      vx[s] = T.px[q] - cx;
+      vy[s] = T.py[q] - cy;
+      vz[s] = T.pz[q] - 0;
+    }
+  }
+  double t[K];
+#define WB_SUM(expr) (([&]() { _Pragma("unroll") for (int s = 0; s < K; s++) t[s] = (expr); }()), \
+                      wave_pairwise_reg<K>(m, t))
+  const double sxx = WB_SUM(vx[s] * vx[s]);
+  const double syx = WB_SUM(vy[s] * vx[s]);
+  const double syy = WB_SUM(vy[s] * vy[s]);
+  const double s1x = WB_SUM(1.0 * vx[s]);
+  const double s1y = WB_SUM(1.0 * vy[s]);
+  const double s11 = WB_SUM(1.0 * 1.0);
+  const double sxz = WB_SUM(vx[s] * vz[s]);
+  const double syz = WB_SUM(vy[s] * vz[s]);
+  const double s1z = WB_SUM(1.0 * vz[s]);
+#undef WB_SUM
+  double sx, sy;
+  plane_tilt(sxx, syx, syy, s1x, s1y, s11, sxz, syz, s1z, &sx, &sy);
+  double u[K];
+  double bottom = __builtin_inf(), top = -__builtin_inf();
+#pragma unroll
+  for (int s = 0; s < K; s++) {
+    u[s] = vz[s] - (sy * vy[s] + sx * vx[s]);
+    if (mine[s]) {
+      bottom = fmin(bottom, u[s]);
+      top = fmax(top, u[s]);
+    }
+  }
+  for (int s = 1; s < WAVE; s <<= 1) {
+    bottom = fmin(bottom, __shfl_xor(bottom, s, WAVE));
+    top = fmax(top, __shfl_xor(top, s, WAVE));
+  }
+  uint32_t first = 0xffffffffu;
+#pragma unroll
+  for (int s = K - 1; s >= 0; s--) {
+    const uint64_t atb = __ballot(mine[s] && u[s] == bottom);
+    if (atb) first = s * WAVE + (uint32_t)(__ffsll((long long)atb) - 1);
+  }
+  double bottom2 = __builtin_inf();
+#pragma unroll
+  for (int s = 0; s < K; s++)
+    if (mine[s] && s * WAVE + lane < first) bottom2 = fmin(bottom2, u[s]);
+  for (int s = 1; s < WAVE; s <<= 1) bottom2 = fmin(bottom2, __shfl_xor(bottom2, s, WAVE));
+  if (wb::disinf(bottom2)) bottom2 = bottom;
+  // 7-sector histogram of the bottom layer
+  uint32_t nbottom = 0, nedge = 0, hist[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < K; s++) {
+    int sec = -1;
+    bool edge = false;
+    if (mine[s] && u[s] < bottom2 + 2 * r) {
+      const double th = atan2(vy[s], vx[s]) * 3 / M_PI;
+      long sc = lrint(th);
+      const double fr = th - floor(th);
+      edge = wb::dabs(fr - 0.5) < 1e-12;
+      if (sc < 0) sc += 6;
+      sc = (sc % 6) + 1;
+      if (wb::hypot(vx[s], vy[s]) < r / kSqrt7) sc = 0;
+      sec = (int)sc;
+    }
+    nbottom += (uint32_t)__popcll(__ballot(sec >= 0));
+    nedge += (uint32_t)__popcll(__ballot(edge));
+    for (int k = 0; k < 7; k++) hist[k] += (uint32_t)__popcll(__ballot(sec == k));
+  }
+  if (lane == 0) {
+    if (nedge) atomicAdd(&ctr[0], (unsigned long long)nedge);
+    double density = 0;
+    int tree = 0;
+    for (int i = 0; i < 7; i++) density += (int)hist[i] * (int)hist[i];
+    if (m > nbottom && density < 7) tree = 1;
+    density = sqrt(density) * kSqrt7 / (r * r) / M_PI;
+    if (m > nbottom && density < 0.5) tree = 1;
+    if (top - bottom > 1.5) tree = 1;
+    TileRec rr;
+    rr.npoints = (int32_t)m;
+    rr.nground = 0;
+    rr.tree_flags = tree;
+    rr.pad = 0;
+    rr.density = density;
+    rr.hs_scan = sqrt(1 / density + min_hs * min_hs);
+    rr.hs = rr.hs_scan;
+    rr.height = top - bottom;
+    *out = rr;
+  }
+}
+
+// One wave per tile, over the tiles of one size class: up to 64, 65-128 and
+// 129-256 points (K = 1, 2, 4 points per lane; the K = 1 kernel also writes
+// the empty tiles).  Each class has its own register budget.
+#ifndef WB_SCAN_MINW
+#define WB_SCAN_MINW 4
+#endif
+template <int K>
+__global__ void __launch_bounds__(64, WB_SCAN_MINW) k_scan_reg(TreeView T, TileGeo g, const int32_t *tex,
+                                                 const int32_t *tey, const uint32_t *tile_off,
+                                                 const uint32_t *tile_cnt, const uint32_t *lists,
+                                                 double min_hs, TileRec *rec,
+                                                 unsigned long long *ctr)
+{
+  for (uint32_t t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
+    const uint32_t m = tile_cnt[t];
+    if (K == 1 && m == 0) {
+      if (threadIdx.x == 0) {
+        TileRec r = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0};
+        rec[t] = r;
+      }
+      continue;
+    }
+    const uint32_t below = K == 1 ? 0u : (uint32_t)(WAVE * K / 2);  // the next smaller class
+    if (m <= below || m > (uint32_t)(WAVE * K)) continue;
+    scan_reg<K>(T, g, tex[t], tey[t], tile_off[t], m, lists, min_hs, rec + t, ctr);
+  }
+}
+
+// the scan of one tile of up to SCAN_CAP points by one wave (LDS sort)
+__device__ void scan_big(const TreeView &T, const TileGeo &g, int ex, int ey, uint32_t off,
+                         uint32_t m, const uint32_t *lists, double min_hs, TileRec *out,
+                         unsigned long long *ctr, uint32_t *cp, int *histo)
+{
+  const int lane = threadIdx.x;
   uint32_t P = 1;
   while (P < m) P <<= 1;
-  const uint32_t off = tile_off[t];
   for (uint32_t j = lane; j < P; j += WAVE) cp[j] = j < m ? lists[off + j] : 0xffffffffu;
   if (lane < 7) histo[lane] = 0;
   __syncthreads();
@@ -483,7 +774,7 @@ __global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_
       __syncthreads();
     }
   double cx, cy;
-  tile_center(g, tex[t], tey[t], &cx, &cy);
+  tile_center(g, ex, ey, &cx, &cy);
   const double r = g.r;
   // the tile's points relative to the cylinder axis, in canonical order
   // (location - xyz(cyl centre, 0)); read through the canonical map
@@ -500,26 +791,8 @@ __global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_
   double sxz = wave_pairwise(m, [&](uint32_t j) { return VX(j) * VZ(j); });
   double syz = wave_pairwise(m, [&](uint32_t j) { return VY(j) * VZ(j); });
   double s1z = wave_pairwise(m, [&](uint32_t j) { return 1.0 * VZ(j); });
-  // the 3x3 solve runs once, on lane 0, with the matrix in LDS (row swaps
-  // index it at run time)
-  __shared__ double gA[9], gB[3], gS[2];
-  if (lane == 0) {
-    gA[0] = sxx; gA[1] = syx; gA[2] = s1x;
-    gA[3] = syx; gA[4] = syy; gA[5] = s1y;
-    gA[6] = s1x; gA[7] = s1y; gA[8] = s11;
-    gB[0] = sxz; gB[1] = syz; gB[2] = s1z;
-    gausselim3(gA, gB);
-    for (int i = 0; i < 3; i++)
-      if (gA[3 * i + i] == 0) gB[i] = __builtin_nan("");
-    double a = gB[0], b = gB[1];
-    double len = wb::hypot(a, b);
-    if (len > 1) { a /= len; b /= len; }
-    if (wb::disnan(a) || wb::disnan(b)) { a = 0; b = 0; }
-    gS[0] = a;
-    gS[1] = b;
-  }
-  __syncthreads();
-  const double sx = gS[0], sy = gS[1];
+  double sx, sy;
+  plane_tilt(sxx, syx, syy, s1x, s1y, s11, sxz, syz, s1z, &sx, &sy);
   // untilted height; bottom, bottom2 (the running minimum before its last
   // update), top
   auto U = [&](uint32_t j) { return VZ(j) - (sy * VY(j) + sx * VX(j)); };
@@ -575,20 +848,44 @@ __global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_
     rr.hs_scan = sqrt(1 / density + min_hs * min_hs);
     rr.hs = rr.hs_scan;
     rr.height = top - bottom;
-    rec[t] = rr;
-    atomicAdd(&ctr[5], 1ull);  // non-empty tiles
+    *out = rr;
+  }
+  __syncthreads();
+}
+
+// Tiles of more than 256 points: the canonical sort in LDS and every value
+// read through the canonical map.
+__global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_t *tex,
+                                             const int32_t *tey, const uint32_t *tile_off,
+                                             const uint32_t *tile_cnt, const uint32_t *lists,
+                                             double min_hs, TileRec *rec, uint32_t *err,
+                                             unsigned long long *ctr)
+{
+  __shared__ uint32_t cp[SCAN_CAP];
+  __shared__ int histo[7];
+  for (uint32_t t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
+    const uint32_t m = tile_cnt[t];
+    if (m <= 4 * WAVE) continue;
+    if (m > SCAN_CAP) {
+      if (threadIdx.x == 0) atomicOr(err, 2u);
+      continue;
+    }
+    scan_big(T, g, tex[t], tey[t], tile_off[t], m, lists, min_hs, rec + t, ctr, cp, histo);
   }
 }
 
 // POSTSCAN (scan.cpp:142-179): tree tiles widen their hyperboloid by the
 // number of tree tiles met along six rays before an empty ring or a
 // non-tree tile.
-__global__ void k_postscan(TileGeo g, const int32_t *tex, const int32_t *tey, TileRec *rec)
+__global__ void k_postscan(TileGeo g, const int32_t *tex, const int32_t *tey, TileRec *rec,
+                           unsigned long long *nonempty)
 {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= g.ntiles) return;
+  const bool live = t < g.ntiles && rec[t].npoints;
+  const uint64_t lv = __ballot(live);
+  if (__lane_id() == 0 && lv) atomicAdd(nonempty, (unsigned long long)__popcll(lv));
+  if (!live) return;
   TileRec me = rec[t];
-  if (!me.npoints) return;
   const int r1x[6] = {1, 1, 0, -1, -1, 0}, r1y[6] = {0, 1, 1, 0, -1, -1};  // root1, eisenstein.cpp:51
   int i = 1, ring, nontree, count = 0;
   do {
@@ -1258,9 +1555,22 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
     hipLaunchKernelGGL(k_tile_lists, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, memb, nmemb,
                        g.lo, t->crank, tile_off, tile_fill, lists);
   WB_CHECK(hipEventRecord(ev[1], st));
-  hipLaunchKernelGGL(k_scan, dim3(g.ntiles), dim3(64), 0, st, T, g, tex, tey, tile_off, tile_cnt,
-                     lists, p->min_hs, rec, err, ctr);
-  hipLaunchKernelGGL(k_postscan, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey, rec);
+  {
+    // one wave per tile of each size class; grid-stride over the tiles
+    const unsigned nb = (unsigned)std::min<uint64_t>(g.ntiles, 1u << 16);
+    if (nb) {
+      hipLaunchKernelGGL(k_scan_reg<1>, dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,
+                         tile_cnt, lists, p->min_hs, rec, ctr);
+      hipLaunchKernelGGL(k_scan_reg<2>, dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,
+                         tile_cnt, lists, p->min_hs, rec, ctr);
+      hipLaunchKernelGGL(k_scan_reg<4>, dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,
+                         tile_cnt, lists, p->min_hs, rec, ctr);
+      hipLaunchKernelGGL(k_scan, dim3(std::min(nb, 4096u)), dim3(64), 0, st, T, g, tex, tey,
+                         tile_off, tile_cnt, lists, p->min_hs, rec, err, ctr);
+    }
+  }
+  hipLaunchKernelGGL(k_postscan, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey, rec,
+                     ctr + 5);
   WB_CHECK(hipEventRecord(ev[2], st));
   // classify
   uint8_t *cls = (uint8_t *)wb_alloc(c, n + 16);

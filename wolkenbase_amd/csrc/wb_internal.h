@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,8 @@ struct wb_ctx {
   // size-keyed cache of device buffers
   std::multimap<size_t, void *> cache;
   size_t cached_bytes;
+  // trees built on this context and not yet freed
+  std::set<struct wb_tree *> trees;
 };
 
 struct wb_tree {
