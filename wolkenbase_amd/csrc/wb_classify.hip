@@ -953,10 +953,28 @@ __device__ inline int gap_verdict(uint64_t occupied)
 
 __device__ inline int dir_bin_pos(int a) { return (int)(((uint32_t)a + 0x40000000u) & 0x7fffffffu); }
 
+// OR over the 64 lanes by DPP row operations: lanes of a row swap within
+// quads, then rotate by 4 and 8, so every lane holds its row's OR; the two
+// row broadcasts fold row r-1 into row r and row 1 into rows 2 and 3, so
+// lane 63 holds the OR of all lanes.  Lanes a broadcast does not write add 0.
+template <int CTRL, int ROWS>
+__device__ inline uint32_t dpp_or(uint32_t v)
+{
+  return v | (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xf, false);
+}
+__device__ inline uint32_t wave_or32(uint32_t v)
+{
+  v = dpp_or<0xb1, 0xf>(v);   // quad_perm 1,0,3,2
+  v = dpp_or<0x4e, 0xf>(v);   // quad_perm 2,3,0,1
+  v = dpp_or<0x124, 0xf>(v);  // row_ror 4
+  v = dpp_or<0x128, 0xf>(v);  // row_ror 8
+  v = dpp_or<0x142, 0xa>(v);  // row_bcast 15 into rows 1 and 3
+  v = dpp_or<0x143, 0xc>(v);  // row_bcast 31 into rows 2 and 3
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
 __device__ inline uint64_t wave_or(uint64_t v)
 {
-  for (int s = 1; s < 64; s <<= 1) v |= __shfl_xor(v, s, 64);
-  return v;
+  return ((uint64_t)wave_or32((uint32_t)(v >> 32)) << 32) | wave_or32((uint32_t)v);
 }
 
 __device__ inline int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -1268,19 +1286,30 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
 }
 
 // One heavy point per 64-lane block.  The shape is the same in every lane,
-// so lanes split the work instead: lanes 0-7 test the eight children of a
-// node, and the lanes share a leaf's sampled groups (walk A) or its points
-// (walk B, and C, the exact border pass for a longest empty run of 24 or 25
-// bins).  Same visiting order as k_ground.
+// so the lanes split the work.  The walk (lanes 0-7 test the eight children
+// of a node; same visiting order as k_ground) collects the leaves it reaches
+// into a batch of up to HB_LEAVES.  A batch is served lane-parallel: per
+// leaf its bounds, whether its cube lies inside the hyperboloid and which
+// 32-point groups' boxes may meet it; then 64 of the batch's samples (walk
+// A: the lowest point of each group) or slots (walk B, and C, the exact
+// border pass for a longest empty run of 24 or 25 bins) per step, with the
+// early exit checked after every step.
+#define HB_LEAVES 256
+
+template <bool COUNT>
 __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const HeavyRec *heavy,
                                                      uint32_t nheavy)
 {
   __shared__ int stk[WB_STK];
+  __shared__ uint32_t lq[HB_LEAVES], lb[HB_LEAVES], le[HB_LEAVES], lmask[HB_LEAVES];
+  __shared__ uint32_t lpre[HB_LEAVES + 1];
   if (blockIdx.x >= nheavy) return;
+  const uint64_t t_start = COUNT ? wall_clock64() : 0;
+  uint64_t c_leaf = 0, c_pts = 0, c_exp = 0;
   const TreeView &T = A.T;
   const HeavyRec hr = heavy[blockIdx.x];
   const uint32_t i = hr.i;
-  const int lane = threadIdx.x;
+  const uint32_t lane = threadIdx.x;
   const double *ttan = A.tab, *tcos = A.tab + 511, *tsin = A.tab + 1023;
   int w = INT32_MIN;
   for (uint32_t k = 0; k < A.nmemb[i]; k++) w = max(w, A.memb[MAX_MEMB * i + k]);
@@ -1322,11 +1351,17 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const Heavy
       }
     }
     __syncthreads();
-    while (sp > 0) {
-      sp--;
-      const int code = rfl(stk[sp]);
-      __syncthreads();
-      if (code >= 0) {
+    bool done = false;
+    // batches grow 16, 64, 256 leaves: an early pass stops after few leaves
+    uint32_t lim = 16;
+    while (!done) {
+      // collect the leaves of the next part of the walk
+      uint32_t nq = 0;
+      while (sp > 0 && nq + 8 <= lim) {
+        sp--;
+        const int code = rfl(stk[sp]);
+        __syncthreads();
+        if (COUNT) c_exp++;
         const double *nc = T.node_cube + 4 * (size_t)code;
         const double cx = nc[0], cy = nc[1], cz = nc[2], side = nc[3];
         const int o = (x >= cx ? 1 : 0) | (y >= cy ? 2 : 0);
@@ -1340,77 +1375,114 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const Heavy
           h = wb::hyp_intersect(H, ncx, ncy, ncz, side / 2);
         }
         const int t = child_rank(k, o, zf);
-        uint32_t rb = h ? 1u << t : 0u;
-        rb |= __shfl_xor(rb, 1, 64);
-        rb |= __shfl_xor(rb, 2, 64);
-        rb |= __shfl_xor(rb, 4, 64);
-        rb = (uint32_t)rfl((int)rb);
-        if (h) stk[sp + __popc(rb >> (t + 1))] = ck;
-        sp += __popc(rb);
+        uint32_t rn = h && ck >= 0 ? 1u << t : 0u, rl = h && ck < -1 ? 1u << t : 0u;
+        for (int s = 1; s < 8; s <<= 1) {
+          rn |= __shfl_xor(rn, s, 64);
+          rl |= __shfl_xor(rl, s, 64);
+        }
+        rn = (uint32_t)rfl((int)rn);
+        rl = (uint32_t)rfl((int)rl);
+        // nodes onto the stack, rank 0 on top; leaves into the batch by rank
+        if (h && ck >= 0) stk[sp + __popc(rn >> (t + 1))] = ck;
+        if (h && ck < -1) lq[nq + __popc(rl & ((1u << t) - 1))] = (uint32_t)(-2 - ck);
+        sp += __popc(rn);
+        nq += __popc(rl);
         __syncthreads();
-        continue;
       }
-      const int leaf = -2 - code;
-      const uint32_t b = (uint32_t)rfl((int)T.leaf_start[leaf]);
-      const uint32_t e = (uint32_t)rfl((int)T.leaf_start[leaf + 1]);
-      if (phase == 0) {
-        const uint32_t g0 = b / WB_CHUNK, ng = (e - 1) / WB_CHUNK - g0 + 1;
-        for (uint32_t g = lane; g < ng; g += 64) {
-          const uint32_t c0 = g == 0 ? b : (g0 + g) * WB_CHUNK;
-          uint32_t j = T.chunk_low[g0 + g];
-          if (j < b || j >= e) j = c0;
-          const double qx = T.px[j], qy = T.py[j];
-          if (wb::hyp_in(H, qx, qy, T.pz[j]) && (qx != x || qy != y))
-            occ |= 1ull << wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
-        }
-      } else {
-        const double *lc = T.leaf_cube + 4 * (size_t)leaf;
-        const bool all = wb::hyp_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
-        // groups whose box may meet the hyperboloid, a group per lane
-        const uint32_t g0 = b / WB_CHUNK, ng = (e - 1) / WB_CHUNK - g0 + 1;
-        uint64_t gmask = ~0ull;
-        if (!all && ng <= WAVE) {
-          bool gh = false;
-          if ((uint32_t)lane < ng) {
-            const double *bx = T.chunk_box + 6 * (uint64_t)(g0 + lane);
-            gh = wb::hyp_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+      if (nq == 0) break;
+      lim = min(4 * lim, (uint32_t)HB_LEAVES);
+      if (COUNT) c_leaf += nq;
+      // per-leaf bounds, inside flag (bit 31 of lmask) and group mask, and
+      // the prefix of the leaves' sample or slot counts
+      uint32_t carry = 0;
+      for (uint32_t k0 = 0; k0 < nq; k0 += WAVE) {
+        const uint32_t k = k0 + lane;
+        uint32_t cnt = 0;
+        if (k < nq) {
+          const uint32_t leaf = lq[k];
+          const uint32_t b = T.leaf_start[leaf], e = T.leaf_start[leaf + 1];
+          const uint32_t g0 = b / WB_CHUNK, ng = (e - 1) / WB_CHUNK - g0 + 1;
+          uint32_t mask = 0x7fffffffu;
+          if (phase != 0) {
+            const double *lc = T.leaf_cube + 4 * (size_t)leaf;
+            if (wb::hyp_in_cube(H, lc[0], lc[1], lc[2], lc[3])) {
+              mask = 0xffffffffu;
+            } else {
+              // bits 0-30: groups whose box may meet the hyperboloid (groups
+              // past the 31st are never pruned)
+              mask = 0;
+              for (uint32_t g = 0; g < ng && g < 31; g++) {
+                const double *bx = T.chunk_box + 6 * (uint64_t)(g0 + g);
+                if (wb::hyp_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]))
+                  mask |= 1u << g;
+              }
+            }
           }
-          gmask = __ballot(gh);
+          lb[k] = b;
+          le[k] = e;
+          lmask[k] = mask;
+          cnt = phase == 0 ? ng : e - b;
         }
-        // 64 slots a step from the first group's start: groups ga and ga+1
-        for (uint32_t j0 = g0 * WB_CHUNK; j0 < e; j0 += WAVE) {
-          const uint32_t ga = j0 / WB_CHUNK - g0;
-          if (ga < 63 && ((gmask >> ga) & 3ull) == 0) continue;
-          const uint32_t j = j0 + lane;
-          const uint32_t g = ga + lane / WB_CHUNK;
-          if (j < b || j >= e || (g < 64 && !((gmask >> g) & 1))) continue;
-          const double qx = T.px[j], qy = T.py[j];
-          if ((all || wb::hyp_in(H, qx, qy, T.pz[j])) && (qx != x || qy != y)) {
-            const int bin = wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
-            if (phase == 1) {
-              occ |= 1ull << bin;
-            } else if (bin == before[0] || bin == afterb[0] || bin == before[1] ||
-                       bin == afterb[1]) {
-              const int pos =
-                  dir_bin_pos(wb::atan2i_ext(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact));
-              for (int k = 0; k < 2; k++) {
-                if (bin == before[k] && pos > hi[k]) hi[k] = pos;
-                if (bin == afterb[k] && pos < lo[k]) lo[k] = pos;
+        uint32_t incl = cnt;
+        for (int s = 1; s < WAVE; s <<= 1) {
+          const uint32_t v = (uint32_t)__shfl_up((int)incl, s, WAVE);
+          if ((int)lane >= s) incl += v;
+        }
+        if (k < nq) lpre[k + 1] = carry + incl;
+        carry += (uint32_t)__shfl((int)incl, WAVE - 1, WAVE);
+      }
+      if (lane == 0) lpre[0] = 0;
+      __syncthreads();
+      const uint32_t S = carry;
+      if (COUNT) c_pts += S;
+      uint32_t kc = 0;  // this lane's leaf in the batch
+      for (uint32_t f0 = 0; f0 < S && !done; f0 += WAVE) {
+        const uint32_t f = f0 + lane;
+        while (kc + 1 < nq && lpre[kc + 1] <= f) kc++;
+        if (f < S) {
+          const uint32_t b = lb[kc], e = le[kc], mask = lmask[kc], o = f - lpre[kc];
+          uint32_t j;
+          bool test = true;
+          if (phase == 0) {
+            const uint32_t g0 = b / WB_CHUNK;
+            j = T.chunk_low[g0 + o];
+            if (j < b || j >= e) j = o == 0 ? b : (g0 + o) * WB_CHUNK;
+          } else {
+            j = b + o;
+            const uint32_t g = j / WB_CHUNK - b / WB_CHUNK;
+            if ((mask >> 31) == 0 && g < 31 && !((mask >> g) & 1)) test = false;
+          }
+          if (test) {
+            const double qx = T.px[j], qy = T.py[j], qz = T.pz[j];
+            const bool in = (phase != 0 && (mask >> 31)) || wb::hyp_in(H, qx, qy, qz);
+            if (in && (qx != x || qy != y)) {
+              const int bin = wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
+              if (phase < 2) {
+                occ |= 1ull << bin;
+              } else if (bin == before[0] || bin == afterb[0] || bin == before[1] ||
+                         bin == afterb[1]) {
+                const int pos =
+                    dir_bin_pos(wb::atan2i_ext(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact));
+                for (int q = 0; q < 2; q++) {
+                  if (bin == before[q] && pos > hi[q]) hi[q] = pos;
+                  if (bin == afterb[q] && pos < lo[q]) lo[q] = pos;
+                }
               }
             }
           }
         }
-      }
-      if (phase < 2) {
-        const uint64_t occw = wave_or(occ);
-        if (occw != checked) {
-          checked = occw;
-          if (gap_verdict(occw) == 1) {
-            verdict = 1;
-            break;
+        if (phase < 2) {
+          const uint64_t occw = wave_or(occ);
+          if (occw != checked) {
+            checked = occw;
+            if (gap_verdict(occw) == 1) {
+              verdict = 1;
+              done = true;
+            }
           }
         }
       }
+      __syncthreads();
     }
   }
   if (verdict == 0) verdict = gap_verdict(wave_or(occ));
@@ -1432,6 +1504,16 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const Heavy
   if (lane == 0) {
     A.cls[i] = (uint8_t)verdict;
     if (nexact) atomicAdd(A.atan_exact, (unsigned long long)nexact);
+    if (COUNT) {
+      // per start walk: points, passes, leaves, slots, expansions, ticks
+      unsigned long long *h = A.counters + CT_N + 6 * hr.start;
+      atomicAdd(&h[0], 1ull);
+      if (verdict == 1) atomicAdd(&h[1], 1ull);
+      atomicAdd(&h[2], (unsigned long long)c_leaf);
+      atomicAdd(&h[3], (unsigned long long)c_pts);
+      atomicAdd(&h[4], (unsigned long long)c_exp);
+      atomicAdd(&h[5], (unsigned long long)(wall_clock64() - t_start));
+    }
   }
 }
 
@@ -1583,8 +1665,8 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   static const bool count_work = getenv("WB_CLASSIFY_COUNTERS") != nullptr;
   unsigned long long *dcount = nullptr;
   if (count_work) {
-    dcount = (unsigned long long *)wb_alloc(c, CT_N * 8);
-    WB_CHECK(hipMemsetAsync(dcount, 0, CT_N * 8, st));
+    dcount = (unsigned long long *)wb_alloc(c, (CT_N + 18) * 8);
+    WB_CHECK(hipMemsetAsync(dcount, 0, (CT_N + 18) * 8, st));
   }
   A.counters = dcount;
   HeavyRec *heavy = (HeavyRec *)wb_alloc(c, sizeof(HeavyRec) * (n + 1));
@@ -1603,8 +1685,12 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   WB_CHECK(hipMemcpyAsync(&h_nheavy, nslow + 1, 4, hipMemcpyDeviceToHost, st));
   WB_CHECK(hipStreamSynchronize(st));
   WB_CHECK(hipEventRecord(ev[6], st));
-  if (h_nheavy)
-    hipLaunchKernelGGL(k_ground_heavy, dim3(h_nheavy), dim3(64), 0, st, A, heavy, h_nheavy);
+  if (h_nheavy) {
+    if (count_work)
+      hipLaunchKernelGGL(k_ground_heavy<true>, dim3(h_nheavy), dim3(64), 0, st, A, heavy, h_nheavy);
+    else
+      hipLaunchKernelGGL(k_ground_heavy<false>, dim3(h_nheavy), dim3(64), 0, st, A, heavy, h_nheavy);
+  }
   WB_CHECK(hipEventRecord(ev[7], st));
   uint32_t h_nslow = 0;
   WB_CHECK(hipMemcpyAsync(&h_nslow, nslow, 4, hipMemcpyDeviceToHost, st));
@@ -1653,8 +1739,14 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   out->hist[0] = hc[2]; out->hist[1] = hc[3]; out->hist[2] = hc[4];
   out->nslow = h_nslow;
   if (dcount) {
-    unsigned long long h[CT_N];
+    unsigned long long h[CT_N + 18];
     WB_CHECK(hipMemcpy(h, dcount, sizeof h, hipMemcpyDeviceToHost));
+    for (int s = 0; s < 3; s++)
+      fprintf(stderr,
+              "[wb counters] heavy from walk %c: points %llu passed %llu leaves %llu slots %llu "
+              "expansions %llu ticks %llu\n",
+              "ABC"[s], h[CT_N + 6 * s], h[CT_N + 6 * s + 1], h[CT_N + 6 * s + 2],
+              h[CT_N + 6 * s + 3], h[CT_N + 6 * s + 4], h[CT_N + 6 * s + 5]);
     fprintf(stderr,
             "[wb counters] lanes A %llu B %llu open after B %llu heavy %llu | A: leaf-lanes %llu "
             "group-lanes %llu dirs %llu | B: leaf-lanes %llu point-lanes %llu dirs %llu | per "
@@ -1670,7 +1762,7 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
             h[CT_TSUM], h[CT_TMAX], rate, h[CT_THIST], h[CT_THIST + 1], h[CT_THIST + 2],
             h[CT_THIST + 3], h[CT_THIST + 4], h[CT_TSUMS], h[CT_TSUMS + 1], h[CT_TSUMS + 2],
             h[CT_TSUMS + 3], h[CT_TSUMS + 4]);
-    wb_free(c, dcount, CT_N * 8);
+    wb_free(c, dcount, (CT_N + 18) * 8);
   }
   out->nonempty = hc[5];
   out->ms_tiles = ms1; out->ms_scan = ms2; out->ms_classify = ms3;
