@@ -44,6 +44,14 @@ enum {
   WB_DEVICE_INPUT = 1,      /* x, y, z are device pointers already resident in HBM */
 };
 
+/* precision of the scan's plane fit (wb_classify_params.fit_precision) */
+enum {
+  WB_FIT_FP64 = 0,          /* the reference's FP64 fit, bit-identical (default) */
+  WB_FIT_FP32 = 1,          /* FP32 moments, naive sums, FP32 solve (study only) */
+  WB_FIT_FP32_COMP = 2,     /* FP32 moments with compensated sums, FP32 solve; tiles
+                               whose xy spread is near-collinear fall back to FP64 */
+};
+
 typedef struct wb_ctx wb_ctx;
 typedef struct wb_tree wb_tree;
 
@@ -66,6 +74,7 @@ typedef struct {
   double max_slope;         /* hyperboloid asymptotic slope (default 1) */
   double thickness;         /* vertex drop below the point (default 0) */
   double min_hs;            /* minimum hyperboloid size, m (default 0.1) */
+  int32_t fit_precision;    /* WB_FIT_*; 0 (FP64) reproduces the reference */
 } wb_classify_params;
 
 typedef struct {
@@ -73,10 +82,12 @@ typedef struct {
   int32_t nground;          /* points it classified as ground */
   int32_t tree_flags;       /* Tile::treeFlags */
   int32_t flowsnake_x, flowsnake_y;  /* Eisenstein tile address */
+  int32_t fit_fp64;         /* 1 if an FP32 fit fell back to FP64 for this tile */
   double density;           /* bottom-layer density (Tile::density) */
   double hs_scan;           /* hyperboloid size after scan */
   double hs;                /* hyperboloid size after postscan (used to classify) */
   double height;            /* Tile::height */
+  double tilt_x, tilt_y;    /* the fitted plane's slope after scan's clamp (scan.cpp:68-73) */
 } wb_tile;
 
 typedef struct {
@@ -96,6 +107,7 @@ typedef struct {
   double ms_classify_kernel;  /* the ground-test kernel alone */
   double ms_heavy_kernel;     /* the wave-per-point kernel for large candidate sets */
   double ms_total;            /* host wall time of wb_classify */
+  uint64_t fit_fallbacks;     /* tiles whose FP32 fit fell back to FP64 */
 } wb_stats;
 
 /* --- context ------------------------------------------------------------ */
@@ -106,6 +118,9 @@ void wb_ctx_destroy(wb_ctx *ctx);
 const char *wb_last_error(wb_ctx *ctx);
 /* synchronise the context's stream */
 int wb_ctx_sync(wb_ctx *ctx);
+/* free and total device memory of a device (admission control of a serving
+ * loop; no counterpart in the reference, whose store lives on disk) */
+int wb_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
 
 /* --- root cube: Octree::sizeFit (octree.cpp:268-310) -------------------- */
 int wb_size_fit(const double *corners, int ncorners, double center[3], double *side);
@@ -151,6 +166,14 @@ int wb_points_in(wb_tree *tree, int kind, const double *params, uint64_t nshapes
 int wb_classify(wb_tree *tree, const double bbmin[3], const double bbmax[3],
                 const wb_classify_params *params, uint8_t *class_out, int flags,
                 wb_tile *tiles_out, uint64_t tiles_cap, wb_stats *stats);
+
+/* --- SCAN and POSTSCAN alone (scanCylinder and postscanCylinder over every
+ *     tile, scan.cpp:31-179, in the phase order of wolkencanvas.cpp:469-535):
+ *     the tile records of wb_classify without the classification.  tiles_out
+ *     receives 7^k records in flowsnake counter order. */
+int wb_scan_tiles(wb_tree *tree, const double bbmin[3], const double bbmax[3],
+                  const wb_classify_params *params, wb_tile *tiles_out, uint64_t tiles_cap,
+                  wb_stats *stats);
 
 /* number of flowsnake tiles the classification uses for these corners */
 int wb_tile_count(const double bbmin[3], const double bbmax[3], double tile_size,

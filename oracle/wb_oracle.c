@@ -793,6 +793,7 @@ int wbo_base_seven(int ex, int ey)
 typedef struct {
   int npoints, nground, tree_flags;
   double density, hs_scan, hs, height;
+  double tilt_x, tilt_y;   /* the plane slope after the clamp, scan.cpp:68-73 */
 } wbo_tile;
 
 typedef struct {
@@ -1045,6 +1046,8 @@ long wbo_run(const int32_t *xi, const int32_t *yi, const int32_t *zi, long n,
       tl[k].hs_scan = sqrt(1 / density + sqrd(pr->min_hs));
       tl[k].hs = tl[k].hs_scan;
       tl[k].height = top - bottom;
+      tl[k].tilt_x = sx;
+      tl[k].tilt_y = sy;
       st->nonempty_tiles++;
       free(px);
     }

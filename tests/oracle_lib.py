@@ -32,7 +32,7 @@ class Params(C.Structure):
 class Tile(C.Structure):
     _fields_ = [("npoints", C.c_int), ("nground", C.c_int), ("tree_flags", C.c_int),
                 ("density", C.c_double), ("hs_scan", C.c_double), ("hs", C.c_double),
-                ("height", C.c_double)]
+                ("height", C.c_double), ("tilt_x", C.c_double), ("tilt_y", C.c_double)]
 
 
 class Stats(C.Structure):

@@ -120,8 +120,9 @@ def test_classify_matches_oracle(ctx, cfg):
     print("stats", st, "oracle", rst.n_class1, rst.n_class2, rst.n_unclassified)
     for f in ("npoints", "tree_flags"):
         assert np.array_equal(tiles[f], rtiles[f]), f
-    for f in ("density", "hs_scan", "hs", "height"):
+    for f in ("density", "hs_scan", "hs", "height", "tilt_x", "tilt_y"):
         assert np.array_equal(tiles[f], rtiles[f]), f
+    assert not tiles["fit_fp64"].any() and st["fit_fallbacks"] == 0
     assert int((cls != rcls).sum()) == 0
     if cfg["seed"] == 26:
         m = tiles["npoints"]
@@ -139,3 +140,74 @@ def test_context_close_frees_live_trees():
     c.close()
     assert tree.h is None
     tree.free()
+
+
+def normal_angle(ax, ay, bx, by):
+    """Angle between the plane normals (-ax, -ay, 1) and (-bx, -by, 1)."""
+    n1 = np.stack([-ax, -ay, np.ones_like(ax)], 1)
+    n2 = np.stack([-bx, -by, np.ones_like(bx)], 1)
+    cr = np.linalg.norm(np.cross(n1, n2), axis=1)
+    return np.arctan2(cr, (n1 * n2).sum(1))
+
+
+def test_scan_tiles_matches_classify_and_oracle(ctx):
+    cloud = synth.generate(seed=41, extent=24.0, density=25.0, embankments=1)
+    tree, _, _ = build_both(ctx, cloud)
+    tiles, st = W.scan_tiles(tree, cloud["bbmin"], cloud["bbmax"])
+    _, ctiles, _ = W.classify(tree, cloud["bbmin"], cloud["bbmax"])
+    _, rtiles, _ = O.run(cloud)
+    for f in ("npoints", "tree_flags", "density", "hs_scan", "hs", "height", "tilt_x", "tilt_y"):
+        assert np.array_equal(tiles[f], ctiles[f]), f
+        assert np.array_equal(tiles[f], rtiles[f]), f
+    assert st["nonempty_tiles"] == int((tiles["npoints"] > 0).sum())
+
+
+def line_cloud(seed):
+    """Points only along parallel lines 3 m apart (x every 2 cm, y within
+    2 mm of the line, z a gentle slope): every tile cylinder is near-collinear
+    in xy, the case the FP32 fit hands to FP64."""
+    rng = np.random.default_rng(seed)
+    xs = np.arange(0, 30000, 20)
+    ys = np.arange(0, 30000, 3000)
+    x = np.repeat(xs[None, :], len(ys), 0).ravel()
+    y = (ys[:, None] + rng.integers(0, 3, (len(ys), len(xs)))).ravel()
+    z = (x // 10 + y // 20 + rng.integers(0, 5, x.size)).astype(np.int64)
+    xi, yi, zi = x.astype(np.int32), y.astype(np.int32), z.astype(np.int32)
+    scale = np.array([0.001, 0.001, 0.001])
+    offset = np.array([500000.0, 4400000.0, 0.0])
+    lo = offset + scale * np.array([xi.min(), yi.min(), zi.min()])
+    hi = offset + scale * np.array([xi.max(), yi.max(), zi.max()])
+    return dict(xi=xi, yi=yi, zi=zi, scale=scale, offset=offset, bbmin=lo, bbmax=hi)
+
+
+# FP32 plane fits (an option for the precision study; FP64 is the reference).
+# Tolerance: on tiles fitted in FP32 the plane normal is within 1e-3 rad of
+# the FP64 normal with compensated sums; tiles that fall back are bit-equal.
+@pytest.mark.parametrize("kind", ["terrain", "lines"])
+def test_fp32_fits_against_fp64(ctx, kind):
+    if kind == "lines":
+        cloud = line_cloud(43)
+    else:
+        cloud = synth.generate(seed=42, extent=40.0, density=25.0, linear=0.05)
+    tree, _, _ = build_both(ctx, cloud)
+    ref, _ = W.scan_tiles(tree, cloud["bbmin"], cloud["bbmax"])
+    comp, st = W.scan_tiles(tree, cloud["bbmin"], cloud["bbmax"],
+                            W.default_params(fit_precision=W.WB_FIT_FP32_COMP))
+    naive, _ = W.scan_tiles(tree, cloud["bbmin"], cloud["bbmax"],
+                            W.default_params(fit_precision=W.WB_FIT_FP32))
+    live = ref["npoints"] > 0
+    assert np.array_equal(comp["npoints"], ref["npoints"])
+    fb = comp["fit_fp64"] == 1
+    assert st["fit_fallbacks"] == int(fb.sum())
+    for f in ("tilt_x", "tilt_y", "density", "height"):
+        assert np.array_equal(comp[f][fb], ref[f][fb]), f
+    ok = live & ~fb
+    ang = normal_angle(comp["tilt_x"][ok], comp["tilt_y"][ok], ref["tilt_x"][ok], ref["tilt_y"][ok])
+    print("fallbacks", int(fb.sum()), "of", int(live.sum()), "max angle comp",
+          ang.max() if ok.any() else None,
+          "naive", normal_angle(naive["tilt_x"][live], naive["tilt_y"][live],
+                                ref["tilt_x"][live], ref["tilt_y"][live]).max())
+    if ok.any():
+        assert ang.max() < 1e-3
+    if kind == "lines":
+        assert fb.sum() > 0.9 * live.sum()

@@ -21,6 +21,7 @@ GPU_SO = os.path.join(HERE, "libwb_gpu.so")
 WB_SHAPE_CYLINDER, WB_SHAPE_SPHERE, WB_SHAPE_HYPERBOLOID = 0, 1, 2
 WB_SHAPE_NPARAMS = 5
 WB_DEVICE_INPUT = 1
+WB_FIT_FP64, WB_FIT_FP32, WB_FIT_FP32_COMP = 0, 1, 2
 
 
 class WbError(RuntimeError):
@@ -38,14 +39,15 @@ class Leaf(C.Structure):
 
 class ClassifyParams(C.Structure):
     _fields_ = [("tile_size", C.c_double), ("max_slope", C.c_double),
-                ("thickness", C.c_double), ("min_hs", C.c_double)]
+                ("thickness", C.c_double), ("min_hs", C.c_double),
+                ("fit_precision", C.c_int32)]
 
 
 class TileRecord(C.Structure):
     _fields_ = [("npoints", C.c_int32), ("nground", C.c_int32), ("tree_flags", C.c_int32),
-                ("flowsnake_x", C.c_int32), ("flowsnake_y", C.c_int32),
+                ("flowsnake_x", C.c_int32), ("flowsnake_y", C.c_int32), ("fit_fp64", C.c_int32),
                 ("density", C.c_double), ("hs_scan", C.c_double), ("hs", C.c_double),
-                ("height", C.c_double)]
+                ("height", C.c_double), ("tilt_x", C.c_double), ("tilt_y", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -59,15 +61,16 @@ class Stats(C.Structure):
                 ("ms_build", C.c_double), ("ms_sort", C.c_double), ("ms_tiles", C.c_double),
                 ("ms_scan", C.c_double), ("ms_classify", C.c_double),
                 ("ms_classify_kernel", C.c_double), ("ms_heavy_kernel", C.c_double),
-                ("ms_total", C.c_double)]
+                ("ms_total", C.c_double), ("fit_fallbacks", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
-def default_params(tile_size=1.0, max_slope=1.0, thickness=0.0, min_hs=0.1):
-    """Reference defaults (mainwindow.cpp:403-410)."""
-    return ClassifyParams(tile_size, max_slope, thickness, min_hs)
+def default_params(tile_size=1.0, max_slope=1.0, thickness=0.0, min_hs=0.1,
+                   fit_precision=0):
+    """Reference defaults (mainwindow.cpp:403-410); fit_precision WB_FIT_FP64."""
+    return ClassifyParams(tile_size, max_slope, thickness, min_hs, fit_precision)
 
 
 _vp = C.c_void_p
@@ -79,6 +82,7 @@ _u32p = C.POINTER(C.c_uint32)
 SIGNATURES = {
     "wb_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "wb_ctx_destroy": (None, [_vp]),
+    "wb_device_memory": (C.c_int, [C.c_int, _u64p, _u64p]),
     "wb_last_error": (C.c_char_p, [_vp]),
     "wb_ctx_sync": (C.c_int, [_vp]),
     "wb_size_fit": (C.c_int, [_dp, C.c_int, _dp, _dp]),
@@ -93,6 +97,8 @@ SIGNATURES = {
     "wb_classify": (C.c_int, [_vp, _dp, _dp, C.POINTER(ClassifyParams), _vp, C.c_int,
                               C.POINTER(TileRecord), C.c_uint64, C.POINTER(Stats)]),
     "wb_tile_count": (C.c_int, [_dp, _dp, C.c_double, _u64p]),
+    "wb_scan_tiles": (C.c_int, [_vp, _dp, _dp, C.POINTER(ClassifyParams),
+                                C.POINTER(TileRecord), C.c_uint64, C.POINTER(Stats)]),
 }
 
 _lib = None
@@ -287,6 +293,14 @@ class Octree:
         return [idx[int(offs[s]):int(offs[s + 1])].copy() for s in range(ns)]
 
 
+def device_memory(device=0):
+    """(free, total) bytes of device memory."""
+    f, t = C.c_uint64(), C.c_uint64()
+    if lib().wb_device_memory(device, C.byref(f), C.byref(t)) != 0:
+        raise WbError("wb_device_memory failed")
+    return f.value, t.value
+
+
 def tile_count(bbmin, bbmax, tile_size=1.0):
     a, b = _arr(bbmin, np.float64), _arr(bbmax, np.float64)
     n = C.c_uint64()
@@ -316,3 +330,18 @@ def classify(tree, bbmin, bbmax, params=None, want_tiles=True, out=None):
     tree.ctx.check(rc, "wb_classify")
     tarr = np.ctypeslib.as_array(tiles)[:ntiles].copy() if want_tiles else None
     return cls, tarr, st.as_dict()
+
+
+def scan_tiles(tree, bbmin, bbmax, params=None):
+    """SCAN + POSTSCAN of every flowsnake tile (scanCylinder and
+    postscanCylinder, scan.cpp:31-179) without the classification.  Returns
+    (tile records, stats dict)."""
+    params = params or default_params()
+    a, b = _arr(bbmin, np.float64), _arr(bbmax, np.float64)
+    ntiles = tile_count(a, b, params.tile_size)
+    tiles = (TileRecord * max(ntiles, 1))()
+    st = Stats()
+    rc = lib().wb_scan_tiles(tree.h, _dptr(a), _dptr(b), C.byref(params), tiles, ntiles,
+                             C.byref(st))
+    tree.ctx.check(rc, "wb_scan_tiles")
+    return np.ctypeslib.as_array(tiles)[:ntiles].copy(), st.as_dict()

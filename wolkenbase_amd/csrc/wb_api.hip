@@ -8,9 +8,6 @@
 #include <algorithm>
 #include <chrono>
 
-void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
-                     const wb_classify_params *p, uint8_t *d_class_out, bool want_tiles,
-                     ClassifyOut *out);
 
 // ---------------------------------------------------------------------------
 // device memory cache: buffers are returned to a size-keyed pool and reused,
@@ -115,6 +112,18 @@ void wb_ctx_destroy(wb_ctx *c)
 }
 
 const char *wb_last_error(wb_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+int wb_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes)
+{
+  if (!free_bytes || !total_bytes) return WB_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return WB_EDEVICE;
+  size_t f = 0, t = 0;
+  if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&f, &t) != hipSuccess) return WB_EDEVICE;
+  *free_bytes = f;
+  *total_bytes = t;
+  return WB_OK;
+}
 
 int wb_ctx_sync(wb_ctx *c)
 {
@@ -323,14 +332,17 @@ int wb_tile_count(const double bbmin[3], const double bbmax[3], double tile_size
   return WB_OK;
 }
 
-int wb_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
-                const wb_classify_params *p, uint8_t *class_out, int flags, wb_tile *tiles_out,
-                uint64_t tiles_cap, wb_stats *stats)
+// wb_classify and wb_scan_tiles: the pass up to POSTSCAN, and CLASSIFY unless
+// scan_only
+static int run_pass(wb_tree *t, const double bbmin[3], const double bbmax[3],
+                    const wb_classify_params *p, uint8_t *class_out, int flags,
+                    wb_tile *tiles_out, uint64_t tiles_cap, wb_stats *stats, bool scan_only)
 {
-  if (!t || !bbmin || !bbmax || !p || !class_out) return WB_EINVAL;
   wb_ctx *c = t->ctx;
   if (!(p->tile_size > 0) || !(p->max_slope > 0) || !(p->min_hs >= 0) || !isfinite(p->thickness))
     return fail(c, "bad classification parameters", WB_EINVAL);
+  if (p->fit_precision < WB_FIT_FP64 || p->fit_precision > WB_FIT_FP32_COMP)
+    return fail(c, "unknown fit precision", WB_EINVAL);
   // the early-exit argument needs every direction >= 30 units below +180 deg
   for (int a = 0; a < 2; a++)
     if (!(bbmax[a] - bbmin[a] < 1e7 * t->scale[1 - a]))
@@ -343,10 +355,10 @@ int wb_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   WB_CHECK(hipSetDevice(c->device));
   auto t0 = std::chrono::steady_clock::now();
   uint8_t *dcls = class_out;
-  bool host_out = !(flags & WB_DEVICE_INPUT);
+  bool host_out = !(flags & WB_DEVICE_INPUT) && !scan_only;
   if (host_out) dcls = (uint8_t *)wb_alloc(c, t->n + 16);
   ClassifyOut co;
-  wb_run_classify(t, bbmin, bbmax, p, dcls, tiles_out != nullptr, &co);
+  wb_run_classify(t, bbmin, bbmax, p, dcls, tiles_out != nullptr, scan_only, &co);
   if (host_out) {
     if (t->n) WB_CHECK(hipMemcpyAsync(class_out, dcls, t->n, hipMemcpyDeviceToHost, c->stream));
     WB_CHECK(hipStreamSynchronize(c->stream));
@@ -362,10 +374,13 @@ int wb_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
       o.tree_flags = r.tree_flags;
       o.flowsnake_x = co.tex[k];
       o.flowsnake_y = co.tey[k];
+      o.fit_fp64 = r.fit_fp64;
       o.density = r.density;
       o.hs_scan = r.hs_scan;
       o.hs = r.hs;
       o.height = r.height;
+      o.tilt_x = r.tilt_x;
+      o.tilt_y = r.tilt_y;
     }
   }
   if (stats) {
@@ -391,9 +406,26 @@ int wb_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
     stats->ms_scan = co.ms_scan;
     stats->ms_classify = co.ms_classify;
     stats->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    stats->fit_fallbacks = co.fallbacks;
   }
   return WB_OK;
   API_CATCH(c)
+}
+
+int wb_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
+                const wb_classify_params *p, uint8_t *class_out, int flags, wb_tile *tiles_out,
+                uint64_t tiles_cap, wb_stats *stats)
+{
+  if (!t || !bbmin || !bbmax || !p || !class_out) return WB_EINVAL;
+  return run_pass(t, bbmin, bbmax, p, class_out, flags, tiles_out, tiles_cap, stats, false);
+}
+
+int wb_scan_tiles(wb_tree *t, const double bbmin[3], const double bbmax[3],
+                  const wb_classify_params *p, wb_tile *tiles_out, uint64_t tiles_cap,
+                  wb_stats *stats)
+{
+  if (!t || !bbmin || !bbmax || !p || !tiles_out) return WB_EINVAL;
+  return run_pass(t, bbmin, bbmax, p, nullptr, 0, tiles_out, tiles_cap, stats, true);
 }
 
 }  // extern "C"

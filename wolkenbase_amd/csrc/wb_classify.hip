@@ -41,6 +41,17 @@
 constexpr double kSqrt34 = 0.86602540378443864676372317;  // eisenstein.h:33
 constexpr double kSqrt7 = 2.6457513110645905905016;       // scan.h:23
 
+// lane values: the first active lane's, and lane l's (l wave-uniform)
+__device__ inline int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// lane l's value of v (l wave-uniform)
+__device__ inline double rdl(double v, uint32_t l)
+{
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, (int)l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), (int)l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 struct TreeView {
   const double *px, *py, *pz;
   const uint32_t *leaf_start;
@@ -580,6 +591,82 @@ __device__ inline double wave_pairwise_reg(uint32_t m, const double (&v)[K])
   return acc;
 }
 
+// Sum over the wave of K float slots per lane: plain (naive), or with each
+// lane's slots added by Neumaier's compensated sum and the lanes' (sum,
+// error) pairs merged by Knuth's TwoSum, so the rounding errors of every
+// addition are carried and added once at the end.
+template <int K>
+__device__ inline float wave_fsum(const float (&v)[K], bool comp)
+{
+  float s = 0, e = 0;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const float t = s + v[k];
+    if (comp) e += wb::dabs(s) >= wb::dabs(v[k]) ? (s - t) + v[k] : (v[k] - t) + s;
+    s = t;
+  }
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const float s2 = __shfl_xor(s, o, WAVE), e2 = __shfl_xor(e, o, WAVE);
+    const float t = s + s2;
+    if (comp) {
+      const float bp = t - s;
+      e = e + e2 + ((s - (t - bp)) + (s2 - bp));
+    }
+    s = t;
+  }
+  return s + e;
+}
+
+// The plane tilt from FP32 moments of (x, y, z - z0) (a shift of z leaves the
+// tilt unchanged), solved through the centred 2x2 normal equations in FP32.
+// Returns false when the xy spread is too close to collinear for FP32
+// (det < 1e-3 trace^2 of the centred xy moments, or fewer than 3 points); the
+// caller then fits in FP64.
+template <int K>
+__device__ inline bool plane_tilt_f32(uint32_t m, const double (&vx)[K], const double (&vy)[K],
+                                      const double (&vz)[K], const bool (&mine)[K], bool comp,
+                                      double *sx, double *sy)
+{
+  const double z0 = rdl(vz[0], 0);
+  float fx[K], fy[K], fz[K], t[K];
+#pragma unroll
+  for (int s = 0; s < K; s++) {
+    fx[s] = mine[s] ? (float)vx[s] : 0.0f;
+    fy[s] = mine[s] ? (float)vy[s] : 0.0f;
+    fz[s] = mine[s] ? (float)(vz[s] - z0) : 0.0f;
+  }
+  const float Sx = wave_fsum<K>(fx, comp), Sy = wave_fsum<K>(fy, comp);
+  const float Sz = wave_fsum<K>(fz, comp);
+#pragma unroll
+  for (int s = 0; s < K; s++) t[s] = fx[s] * fx[s];
+  const float Sxx = wave_fsum<K>(t, comp);
+#pragma unroll
+  for (int s = 0; s < K; s++) t[s] = fx[s] * fy[s];
+  const float Sxy = wave_fsum<K>(t, comp);
+#pragma unroll
+  for (int s = 0; s < K; s++) t[s] = fy[s] * fy[s];
+  const float Syy = wave_fsum<K>(t, comp);
+#pragma unroll
+  for (int s = 0; s < K; s++) t[s] = fx[s] * fz[s];
+  const float Sxz = wave_fsum<K>(t, comp);
+#pragma unroll
+  for (int s = 0; s < K; s++) t[s] = fy[s] * fz[s];
+  const float Syz = wave_fsum<K>(t, comp);
+  const float n = (float)m;
+  const float mx = Sx / n, my = Sy / n, mz = Sz / n;
+  const float cxx = Sxx - Sx * mx, cyy = Syy - Sy * my, cxy = Sxy - Sx * my;
+  const float cxz = Sxz - Sx * mz, cyz = Syz - Sy * mz;
+  const float det = cxx * cyy - cxy * cxy, tr = cxx + cyy;
+  const bool well = m >= 3 && det > 1e-3f * tr * tr;
+  double p = (double)((cxz * cyy - cyz * cxy) / det), q = (double)((cyz * cxx - cxz * cxy) / det);
+  const double len = wb::hypot(p, q);
+  if (len > 1) { p /= len; q /= len; }
+  if (wb::disnan(p) || wb::disnan(q)) { p = 0; q = 0; }
+  *sx = p;
+  *sy = q;
+  return well;
+}
+
 // The scan of a tile of at most 64*K points, K per lane (point e in slot
 // e/64 of lane e%64): the same operations and summation order as the
 // general path of k_scan, with the canonical sort done by a bitonic network
@@ -587,8 +674,8 @@ __device__ inline double wave_pairwise_reg(uint32_t m, const double (&v)[K])
 template <int K>
 __device__ __attribute__((always_inline)) inline void scan_reg(const TreeView &T, const TileGeo &g,
                                                                 int ex, int ey, uint32_t off,
-                         uint32_t m, const uint32_t *lists, double min_hs, TileRec *out,
-                         unsigned long long *ctr)
+                         uint32_t m, const uint32_t *lists, double min_hs, int prec,
+                         TileRec *out, unsigned long long *ctr)
 {
   const uint32_t lane = __lane_id();
   uint32_t c[K];
@@ -638,21 +725,30 @@ __device__ __attribute__((always_inline)) inline void scan_reg(const TreeView &T
       vz[s] = T.pz[q] - 0;
     }
   }
-  double t[K];
+  double sx = 0, sy = 0;
+  // FP32 fits are an option for the precision study; FP64 is the reference
+  bool fit64 = prec == WB_FIT_FP64;
+  int fell_back = 0;
+  if (!fit64) {
+    const bool well = plane_tilt_f32<K>(m, vx, vy, vz, mine, prec == WB_FIT_FP32_COMP, &sx, &sy);
+    if (!well && prec == WB_FIT_FP32_COMP) fit64 = true, fell_back = 1;
+  }
+  if (fit64) {
+    double t[K];
 #define WB_SUM(expr) (([&]() { _Pragma("unroll") for (int s = 0; s < K; s++) t[s] = (expr); }()), \
                       wave_pairwise_reg<K>(m, t))
-  const double sxx = WB_SUM(vx[s] * vx[s]);
-  const double syx = WB_SUM(vy[s] * vx[s]);
-  const double syy = WB_SUM(vy[s] * vy[s]);
-  const double s1x = WB_SUM(1.0 * vx[s]);
-  const double s1y = WB_SUM(1.0 * vy[s]);
-  const double s11 = WB_SUM(1.0 * 1.0);
-  const double sxz = WB_SUM(vx[s] * vz[s]);
-  const double syz = WB_SUM(vy[s] * vz[s]);
-  const double s1z = WB_SUM(1.0 * vz[s]);
+    const double sxx = WB_SUM(vx[s] * vx[s]);
+    const double syx = WB_SUM(vy[s] * vx[s]);
+    const double syy = WB_SUM(vy[s] * vy[s]);
+    const double s1x = WB_SUM(1.0 * vx[s]);
+    const double s1y = WB_SUM(1.0 * vy[s]);
+    const double s11 = WB_SUM(1.0 * 1.0);
+    const double sxz = WB_SUM(vx[s] * vz[s]);
+    const double syz = WB_SUM(vy[s] * vz[s]);
+    const double s1z = WB_SUM(1.0 * vz[s]);
 #undef WB_SUM
-  double sx, sy;
-  plane_tilt(sxx, syx, syy, s1x, s1y, s11, sxz, syz, s1z, &sx, &sy);
+    plane_tilt(sxx, syx, syy, s1x, s1y, s11, sxz, syz, s1z, &sx, &sy);
+  }
   double u[K];
   double bottom = __builtin_inf(), top = -__builtin_inf();
 #pragma unroll
@@ -712,12 +808,15 @@ __device__ __attribute__((always_inline)) inline void scan_reg(const TreeView &T
     rr.npoints = (int32_t)m;
     rr.nground = 0;
     rr.tree_flags = tree;
-    rr.pad = 0;
+    rr.fit_fp64 = fell_back;
     rr.density = density;
     rr.hs_scan = sqrt(1 / density + min_hs * min_hs);
     rr.hs = rr.hs_scan;
     rr.height = top - bottom;
+    rr.tilt_x = sx;
+    rr.tilt_y = sy;
     *out = rr;
+    if (fell_back) atomicAdd(&ctr[7], 1ull);
   }
 }
 
@@ -731,28 +830,28 @@ template <int K>
 __global__ void __launch_bounds__(64, WB_SCAN_MINW) k_scan_reg(TreeView T, TileGeo g, const int32_t *tex,
                                                  const int32_t *tey, const uint32_t *tile_off,
                                                  const uint32_t *tile_cnt, const uint32_t *lists,
-                                                 double min_hs, TileRec *rec,
+                                                 double min_hs, int prec, TileRec *rec,
                                                  unsigned long long *ctr)
 {
   for (uint32_t t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
     const uint32_t m = tile_cnt[t];
     if (K == 1 && m == 0) {
       if (threadIdx.x == 0) {
-        TileRec r = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0};
+        TileRec r = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         rec[t] = r;
       }
       continue;
     }
     const uint32_t below = K == 1 ? 0u : (uint32_t)(WAVE * K / 2);  // the next smaller class
     if (m <= below || m > (uint32_t)(WAVE * K)) continue;
-    scan_reg<K>(T, g, tex[t], tey[t], tile_off[t], m, lists, min_hs, rec + t, ctr);
+    scan_reg<K>(T, g, tex[t], tey[t], tile_off[t], m, lists, min_hs, prec, rec + t, ctr);
   }
 }
 
 // the scan of one tile of up to SCAN_CAP points by one wave (LDS sort)
 __device__ void scan_big(const TreeView &T, const TileGeo &g, int ex, int ey, uint32_t off,
-                         uint32_t m, const uint32_t *lists, double min_hs, TileRec *out,
-                         unsigned long long *ctr, uint32_t *cp, int *histo)
+                         uint32_t m, const uint32_t *lists, double min_hs, int prec,
+                         TileRec *out, unsigned long long *ctr, uint32_t *cp, int *histo)
 {
   const int lane = threadIdx.x;
   uint32_t P = 1;
@@ -843,12 +942,15 @@ __device__ void scan_big(const TreeView &T, const TileGeo &g, int ex, int ey, ui
     rr.npoints = (int32_t)m;
     rr.nground = 0;
     rr.tree_flags = tree;
-    rr.pad = 0;
+    rr.fit_fp64 = prec != WB_FIT_FP64;  // tiles of this size always fit in FP64
     rr.density = density;
     rr.hs_scan = sqrt(1 / density + min_hs * min_hs);
     rr.hs = rr.hs_scan;
     rr.height = top - bottom;
+    rr.tilt_x = sx;
+    rr.tilt_y = sy;
     *out = rr;
+    if (prec != WB_FIT_FP64) atomicAdd(&ctr[7], 1ull);
   }
   __syncthreads();
 }
@@ -858,7 +960,7 @@ __device__ void scan_big(const TreeView &T, const TileGeo &g, int ex, int ey, ui
 __global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_t *tex,
                                              const int32_t *tey, const uint32_t *tile_off,
                                              const uint32_t *tile_cnt, const uint32_t *lists,
-                                             double min_hs, TileRec *rec, uint32_t *err,
+                                             double min_hs, int prec, TileRec *rec, uint32_t *err,
                                              unsigned long long *ctr)
 {
   __shared__ uint32_t cp[SCAN_CAP];
@@ -870,7 +972,7 @@ __global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_
       if (threadIdx.x == 0) atomicOr(err, 2u);
       continue;
     }
-    scan_big(T, g, tex[t], tey[t], tile_off[t], m, lists, min_hs, rec + t, ctr, cp, histo);
+    scan_big(T, g, tex[t], tey[t], tile_off[t], m, lists, min_hs, prec, rec + t, ctr, cp, histo);
   }
 }
 
@@ -977,20 +1079,6 @@ __device__ inline uint64_t wave_or(uint64_t v)
   return ((uint64_t)wave_or32((uint32_t)(v >> 32)) << 32) | wave_or32((uint32_t)v);
 }
 
-__device__ inline int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
-// lane l's value of v (l wave-uniform)
-__device__ inline double rdl(double v, uint32_t l)
-{
-  const uint64_t u = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, (int)l);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), (int)l);
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-__device__ inline uint64_t rfl64(uint64_t v)
-{
-  return ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
-         (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-}
 
 // ---------------------------------------------------------------------------
 // Ground-test walk: findBlocks (octree.cpp:234-251) for 64 hyperboloids at
@@ -1583,7 +1671,7 @@ void wb_run_points(wb_tree *t, int kind, const double *dparams, uint64_t ns, con
 
 void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
                      const wb_classify_params *p, uint8_t *d_class_out, bool want_tiles,
-                     ClassifyOut *out)
+                     bool scan_only, ClassifyOut *out)
 {
   wb_ctx *c = t->ctx;
   hipStream_t st = c->stream;
@@ -1602,10 +1690,13 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   TileRec *rec = (TileRec *)wb_alloc(c, sizeof(TileRec) * (size_t)g.ntiles);
   int32_t *memb = (int32_t *)wb_alloc(c, 4ull * MAX_MEMB * n + 16);
   uint32_t *nmemb = (uint32_t *)wb_alloc(c, 4ull * n + 4);
-  uint32_t *err = (uint32_t *)wb_alloc(c, 64);
-  unsigned long long *ctr = (unsigned long long *)err + 1;  // borderline, atan_exact, hist[3]
+  // err[0] error flags; then u64 counters: 0 sector borderline, 1 exact atan2i
+  // steps, 2-4 class histogram, 5 non-empty tiles, 6 (two u32) exact gap
+  // points and heavy points, 7 FP64 fit fallbacks
+  uint32_t *err = (uint32_t *)wb_alloc(c, 128);
+  unsigned long long *ctr = (unsigned long long *)err + 1;
   uint32_t *nslow = (uint32_t *)(ctr + 6);
-  WB_CHECK(hipMemsetAsync(err, 0, 64, st));
+  WB_CHECK(hipMemsetAsync(err, 0, 128, st));
   WB_CHECK(hipMemsetAsync(tile_cnt, 0, 4ull * g.ntiles, st));
   WB_CHECK(hipMemsetAsync(tile_fill, 0, 4ull * g.ntiles, st));
   TreeView T = view(t);
@@ -1640,84 +1731,95 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   {
     // one wave per tile of each size class; grid-stride over the tiles
     const unsigned nb = (unsigned)std::min<uint64_t>(g.ntiles, 1u << 16);
+    const int prec = p->fit_precision;
     if (nb) {
       hipLaunchKernelGGL(k_scan_reg<1>, dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,
-                         tile_cnt, lists, p->min_hs, rec, ctr);
+                         tile_cnt, lists, p->min_hs, prec, rec, ctr);
       hipLaunchKernelGGL(k_scan_reg<2>, dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,
-                         tile_cnt, lists, p->min_hs, rec, ctr);
+                         tile_cnt, lists, p->min_hs, prec, rec, ctr);
       hipLaunchKernelGGL(k_scan_reg<4>, dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,
-                         tile_cnt, lists, p->min_hs, rec, ctr);
+                         tile_cnt, lists, p->min_hs, prec, rec, ctr);
       hipLaunchKernelGGL(k_scan, dim3(std::min(nb, 4096u)), dim3(64), 0, st, T, g, tex, tey,
-                         tile_off, tile_cnt, lists, p->min_hs, rec, err, ctr);
+                         tile_off, tile_cnt, lists, p->min_hs, prec, rec, err, ctr);
     }
   }
   hipLaunchKernelGGL(k_postscan, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey, rec,
                      ctr + 5);
   WB_CHECK(hipEventRecord(ev[2], st));
-  // classify
-  uint8_t *cls = (uint8_t *)wb_alloc(c, n + 16);
-  ClassifyArgs A;
-  A.T = T; A.g = g; A.memb = memb; A.nmemb = nmemb; A.rec = rec;
-  A.thickness = p->thickness; A.max_slope = p->max_slope;
-  A.tab = c->d_tan; A.K = c->kconst; A.cls = cls; A.nslow = nslow;
-  A.atan_exact = ctr + 1;
-  WB_CHECK(hipEventRecord(ev[4], st));
-  static const bool count_work = getenv("WB_CLASSIFY_COUNTERS") != nullptr;
+  uint8_t *cls = nullptr;
+  HeavyRec *heavy = nullptr;
   unsigned long long *dcount = nullptr;
-  if (count_work) {
-    dcount = (unsigned long long *)wb_alloc(c, (CT_N + 18) * 8);
-    WB_CHECK(hipMemsetAsync(dcount, 0, (CT_N + 18) * 8, st));
-  }
-  A.counters = dcount;
-  HeavyRec *heavy = (HeavyRec *)wb_alloc(c, sizeof(HeavyRec) * (n + 1));
-  A.heavy = heavy;
-  A.nheavy = nslow + 1;
-  if (n > 0) {
-    if (count_work)
-      hipLaunchKernelGGL((k_ground<true, WB_GROUND_MINW>), dim3(wb_blocks(n, 256)), dim3(256), 0,
-                         st, A);
-    else
-      hipLaunchKernelGGL((k_ground<false, WB_GROUND_MINW>), dim3(wb_blocks(n, 256)), dim3(256), 0,
-                         st, A);
-  }
+  uint32_t h_nheavy = 0, h_nslow = 0;
+  int32_t *nground = (int32_t *)tile_fill;  // reused once the lists are built
+  WB_CHECK(hipEventRecord(ev[4], st));
   WB_CHECK(hipEventRecord(ev[5], st));
-  uint32_t h_nheavy = 0;
-  WB_CHECK(hipMemcpyAsync(&h_nheavy, nslow + 1, 4, hipMemcpyDeviceToHost, st));
-  WB_CHECK(hipStreamSynchronize(st));
   WB_CHECK(hipEventRecord(ev[6], st));
-  if (h_nheavy) {
-    if (count_work)
-      hipLaunchKernelGGL(k_ground_heavy<true>, dim3(h_nheavy), dim3(64), 0, st, A, heavy, h_nheavy);
-    else
-      hipLaunchKernelGGL(k_ground_heavy<false>, dim3(h_nheavy), dim3(64), 0, st, A, heavy, h_nheavy);
-  }
   WB_CHECK(hipEventRecord(ev[7], st));
-  uint32_t h_nslow = 0;
-  WB_CHECK(hipMemcpyAsync(&h_nslow, nslow, 4, hipMemcpyDeviceToHost, st));
-  int32_t *nground = (int32_t *)tile_fill;  // reuse
-  WB_CHECK(hipMemsetAsync(nground, 0, 4ull * g.ntiles, st));
-  if (n > 0) {
-    const unsigned nbk = wb_blocks(n, 256);
-    uint32_t *partial = (uint32_t *)wb_alloc(c, 12ull * nbk);
-    hipLaunchKernelGGL(k_count_ground, dim3(nbk), dim3(256), 0, st, n, cls, memb, nmemb, g.lo,
-                       nground, partial, want_tiles ? 1 : 0);
-    hipLaunchKernelGGL(k_sum_hist, dim3(1), dim3(256), 0, st, partial, nbk, ctr + 2);
-    wb_free(c, partial, 12ull * nbk);
-    hipLaunchKernelGGL(k_scatter_class, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, cls, t->sidx,
-                       d_class_out);
+  if (!scan_only) {
+    // classify
+    cls = (uint8_t *)wb_alloc(c, n + 16);
+    ClassifyArgs A;
+    A.T = T; A.g = g; A.memb = memb; A.nmemb = nmemb; A.rec = rec;
+    A.thickness = p->thickness; A.max_slope = p->max_slope;
+    A.tab = c->d_tan; A.K = c->kconst; A.cls = cls; A.nslow = nslow;
+    A.atan_exact = ctr + 1;
+    WB_CHECK(hipEventRecord(ev[4], st));
+    static const bool count_work = getenv("WB_CLASSIFY_COUNTERS") != nullptr;
+    if (count_work) {
+      dcount = (unsigned long long *)wb_alloc(c, (CT_N + 18) * 8);
+      WB_CHECK(hipMemsetAsync(dcount, 0, (CT_N + 18) * 8, st));
+    }
+    A.counters = dcount;
+    heavy = (HeavyRec *)wb_alloc(c, sizeof(HeavyRec) * (n + 1));
+    A.heavy = heavy;
+    A.nheavy = nslow + 1;
+    if (n > 0) {
+      if (count_work)
+        hipLaunchKernelGGL((k_ground<true, WB_GROUND_MINW>), dim3(wb_blocks(n, 256)), dim3(256),
+                           0, st, A);
+      else
+        hipLaunchKernelGGL((k_ground<false, WB_GROUND_MINW>), dim3(wb_blocks(n, 256)), dim3(256),
+                           0, st, A);
+    }
+    WB_CHECK(hipEventRecord(ev[5], st));
+    WB_CHECK(hipMemcpyAsync(&h_nheavy, nslow + 1, 4, hipMemcpyDeviceToHost, st));
+    WB_CHECK(hipStreamSynchronize(st));
+    WB_CHECK(hipEventRecord(ev[6], st));
+    if (h_nheavy) {
+      if (count_work)
+        hipLaunchKernelGGL(k_ground_heavy<true>, dim3(h_nheavy), dim3(64), 0, st, A, heavy,
+                           h_nheavy);
+      else
+        hipLaunchKernelGGL(k_ground_heavy<false>, dim3(h_nheavy), dim3(64), 0, st, A, heavy,
+                           h_nheavy);
+    }
+    WB_CHECK(hipEventRecord(ev[7], st));
+    WB_CHECK(hipMemcpyAsync(&h_nslow, nslow, 4, hipMemcpyDeviceToHost, st));
+    WB_CHECK(hipMemsetAsync(nground, 0, 4ull * g.ntiles, st));
+    if (n > 0) {
+      const unsigned nbk = wb_blocks(n, 256);
+      uint32_t *partial = (uint32_t *)wb_alloc(c, 12ull * nbk);
+      hipLaunchKernelGGL(k_count_ground, dim3(nbk), dim3(256), 0, st, n, cls, memb, nmemb, g.lo,
+                         nground, partial, want_tiles ? 1 : 0);
+      hipLaunchKernelGGL(k_sum_hist, dim3(1), dim3(256), 0, st, partial, nbk, ctr + 2);
+      wb_free(c, partial, 12ull * nbk);
+      hipLaunchKernelGGL(k_scatter_class, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, cls,
+                         t->sidx, d_class_out);
+    }
   }
   WB_CHECK(hipEventRecord(ev[3], st));
-  uint32_t h_err[16];
-  WB_CHECK(hipMemcpyAsync(h_err, err, 64, hipMemcpyDeviceToHost, st));
+  uint32_t h_err[32];
+  WB_CHECK(hipMemcpyAsync(h_err, err, 128, hipMemcpyDeviceToHost, st));
   if (want_tiles) {
     out->tiles.resize(g.ntiles);
     out->tex.resize(g.ntiles);
     out->tey.resize(g.ntiles);
-    std::vector<int32_t> ng(g.ntiles);
+    std::vector<int32_t> ng(g.ntiles, 0);
     WB_CHECK(hipMemcpyAsync(out->tiles.data(), rec, sizeof(TileRec) * (size_t)g.ntiles, hipMemcpyDeviceToHost, st));
     WB_CHECK(hipMemcpyAsync(out->tex.data(), tex, 4ull * g.ntiles, hipMemcpyDeviceToHost, st));
     WB_CHECK(hipMemcpyAsync(out->tey.data(), tey, 4ull * g.ntiles, hipMemcpyDeviceToHost, st));
-    WB_CHECK(hipMemcpyAsync(ng.data(), nground, 4ull * g.ntiles, hipMemcpyDeviceToHost, st));
+    if (!scan_only)
+      WB_CHECK(hipMemcpyAsync(ng.data(), nground, 4ull * g.ntiles, hipMemcpyDeviceToHost, st));
     WB_CHECK(hipStreamSynchronize(st));
     for (uint32_t k = 0; k < g.ntiles; k++) out->tiles[k].nground = ng[k];
   }
@@ -1737,6 +1839,8 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   out->borderline = hc[0];
   out->atan_exact = hc[1];
   out->hist[0] = hc[2]; out->hist[1] = hc[3]; out->hist[2] = hc[4];
+  out->nonempty = hc[5];
+  out->fallbacks = hc[7];
   out->nslow = h_nslow;
   if (dcount) {
     unsigned long long h[CT_N + 18];
@@ -1764,7 +1868,6 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
             h[CT_TSUMS + 3], h[CT_TSUMS + 4]);
     wb_free(c, dcount, (CT_N + 18) * 8);
   }
-  out->nonempty = hc[5];
   out->ms_tiles = ms1; out->ms_scan = ms2; out->ms_classify = ms3;
   wb_free(c, grid, 4 * gcells);
   wb_free(c, tex, 4ull * g.ntiles);
@@ -1776,10 +1879,10 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
   wb_free(c, nmemb, 4ull * n + 4);
   wb_free(c, tile_off, 4ull * g.ntiles + 4);
   wb_free(c, lists, 4 * total + 4);
-  wb_free(c, cls, n + 16);
-  wb_free(c, heavy, sizeof(HeavyRec) * (n + 1));
+  if (cls) wb_free(c, cls, n + 16);
+  if (heavy) wb_free(c, heavy, sizeof(HeavyRec) * (n + 1));
   out->nheavy = h_nheavy;
-  if (h_err[0] & 1u) { wb_free(c, err, 64); throw std::string("a point lies in more than 4 tile cylinders"); }
-  if (h_err[0] & 2u) { wb_free(c, err, 64); throw std::string("a tile cylinder holds more than 1024 points"); }
-  wb_free(c, err, 64);
+  if (h_err[0] & 1u) { wb_free(c, err, 128); throw std::string("a point lies in more than 4 tile cylinders"); }
+  if (h_err[0] & 2u) { wb_free(c, err, 128); throw std::string("a tile cylinder holds more than 1024 points"); }
+  wb_free(c, err, 128);
 }

@@ -93,14 +93,14 @@ T *wb_talloc(wb_tree *t, size_t count)
 
 // per-tile record of the classification pass (device and host)
 struct TileRec {
-  int32_t npoints, nground, tree_flags, pad;
-  double density, hs_scan, hs, height;
+  int32_t npoints, nground, tree_flags, fit_fp64;
+  double density, hs_scan, hs, height, tilt_x, tilt_y;
 };
 
 struct ClassifyOut {
   std::vector<TileRec> tiles;
   std::vector<int32_t> tex, tey;
-  uint64_t borderline, atan_exact, nslow, hist[3], nonempty, nheavy;
+  uint64_t borderline, atan_exact, nslow, hist[3], nonempty, nheavy, fallbacks;
   double ms_tiles, ms_scan, ms_classify, ms_classify_kernel, ms_heavy_kernel;
 };
 
@@ -108,6 +108,11 @@ struct ClassifyOut {
 void wb_run_count(wb_tree *t, int kind, const double *dparams, uint64_t ns, uint64_t *dcounts);
 void wb_run_points(wb_tree *t, int kind, const double *dparams, uint64_t ns, const uint64_t *doffs,
                    uint32_t *dout);
+// the classification pass (tile geometry, membership, SCAN, POSTSCAN and,
+// unless scan_only, CLASSIFY); class bytes to d_class_out in input order
+void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
+                     const wb_classify_params *p, uint8_t *d_class_out, bool want_tiles,
+                     bool scan_only, ClassifyOut *out);
 
 // ---- building blocks implemented in wb_build.hip ----
 // exclusive scan of u32 in place, returns the total (synchronous read-back)
