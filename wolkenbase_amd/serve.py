@@ -49,12 +49,16 @@ class Request:
 
 
 class Server:
-    def __init__(self, slots=4, device=0, mem_fraction=0.8, params=None):
+    """policy "fifo": requests start in arrival order; "sjf": the waiting
+    request with the fewest points starts first."""
+
+    def __init__(self, slots=4, device=0, mem_fraction=0.8, params=None, policy="fifo"):
         _, total = W.device_memory(device)
         self.budget = int(total * mem_fraction)
         self.used = 0
         self.device = device
         self.params = params or W.default_params()
+        self.policy = policy
         self.q = collections.deque()
         self.cv = threading.Condition()
         self.stopping = False
@@ -77,14 +81,17 @@ class Server:
             w.join()
 
     def _next(self):
-        # first come, first served; the head waits until its memory fits
-        # (a request larger than the budget runs alone)
+        # the chosen request waits until its memory fits (a request larger
+        # than the budget runs alone)
         with self.cv:
             while True:
                 if self.q:
-                    need = self.q[0].need
+                    k = 0 if self.policy == "fifo" else min(range(len(self.q)),
+                                                            key=lambda j: self.q[j].n)
+                    need = self.q[k].need
                     if self.used == 0 or self.used + need <= self.budget:
-                        req = self.q.popleft()
+                        req = self.q[k]
+                        del self.q[k]
                         self.used += need
                         return req
                 elif self.stopping:
@@ -138,7 +145,7 @@ def make_pool(ntiles, seed, max_points):
     return pool
 
 
-def load_test(pool, slots, load, nreq, seed, device=0):
+def load_test(pool, slots, load, nreq, seed, device=0, policy="fifo"):
     ctx = W.Context(device)
     params = W.default_params()
     # each tile alone: warm-up, then timed
@@ -151,7 +158,7 @@ def load_test(pool, slots, load, nreq, seed, device=0):
     ctx.close()
     npts = np.array([len(t["xi"]) for t in pool], dtype=np.float64)
     single = npts.sum() / sum(iso_t)
-    srv = Server(slots=slots, device=device, params=params)
+    srv = Server(slots=slots, device=device, params=params, policy=policy)
     try:
         # capacity: every pool tile submitted at once
         t0 = time.perf_counter()
@@ -188,7 +195,8 @@ def load_test(pool, slots, load, nreq, seed, device=0):
     mismatched = sum(int(not np.array_equal(r.cls, iso_cls[r.tag])) for r in reqs)
     return {
         "metric": "serving: classified points/s and p50/p99 tile latency",
-        "slots": slots, "load": load, "requests": nreq, "pool_tiles": len(pool),
+        "slots": slots, "policy": policy, "load": load, "requests": nreq,
+        "pool_tiles": len(pool),
         "pool_points": {"min": int(npts.min()), "median": int(np.median(npts)),
                         "max": int(npts.max())},
         "single_tile_points_per_s": round(single, 1),
@@ -212,9 +220,10 @@ def main(argv=None):
     ap.add_argument("--load", type=float, default=0.7, help="arrival rate / capacity")
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--seed", type=int, default=3000)
+    ap.add_argument("--policy", choices=("fifo", "sjf"), default="fifo")
     args = ap.parse_args(argv)
     pool = make_pool(args.pool, args.seed, args.max_points)
-    res = load_test(pool, args.slots, args.load, args.requests, args.seed)
+    res = load_test(pool, args.slots, args.load, args.requests, args.seed, policy=args.policy)
     print(json.dumps(res), flush=True)
     return 0 if res["mismatched_results"] == 0 else 1
 
