@@ -126,6 +126,60 @@ def test_dir_bin_matches_exact_bin(hm):
     assert bad == 0
 
 
+def care_bins(occ):
+    """Bins of the empty circular runs of at least 24 bins."""
+    empty = [not (occ >> b) & 1 for b in range(64)]
+    care = set()
+    if not any(empty):
+        return care
+    if all(empty):
+        return set(range(64))
+    for st in range(64):
+        if empty[st] and not empty[(st - 1) % 64]:
+            run = []
+            while empty[(st + len(run)) % 64]:
+                run.append((st + len(run)) % 64)
+            if len(run) >= 24:
+                care.update(run)
+    return care
+
+
+def test_wedge_pruning_is_conservative(hm):
+    # a box the wedge test rejects holds no direction in a run of >= 24 empty
+    # bins: brute force over points inside rejected boxes (corners, edges and
+    # random interior points), with bins from the exact direction binning
+    hm.wbh_wedge_hit.restype = C.c_int
+    hm.wbh_wedge_hit.argtypes = [C.c_uint64] + [C.c_double] * 6
+    hm.wbh_dir_bin.restype = C.c_int
+    hm.wbh_dir_bin.argtypes = [C.c_double, C.c_double]
+    rng = np.random.default_rng(21)
+    rejected = 0
+    for it in range(3000):
+        nb = int(rng.integers(1, 12))
+        occ = 0
+        start = int(rng.integers(0, 64))
+        for b in rng.integers(0, 40, nb):       # bins clustered on one side
+            occ |= 1 << int((start + b) % 64)
+        care = care_bins(occ)
+        px, py = rng.normal(size=2) * 50 + np.array([500000.0, 4400000.0])
+        w, h = rng.uniform(0.01, 40, 2)
+        cx, cy = px + rng.normal() * 60, py + rng.normal() * 60
+        x0, x1, y0, y1 = cx - w, cx + w, cy - h, cy + h
+        if hm.wbh_wedge_hit(occ, px, py, x0, x1, y0, y1):
+            continue
+        rejected += 1
+        us = np.concatenate([[0, 1, 0, 1, 0.5, 0.5, 0, 1], rng.random(60)])
+        vs = np.concatenate([[0, 0, 1, 1, 0, 1, 0.5, 0.5], rng.random(60)])
+        for u, v in zip(us, vs):
+            qx, qy = x0 + u * (x1 - x0), y0 + v * (y1 - y0)
+            if qx == px and qy == py:
+                continue
+            assert hm.wbh_dir_bin(qy - py, qx - px) not in care, (it, occ, u, v)
+    assert rejected > 300
+    # no directions yet: nothing may be skipped
+    assert hm.wbh_wedge_hit(0, 0.0, 0.0, 5.0, 6.0, 5.0, 6.0) == 1
+
+
 def test_x87_emulation_ops(hm):
     rng = np.random.default_rng(14)
     for _ in range(40000):

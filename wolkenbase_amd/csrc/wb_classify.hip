@@ -1391,6 +1391,7 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const Heavy
   __shared__ int stk[WB_STK];
   __shared__ uint32_t lq[HB_LEAVES], lb[HB_LEAVES], le[HB_LEAVES], lmask[HB_LEAVES];
   __shared__ uint32_t lpre[HB_LEAVES + 1];
+  __shared__ wb::Wedges wg;  // the care wedges of walks A and B (uniform)
   if (blockIdx.x >= nheavy) return;
   const uint64_t t_start = COUNT ? wall_clock64() : 0;
   uint64_t c_leaf = 0, c_pts = 0, c_exp = 0;
@@ -1438,6 +1439,10 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const Heavy
         sp = 1;
       }
     }
+    if (lane == 0) {
+      if (phase < 2) wb::care_wedges(checked, wg);
+      else wg.n = -1;
+    }
     __syncthreads();
     bool done = false;
     // batches grow 16, 64, 256 leaves: an early pass stops after few leaves
@@ -1460,7 +1465,8 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const Heavy
           const double ncx = cx + (double)(2 * (k & 1) - 1) * side / 4;
           const double ncy = cy + (double)(2 * ((k & 2) / 2) - 1) * side / 4;
           const double ncz = cz + (double)(2 * ((k & 4) / 4) - 1) * side / 4;
-          h = wb::hyp_intersect(H, ncx, ncy, ncz, side / 2);
+          h = wb::hyp_intersect(H, ncx, ncy, ncz, side / 2) &&
+              wb::wedge_hit(wg, x, y, ncx - side / 4, ncx + side / 4, ncy - side / 4, ncy + side / 4);
         }
         const int t = child_rank(k, o, zf);
         uint32_t rn = h && ck >= 0 ? 1u << t : 0u, rl = h && ck < -1 ? 1u << t : 0u;
@@ -1490,26 +1496,27 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const Heavy
           const uint32_t leaf = lq[k];
           const uint32_t b = T.leaf_start[leaf], e = T.leaf_start[leaf + 1];
           const uint32_t g0 = b / WB_CHUNK, ng = (e - 1) / WB_CHUNK - g0 + 1;
+          // bits 0-30: the groups to read (groups past the 31st are always
+          // read); bit 31: the leaf cube lies inside the hyperboloid
           uint32_t mask = 0x7fffffffu;
-          if (phase != 0) {
-            const double *lc = T.leaf_cube + 4 * (size_t)leaf;
-            if (wb::hyp_in_cube(H, lc[0], lc[1], lc[2], lc[3])) {
-              mask = 0xffffffffu;
-            } else {
-              // bits 0-30: groups whose box may meet the hyperboloid (groups
-              // past the 31st are never pruned)
-              mask = 0;
-              for (uint32_t g = 0; g < ng && g < 31; g++) {
-                const double *bx = T.chunk_box + 6 * (uint64_t)(g0 + g);
-                if (wb::hyp_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]))
-                  mask |= 1u << g;
-              }
+          const double *lc = T.leaf_cube + 4 * (size_t)leaf;
+          const double h2 = lc[3] / 2;
+          // the wedges may have narrowed since the leaf was collected
+          const bool seen = wb::wedge_hit(wg, x, y, lc[0] - h2, lc[0] + h2, lc[1] - h2, lc[1] + h2);
+          if (phase != 0 && seen) {
+            const bool inside = wb::hyp_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
+            mask = inside ? 0x80000000u : 0u;
+            for (uint32_t g = 0; g < ng && g < 31; g++) {
+              const double *bx = T.chunk_box + 6 * (uint64_t)(g0 + g);
+              if ((inside || wb::hyp_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5])) &&
+                  wb::wedge_hit(wg, x, y, bx[0], bx[1], bx[2], bx[3]))
+                mask |= 1u << g;
             }
           }
           lb[k] = b;
           le[k] = e;
           lmask[k] = mask;
-          cnt = phase == 0 ? ng : e - b;
+          cnt = !seen ? 0 : phase == 0 ? ng : e - b;
         }
         uint32_t incl = cnt;
         for (int s = 1; s < WAVE; s <<= 1) {
@@ -1538,7 +1545,7 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const Heavy
           } else {
             j = b + o;
             const uint32_t g = j / WB_CHUNK - b / WB_CHUNK;
-            if ((mask >> 31) == 0 && g < 31 && !((mask >> g) & 1)) test = false;
+            if (g < 31 && !((mask >> g) & 1)) test = false;
           }
           if (test) {
             const double qx = T.px[j], qy = T.py[j], qz = T.pz[j];
@@ -1567,6 +1574,9 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const Heavy
               verdict = 1;
               done = true;
             }
+            __syncthreads();
+            if (lane == 0) wb::care_wedges(occw, wg);
+            __syncthreads();
           }
         }
       }

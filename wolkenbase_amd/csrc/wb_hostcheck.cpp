@@ -38,6 +38,14 @@ int wbh_dir_bin(double dy, double dx)
                      g_k);
 }
 
+// the care wedges of a direction mask and the box test against them
+int wbh_wedge_hit(uint64_t occ, double px, double py, double x0, double x1, double y0, double y1)
+{
+  wb::Wedges w;
+  wb::care_wedges(occ, w);
+  return wb::wedge_hit(w, px, py, x0, x1, y0, y1) ? 1 : 0;
+}
+
 int wbh_atan2i_forced(double y, double x)
 {
   ready();

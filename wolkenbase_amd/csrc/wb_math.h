@@ -615,4 +615,68 @@ WB_HD bool hyp_may_hit_box(const Shape &s, double x0, double x1, double y0, doub
   return hyp_in(s, qx, qy, s.slope > 0 ? z0 : z1);
 }
 
+// ---------------------------------------------------------------------------
+// Direction wedges that can still change a ground test.  With the direction
+// bins found so far in `occ`, only a direction in an empty run of >= 24 bins
+// can change the outcome (a pass needs every such run broken; a direction in
+// a shorter run or an occupied bin leaves every run of >= 24 as it is, and
+// runs only shrink as directions are added).  So a box whose xy footprint
+// lies, as seen from p, outside every such run cannot change the outcome and
+// need not be read.  The wedges are the runs widened by 1e-3 rad on each side
+// (bins are pi/32 wide and the bin of a direction is that of atan2i, within
+// 2^-31 turn of the true angle), split in two where wider than pi so that
+// each is convex; at most two runs of >= 24 bins fit in 64 bins.  The exact
+// border pass never uses them: it needs the directions in the occupied bins
+// next to a run.
+// ---------------------------------------------------------------------------
+struct Wedges {
+  int n;                // < 0: no pruning
+  float ax[4], ay[4], bx[4], by[4];  // wedge k spans counterclockwise a_k .. b_k
+};
+
+WB_HD void care_wedges(uint64_t occ, Wedges &w)
+{
+  w.n = -1;
+  if (occ == 0) return;
+  int n = 0;
+  const double u = 3.14159265358979323846 / 32, d = 1e-3;
+  for (int st = 0; st < 64; st++) {
+    if ((occ >> st) & 1) continue;
+    if (!((occ >> ((st + 63) & 63)) & 1)) continue;
+    int len = 0;
+    while (len < 64 && !((occ >> ((st + len) & 63)) & 1)) len++;
+    if (len < 24) continue;
+    const double t0 = -3.14159265358979323846 + st * u - d, t1 = t0 + len * u + 2 * d;
+    const int parts = t1 - t0 > 3.14159265358979323846 ? 2 : 1;
+    for (int k = 0; k < parts && n < 4; k++) {
+      const double a = t0 + (t1 - t0) * k / parts, b = t0 + (t1 - t0) * (k + 1) / parts;
+      w.ax[n] = (float)cos(a); w.ay[n] = (float)sin(a);
+      w.bx[n] = (float)cos(b); w.by[n] = (float)sin(b);
+      n++;
+    }
+  }
+  w.n = n;
+}
+
+// false only if the box [x0,x1] x [y0,y1] lies outside every wedge as seen
+// from (px, py): all four corners strictly right of a wedge's first edge, or
+// all strictly left of its second edge (the box is convex)
+WB_HD bool wedge_hit(const Wedges &w, double px, double py, double x0, double x1,
+                                 double y0, double y1)
+{
+  if (w.n < 0) return true;
+  const float u0 = (float)(x0 - px), u1 = (float)(x1 - px);
+  const float v0 = (float)(y0 - py), v1 = (float)(y1 - py);
+  for (int k = 0; k < w.n; k++) {
+    const float ax = w.ax[k], ay = w.ay[k], bx = w.bx[k], by = w.by[k];
+    const bool right_a = ax * v0 - ay * u0 < 0 && ax * v0 - ay * u1 < 0 &&
+                         ax * v1 - ay * u0 < 0 && ax * v1 - ay * u1 < 0;
+    const bool left_b = u0 * by - v0 * bx < 0 && u1 * by - v0 * bx < 0 &&
+                        u0 * by - v1 * bx < 0 && u1 * by - v1 * bx < 0;
+    if (!right_a && !left_b) return true;
+  }
+  return false;
+}
+
+
 }  // namespace wb
