@@ -144,6 +144,22 @@ def care_bins(occ):
     return care
 
 
+def test_care_runs_match_brute_force(hm):
+    hm.wbh_care_mask.restype = C.c_uint64
+    hm.wbh_care_mask.argtypes = [C.c_uint64]
+    rng = np.random.default_rng(22)
+    masks = [0, 1, 1 << 63, (1 << 64) - 1, 1 | (1 << 32), 1 | (1 << 25), 1 | (1 << 24)]
+    for _ in range(20000):
+        nb = int(rng.integers(1, 10))
+        occ = 0
+        for b in rng.integers(0, 64, nb):
+            occ |= 1 << int(b)
+        masks.append(occ)
+    for occ in masks:
+        want = sum(1 << b for b in care_bins(occ)) if occ else (1 << 64) - 1
+        assert hm.wbh_care_mask(occ) == want, hex(occ)
+
+
 def test_wedge_pruning_is_conservative(hm):
     # a box the wedge test rejects holds no direction in a run of >= 24 empty
     # bins: brute force over points inside rejected boxes (corners, edges and
@@ -178,6 +194,40 @@ def test_wedge_pruning_is_conservative(hm):
     assert rejected > 300
     # no directions yet: nothing may be skipped
     assert hm.wbh_wedge_hit(0, 0.0, 0.0, 5.0, 6.0, 5.0, 6.0) == 1
+    # runs of exactly 24, 31, 32 (more than pi with the margins), 33, 40 and
+    # two runs, with small boxes all round p at every angle
+    specials = []
+    for run in (24, 31, 32, 33, 40, 62):
+        s = int(rng.integers(0, 64))
+        specials.append(sum(1 << ((s + run + k) % 64) for k in range(64 - run)))
+    specials.append(((1 << 64) - 1) & ~(((1 << 24) - 1) | (((1 << 25) - 1) << 30)))
+    for occ in specials:
+        care = care_bins(occ)
+        # run edges, the edges half a turn away, and sub-margin offsets
+        starts = [b for b in care if (b - 1) % 64 not in care]
+        ends = [(b + 1) % 64 for b in care if (b + 1) % 64 not in care]
+        edges = [-np.pi + k * np.pi / 32 for k in starts + ends]
+        edges += [e + np.pi for e in edges]
+        for e in edges:
+            for off in (-3e-3, -1e-3, -6e-4, -2e-4, -1e-5, 0.0, 1e-5, 2e-4, 6e-4, 1e-3, 3e-3):
+                r = rng.uniform(5, 80)
+                cx, cy = r * np.cos(e + off), r * np.sin(e + off)
+                w = 1e-5 * r
+                x0, x1, y0, y1 = cx - w, cx + w, cy - w, cy + w
+                if hm.wbh_wedge_hit(occ, 0.0, 0.0, x0, x1, y0, y1):
+                    continue
+                for qx, qy in ((x0, y0), (x0, y1), (x1, y0), (x1, y1), (cx, cy)):
+                    assert hm.wbh_dir_bin(qy, qx) not in care, (hex(occ), e, off)
+        for ang in np.linspace(-np.pi, np.pi, 721):
+            r = rng.uniform(0.5, 80)
+            cx, cy = r * np.cos(ang), r * np.sin(ang)
+            w = rng.uniform(0.001, 0.3 * r)
+            x0, x1, y0, y1 = cx - w, cx + w, cy - w, cy + w
+            if hm.wbh_wedge_hit(occ, 0.0, 0.0, x0, x1, y0, y1):
+                continue
+            for u, v in zip(rng.random(40), rng.random(40)):
+                qx, qy = x0 + u * (x1 - x0), y0 + v * (y1 - y0)
+                assert hm.wbh_dir_bin(qy, qx) not in care, (hex(occ), ang, u, v)
 
 
 def test_x87_emulation_ops(hm):

@@ -85,6 +85,10 @@ int wb_ctx_create(int device, wb_ctx **out)
     WB_CHECK(hipMemcpy(c->d_tan, tabs, sizeof tabs, hipMemcpyHostToDevice));
     c->d_cos = c->d_tan + 511;
     c->d_sin = c->d_tan + 1023;
+    float edges[3 * 128];
+    wb::fill_edge_table(edges);
+    WB_CHECK(hipMalloc(&c->d_edges, sizeof edges));
+    WB_CHECK(hipMemcpy(c->d_edges, edges, sizeof edges, hipMemcpyHostToDevice));
     c->kconst = wb::atan_const();
   } catch (const std::string &m) {
     delete c;
@@ -107,6 +111,7 @@ void wb_ctx_destroy(wb_ctx *c)
   c->trees.clear();
   for (auto &kv : c->cache) hipFree(kv.second);
   hipFree(c->d_tan);
+  hipFree(c->d_edges);
   hipStreamDestroy(c->stream);
   delete c;
 }

@@ -1027,6 +1027,7 @@ struct ClassifyArgs {
   const TileRec *rec;
   double thickness, max_slope;
   const double *tab;     // tan[511], cos[512], sin[512] packed
+  const float *edges;    // bin-edge directions (wb::fill_edge_table)
   wb::AtanConst K;
   uint8_t *cls;          // per sorted slot
   uint32_t *nslow;       // points decided by the exact gap pass
@@ -1037,6 +1038,40 @@ struct ClassifyArgs {
 };
 
 __device__ inline uint64_t rotr64(uint64_t x, int a) { return (x >> a) | (x << (64 - a)); }
+
+// Per-lane care wedges of k_ground: the wedges of wb::care_wedges for each
+// lane, kept in LDS; a lane with more than two wedges (two runs, one of them
+// 32 bins or more) is not pruned (n < 0).
+struct LaneWedges {
+  float ax[2][256], ay[2][256], bx[2][256], by[2][256];
+  int n[256];
+};
+
+__device__ inline void lane_wedges_update(uint64_t occ, const float *edges, LaneWedges &W, int t)
+{
+  wb::Wedges w;
+  wb::care_wedges(occ, w, edges);
+  const int n = w.n >= 0 && w.n <= 2 ? w.n : -1;
+  for (int k = 0; k < n; k++) {
+    W.ax[k][t] = w.ax[k]; W.ay[k][t] = w.ay[k];
+    W.bx[k][t] = w.bx[k]; W.by[k][t] = w.by[k];
+  }
+  W.n[t] = n;
+}
+
+// wb::wedge_hit against thread t's wedges
+__device__ inline bool lane_hit(const LaneWedges &W, int t, double px, double py, double x0,
+                                double x1, double y0, double y1)
+{
+  wb::Wedges w;
+  w.n = W.n[t];
+  if (w.n < 0) return true;
+  for (int k = 0; k < 2; k++) {
+    w.ax[k] = W.ax[k][t]; w.ay[k] = W.ay[k][t];
+    w.bx[k] = W.bx[k][t]; w.by[k] = W.by[k][t];
+  }
+  return wb::wedge_hit(w, px, py, x0, x1, y0, y1);
+}
 
 // classification of the empty-bin mask: 1 pass, 2 fail, 0 undecided
 __device__ inline int gap_verdict(uint64_t occupied)
@@ -1112,7 +1147,8 @@ __device__ inline int child_rank(int k, int o, int zf) { return ((k ^ o) & 3) | 
 // hyperboloid of a lane in `m` (this lane takes part iff `on`), in reverse
 // visiting order.  Returns the new stack depth.
 __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int code, bool on,
-                                  const wb::Shape &H, double rx, double ry, int zf)
+                                  const wb::Shape &H, double rx, double ry, int zf,
+                                  const LaneWedges &LW)
 {
   const double *nc = T.node_cube + 4 * (size_t)code;
   const double cx = nc[0], cy = nc[1], cz = nc[2], side = nc[3];
@@ -1129,7 +1165,9 @@ __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int c
       const double ncx = cx + (double)(2 * (k & 1) - 1) * side / 4;
       const double ncy = cy + (double)(2 * ((k & 2) / 2) - 1) * side / 4;
       const double ncz = cz + (double)(2 * ((k & 4) / 4) - 1) * side / 4;
-      const bool h = on && wb::hyp_intersect(H, ncx, ncy, ncz, side / 2);
+      const bool h = on && wb::hyp_intersect(H, ncx, ncy, ncz, side / 2) &&
+                     lane_hit(LW, threadIdx.x, H.cx, H.cy, ncx - side / 4, ncx + side / 4,
+                              ncy - side / 4, ncy + side / 4);
       hits |= (h ? 1u : 0u) << k;
       if (__ballot(h)) pushed |= 1u << child_rank(k, o, zf);
     }
@@ -1181,7 +1219,10 @@ template <bool COUNT, int MINW>
 __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
 {
   __shared__ WaveStack stacks[256 / WAVE];
+  __shared__ LaneWedges LW;
   WaveStack &S = stacks[threadIdx.x / WAVE];
+  LW.n[threadIdx.x] = -1;
+  __syncthreads();
   const TreeView &T = A.T;
   const uint32_t lane = __lane_id();
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1250,20 +1291,25 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
           if (phase == 0) cnt[CT_EXP_A]++;
           else cnt[CT_EXP_B]++;
         }
-        sp = expand_node(T, S, sp, code, on, H, rx, ry, zf);
+        sp = expand_node(T, S, sp, code, on, H, rx, ry, zf, LW);
         continue;
       }
       const int leaf = -2 - code;
       const uint32_t b = (uint32_t)rfl((int)T.leaf_start[leaf]);
       const uint32_t e = (uint32_t)rfl((int)T.leaf_start[leaf + 1]);
+      // the lane skips a leaf outside its care wedges
+      const double *lc = T.leaf_cube + 4 * (size_t)leaf;
+      const double lh = lc[3] / 2;
+      const bool onl = on && lane_hit(LW, threadIdx.x, x, y, lc[0] - lh, lc[0] + lh,
+                                      lc[1] - lh, lc[1] + lh);
       if (phase == 0) {
         if (COUNT) {
           cnt[CT_WLEAF_A]++;
-          if (on) cnt[CT_LEAF_A]++;
+          if (onl) cnt[CT_LEAF_A]++;
         }
         const uint32_t g0 = b / WB_CHUNK, ng = (e - 1) / WB_CHUNK - g0 + 1;
-        if (on) groups_a += ng;
-        if (COUNT && on) cnt[CT_GROUP_A] += ng;
+        if (onl) groups_a += ng;
+        if (COUNT && onl) cnt[CT_GROUP_A] += ng;
         // the lowest point of each group (the likeliest to lie in a downward
         // hyperboloid), or the group's first slot in the leaf when the
         // lowest one belongs to the neighbouring leaf; lane g fetches the
@@ -1279,7 +1325,7 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
           const uint32_t ns = min((uint32_t)WAVE, ng - gb);
           for (uint32_t s = 0; s < ns; s++) {
             const double qx = rdl(sx, s), qy = rdl(sy, s), qz = rdl(sz, s);
-            if (on && wb::hyp_in(H, qx, qy, qz) && (qx != x || qy != y)) {
+            if (onl && wb::hyp_in(H, qx, qy, qz) && (qx != x || qy != y)) {
               if (COUNT) cnt[CT_DIR_A]++;
               occ |= 1ull << wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
             }
@@ -1288,10 +1334,9 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
       } else {
         if (COUNT) {
           cnt[CT_WLEAF_B]++;
-          if (on) cnt[CT_LEAF_B]++;
+          if (onl) cnt[CT_LEAF_B]++;
         }
-        const double *lc = T.leaf_cube + 4 * (size_t)leaf;
-        const bool all = on && wb::hyp_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
+        const bool all = onl && wb::hyp_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
         for (uint32_t c0 = b; c0 < e;) {
           const uint32_t chk = c0 / WB_CHUNK;
           const uint32_t ce = min(e, (chk + 1) * WB_CHUNK);
@@ -1300,10 +1345,11 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
           if (lane < ce - c0) {
             sx = T.px[c0 + lane]; sy = T.py[c0 + lane]; sz = T.pz[c0 + lane];
           }
-          bool chit = on;
-          if (chit && !all) {
+          bool chit = onl;
+          if (chit) {
             const double *bx = T.chunk_box + 6 * (uint64_t)chk;
-            chit = wb::hyp_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+            if (!all) chit = wb::hyp_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+            if (chit) chit = lane_hit(LW, threadIdx.x, x, y, bx[0], bx[1], bx[2], bx[3]);
           }
           if (__ballot(chit)) {
             if (COUNT && chit) cnt[CT_POINT_B] += ce - c0;
@@ -1321,6 +1367,7 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
       if (state == 0 && occ != checked) {
         checked = occ;
         if (gap_verdict(occ) == 1) state = 1;
+        else lane_wedges_update(occ, A.edges, LW, threadIdx.x);
       }
       if (phase == 0 && state == 0 && groups_a > HEAVY_A_GROUPS) state = 5;
       alive = __ballot(state == 0);
@@ -1440,7 +1487,7 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const Heavy
       }
     }
     if (lane == 0) {
-      if (phase < 2) wb::care_wedges(checked, wg);
+      if (phase < 2) wb::care_wedges(checked, wg, A.edges);
       else wg.n = -1;
     }
     __syncthreads();
@@ -1575,7 +1622,7 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const Heavy
               done = true;
             }
             __syncthreads();
-            if (lane == 0) wb::care_wedges(occw, wg);
+            if (lane == 0) wb::care_wedges(occw, wg, A.edges);
             __syncthreads();
           }
         }
@@ -1772,6 +1819,7 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
     A.T = T; A.g = g; A.memb = memb; A.nmemb = nmemb; A.rec = rec;
     A.thickness = p->thickness; A.max_slope = p->max_slope;
     A.tab = c->d_tan; A.K = c->kconst; A.cls = cls; A.nslow = nslow;
+    A.edges = c->d_edges;
     A.atan_exact = ctr + 1;
     WB_CHECK(hipEventRecord(ev[4], st));
     static const bool count_work = getenv("WB_CLASSIFY_COUNTERS") != nullptr;

@@ -38,11 +38,29 @@ int wbh_dir_bin(double dy, double dx)
                      g_k);
 }
 
+// the empty runs of >= 24 bins as a bin mask (all ones for -1 runs)
+uint64_t wbh_care_mask(uint64_t occ)
+{
+  int s[2], l[2];
+  const int n = wb::care_runs(occ, s, l);
+  if (n < 0) return ~0ull;
+  uint64_t m = 0;
+  for (int r = 0; r < n; r++)
+    for (int k = 0; k < l[r]; k++) m |= 1ull << ((s[r] + k) & 63);
+  return m;
+}
+
 // the care wedges of a direction mask and the box test against them
 int wbh_wedge_hit(uint64_t occ, double px, double py, double x0, double x1, double y0, double y1)
 {
+  static float tab[3 * 128];
+  static int filled;
+  if (!filled) {
+    wb::fill_edge_table(tab);
+    filled = 1;
+  }
   wb::Wedges w;
-  wb::care_wedges(occ, w);
+  wb::care_wedges(occ, w, tab);
   return wb::wedge_hit(w, px, py, x0, x1, y0, y1) ? 1 : 0;
 }
 

@@ -39,6 +39,7 @@ struct wb_ctx {
   std::string err;
   // atan2i tables and constants (device)
   double *d_tan, *d_cos, *d_sin;
+  float *d_edges;  // bin-edge directions of the ground test (wb::fill_edge_table)
   wb::AtanConst kconst;
   // size-keyed cache of device buffers
   std::multimap<size_t, void *> cache;

@@ -629,30 +629,76 @@ WB_HD bool hyp_may_hit_box(const Shape &s, double x0, double x1, double y0, doub
 // border pass never uses them: it needs the directions in the occupied bins
 // next to a run.
 // ---------------------------------------------------------------------------
+WB_HD uint64_t rotr64_(uint64_t x, int a) { return a ? (x >> a) | (x << (64 - a)) : x; }
+WB_HD uint64_t rotl64_(uint64_t x, int a) { return a ? (x << a) | (x >> (64 - a)) : x; }
+
+// The empty circular runs of >= 24 bins of occ: returns their number (at
+// most two fit) with start bins and lengths, or -1 when occ is empty (one
+// run of all 64 bins: every direction may still matter).
+WB_HD int care_runs(uint64_t occ, int start[2], int len[2])
+{
+  if (occ == 0) return -1;
+  const uint64_t e = ~occ;
+  const uint64_t r2 = e & rotr64_(e, 1), r4 = r2 & rotr64_(r2, 2), r8 = r4 & rotr64_(r4, 4);
+  const uint64_t r24 = r8 & rotr64_(r8, 8) & rotr64_(r8, 16);  // bins b..b+23 empty
+  // every bin within 23 after such a b
+  uint64_t c = r24 | rotl64_(r24, 1);
+  c |= rotl64_(c, 2);
+  c |= rotl64_(c, 4);
+  c |= rotl64_(c, 8);
+  c |= rotl64_(c, 8);
+  int n = 0;
+  for (uint64_t sm = c & ~rotl64_(c, 1); sm && n < 2; sm &= sm - 1) {
+    const int s = __builtin_ctzll(sm);
+    start[n] = s;
+    len[n] = __builtin_ctzll(~rotr64_(c, s));
+    n++;
+  }
+  return n;
+}
+
 struct Wedges {
   int n;                // < 0: no pruning
   float ax[4], ay[4], bx[4], by[4];  // wedge k spans counterclockwise a_k .. b_k
 };
 
-WB_HD void care_wedges(uint64_t occ, Wedges &w)
+// Unit vectors of the bin edges (angle -pi + k pi/32, k = 0..63), turned by
+// side * 1e-3 rad (side -1, 0, +1): 3 x 64 (cos, sin) pairs, side-major.
+// The kernels read this table, built once on the host by the same function.
+WB_HD void fill_edge_table(float *tab)
+{
+  for (int side = -1; side <= 1; side++)
+    for (int k = 0; k < 64; k++) {
+      const double t = -3.14159265358979323846 + k * (3.14159265358979323846 / 32) + side * 1e-3;
+      tab[(side + 1) * 128 + 2 * k] = (float)cos(t);
+      tab[(side + 1) * 128 + 2 * k + 1] = (float)sin(t);
+    }
+}
+
+// The care wedges of occ from the edge table: a run of bins [s, s+len) is
+// the wedge from edge s turned back by 1e-3 rad to edge s+len turned on by
+// 1e-3 rad; a run of 32 bins or more spans more than pi with the margins and
+// is split at its middle edge into two convex wedges.
+WB_HD void care_wedges(uint64_t occ, Wedges &w, const float *tab)
 {
   w.n = -1;
-  if (occ == 0) return;
+  int rs[2], rl[2];
+  const int nr = care_runs(occ, rs, rl);
+  if (nr < 0) return;
   int n = 0;
-  const double u = 3.14159265358979323846 / 32, d = 1e-3;
-  for (int st = 0; st < 64; st++) {
-    if ((occ >> st) & 1) continue;
-    if (!((occ >> ((st + 63) & 63)) & 1)) continue;
-    int len = 0;
-    while (len < 64 && !((occ >> ((st + len) & 63)) & 1)) len++;
-    if (len < 24) continue;
-    const double t0 = -3.14159265358979323846 + st * u - d, t1 = t0 + len * u + 2 * d;
-    const int parts = t1 - t0 > 3.14159265358979323846 ? 2 : 1;
-    for (int k = 0; k < parts && n < 4; k++) {
-      const double a = t0 + (t1 - t0) * k / parts, b = t0 + (t1 - t0) * (k + 1) / parts;
-      w.ax[n] = (float)cos(a); w.ay[n] = (float)sin(a);
-      w.bx[n] = (float)cos(b); w.by[n] = (float)sin(b);
-      n++;
+  auto add = [&](int ka, int sa, int kb, int sb) {
+    w.ax[n] = tab[(sa + 1) * 128 + 2 * ka]; w.ay[n] = tab[(sa + 1) * 128 + 2 * ka + 1];
+    w.bx[n] = tab[(sb + 1) * 128 + 2 * kb]; w.by[n] = tab[(sb + 1) * 128 + 2 * kb + 1];
+    n++;
+  };
+  for (int r = 0; r < nr; r++) {
+    const int s = rs[r], len = rl[r], e = (s + len) & 63;
+    if (len < 32) {
+      add(s, -1, e, 1);
+    } else {
+      const int m = (s + len / 2) & 63;
+      add(s, -1, m, 0);
+      add(m, 0, e, 1);
     }
   }
   w.n = n;
