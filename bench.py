@@ -152,8 +152,8 @@ def main():
 
     st = stats[-1]
     kern = {
-        "k_classify": avg("ms_classify_kernel"),
-        "k_classify_heavy": avg("ms_heavy_kernel"),
+        "k_ground": avg("ms_classify_kernel"),
+        "k_ground_heavy": avg("ms_heavy_kernel"),
         "radix_sort": avg("ms_sort"),
     }
     dominant = max(kern, key=kern.get)
@@ -163,7 +163,7 @@ def main():
     # the sort moves (u64 key, u32 index) records, 24 B per point per pass,
     # plus the 8-B key read of each digit histogram.
     nh = int(st["heavy_points"])
-    alg = {"k_classify": 13.0 * n, "k_classify_heavy": 13.0 * nh,
+    alg = {"k_ground": 13.0 * n, "k_ground_heavy": 13.0 * nh,
            "radix_sort": (24.0 + 8.0) * passes * n}
     achieved = alg[dominant] / (kern[dominant] * 1e-3) / 1e9
     traffic = None

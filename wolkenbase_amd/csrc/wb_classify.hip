@@ -1040,8 +1040,8 @@ struct ClassifyArgs {
 __device__ inline uint64_t rotr64(uint64_t x, int a) { return (x >> a) | (x << (64 - a)); }
 
 // Per-lane care wedges of k_ground: the wedges of wb::care_wedges for each
-// lane, kept in LDS; a lane with more than two wedges (two runs, one of them
-// 32 bins or more) is not pruned (n < 0).
+// lane, at most two, kept in LDS; a lane with more than two wedges (two runs,
+// one of them 32 bins or more) is not pruned (n < 0).
 struct LaneWedges {
   float ax[2][256], ay[2][256], bx[2][256], by[2][256];
   int n[256];
@@ -1051,21 +1051,30 @@ __device__ inline void lane_wedges_update(uint64_t occ, const float *edges, Lane
 {
   wb::Wedges w;
   wb::care_wedges(occ, w, edges);
-  const int n = w.n >= 0 && w.n <= 2 ? w.n : -1;
-  for (int k = 0; k < n; k++) {
-    W.ax[k][t] = w.ax[k]; W.ay[k][t] = w.ay[k];
-    W.bx[k][t] = w.bx[k]; W.by[k][t] = w.by[k];
+  // live is 1, 3 (one run, split) or 5 (two runs) when at most two are in use;
+  // the second kept wedge is slot 1 or slot 2
+  const int n = w.live < 0 ? -1 : __builtin_popcount(w.live);
+  if (n >= 0 && n <= 2) {
+    const bool s1 = (w.live & 2) != 0;
+    W.ax[0][t] = w.ax[0]; W.ay[0][t] = w.ay[0];
+    W.bx[0][t] = w.bx[0]; W.by[0][t] = w.by[0];
+    W.ax[1][t] = s1 ? w.ax[1] : w.ax[2]; W.ay[1][t] = s1 ? w.ay[1] : w.ay[2];
+    W.bx[1][t] = s1 ? w.bx[1] : w.bx[2]; W.by[1][t] = s1 ? w.by[1] : w.by[2];
+    W.n[t] = n;
+  } else {
+    W.n[t] = -1;
   }
-  W.n[t] = n;
 }
 
 // wb::wedge_hit against thread t's wedges
 __device__ inline bool lane_hit(const LaneWedges &W, int t, double px, double py, double x0,
                                 double x1, double y0, double y1)
 {
+  const int n = W.n[t];
+  if (n < 0) return true;
   wb::Wedges w;
-  w.n = W.n[t];
-  if (w.n < 0) return true;
+  w.live = (1 << n) - 1;
+#pragma unroll
   for (int k = 0; k < 2; k++) {
     w.ax[k] = W.ax[k][t]; w.ay[k] = W.ay[k][t];
     w.bx[k] = W.bx[k][t]; w.by[k] = W.by[k][t];
@@ -1432,7 +1441,10 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
 #define HB_LEAVES 256
 
 template <bool COUNT>
-__global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const HeavyRec *heavy,
+#ifndef WB_HEAVY_MINW
+#define WB_HEAVY_MINW 4
+#endif
+__global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs A, const HeavyRec *heavy,
                                                      uint32_t nheavy)
 {
   __shared__ int stk[WB_STK];
@@ -1488,7 +1500,7 @@ __global__ void __launch_bounds__(64) k_ground_heavy(ClassifyArgs A, const Heavy
     }
     if (lane == 0) {
       if (phase < 2) wb::care_wedges(checked, wg, A.edges);
-      else wg.n = -1;
+      else wg.live = -1;
     }
     __syncthreads();
     bool done = false;

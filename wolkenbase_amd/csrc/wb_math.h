@@ -647,18 +647,20 @@ WB_HD int care_runs(uint64_t occ, int start[2], int len[2])
   c |= rotl64_(c, 4);
   c |= rotl64_(c, 8);
   c |= rotl64_(c, 8);
-  int n = 0;
-  for (uint64_t sm = c & ~rotl64_(c, 1); sm && n < 2; sm &= sm - 1) {
-    const int s = __builtin_ctzll(sm);
-    start[n] = s;
-    len[n] = __builtin_ctzll(~rotr64_(c, s));
-    n++;
-  }
-  return n;
+  // the two iterations written out, so the arrays stay in registers
+  uint64_t sm = c & ~rotl64_(c, 1);
+  if (!sm) return 0;
+  start[0] = __builtin_ctzll(sm);
+  len[0] = __builtin_ctzll(~rotr64_(c, start[0]));
+  sm &= sm - 1;
+  if (!sm) return 1;
+  start[1] = __builtin_ctzll(sm);
+  len[1] = __builtin_ctzll(~rotr64_(c, start[1]));
+  return 2;
 }
 
 struct Wedges {
-  int n;                // < 0: no pruning
+  int live;  // < 0: no pruning; else bit k set when wedge k is in use
   float ax[4], ay[4], bx[4], by[4];  // wedge k spans counterclockwise a_k .. b_k
 };
 
@@ -678,30 +680,34 @@ WB_HD void fill_edge_table(float *tab)
 // The care wedges of occ from the edge table: a run of bins [s, s+len) is
 // the wedge from edge s turned back by 1e-3 rad to edge s+len turned on by
 // 1e-3 rad; a run of 32 bins or more spans more than pi with the margins and
-// is split at its middle edge into two convex wedges.
+// is split at its middle edge into two convex wedges.  Run r fills slots 2r
+// and, when split, 2r+1 (fixed slots keep every index a constant).
 WB_HD void care_wedges(uint64_t occ, Wedges &w, const float *tab)
 {
-  w.n = -1;
+  w.live = -1;
   int rs[2], rl[2];
   const int nr = care_runs(occ, rs, rl);
   if (nr < 0) return;
-  int n = 0;
-  auto add = [&](int ka, int sa, int kb, int sb) {
-    w.ax[n] = tab[(sa + 1) * 128 + 2 * ka]; w.ay[n] = tab[(sa + 1) * 128 + 2 * ka + 1];
-    w.bx[n] = tab[(sb + 1) * 128 + 2 * kb]; w.by[n] = tab[(sb + 1) * 128 + 2 * kb + 1];
-    n++;
-  };
-  for (int r = 0; r < nr; r++) {
-    const int s = rs[r], len = rl[r], e = (s + len) & 63;
-    if (len < 32) {
-      add(s, -1, e, 1);
-    } else {
-      const int m = (s + len / 2) & 63;
-      add(s, -1, m, 0);
-      add(m, 0, e, 1);
+  int live = 0;
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    if (r < nr) {
+      const int s = rs[r], len = rl[r], e = (s + len) & 63;
+      const bool split = len >= 32;
+      const int m = split ? (s + len / 2) & 63 : e;
+      const int sm = split ? 0 : 1;
+      w.ax[2 * r] = tab[2 * s];
+      w.ay[2 * r] = tab[2 * s + 1];
+      w.bx[2 * r] = tab[(sm + 1) * 128 + 2 * m];
+      w.by[2 * r] = tab[(sm + 1) * 128 + 2 * m + 1];
+      w.ax[2 * r + 1] = tab[128 + 2 * m];
+      w.ay[2 * r + 1] = tab[128 + 2 * m + 1];
+      w.bx[2 * r + 1] = tab[256 + 2 * e];
+      w.by[2 * r + 1] = tab[256 + 2 * e + 1];
+      live |= (split ? 3 : 1) << (2 * r);
     }
   }
-  w.n = n;
+  w.live = live;
 }
 
 // false only if the box [x0,x1] x [y0,y1] lies outside every wedge as seen
@@ -710,10 +716,12 @@ WB_HD void care_wedges(uint64_t occ, Wedges &w, const float *tab)
 WB_HD bool wedge_hit(const Wedges &w, double px, double py, double x0, double x1,
                                  double y0, double y1)
 {
-  if (w.n < 0) return true;
+  if (w.live < 0) return true;
   const float u0 = (float)(x0 - px), u1 = (float)(x1 - px);
   const float v0 = (float)(y0 - py), v1 = (float)(y1 - py);
-  for (int k = 0; k < w.n; k++) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (!((w.live >> k) & 1)) continue;
     const float ax = w.ax[k], ay = w.ay[k], bx = w.bx[k], by = w.by[k];
     const bool right_a = ax * v0 - ay * u0 < 0 && ax * v0 - ay * u1 < 0 &&
                          ax * v1 - ay * u0 < 0 && ax * v1 - ay * u1 < 0;
