@@ -39,9 +39,13 @@ enum {
 };
 #define WB_SHAPE_NPARAMS 5  /* each shape takes 5 doubles; unused ones are ignored */
 
-/* flags of wb_tree_build */
+/* flags of wb_tree_build / wb_tree_build_xyz */
 enum {
   WB_DEVICE_INPUT = 1,      /* x, y, z are device pointers already resident in HBM */
+  WB_BUILD_DEDUPE = 2,      /* merge points at one location as OctBuffer::put does
+                               (octree.cpp:620-660): the location keeps the place of its
+                               first point and the record of its last; the tree then
+                               holds each location once */
 };
 
 /* precision of the scan's plane fit (wb_classify_params.fit_precision) */
@@ -112,8 +116,9 @@ typedef struct {
 
 /* --- context ------------------------------------------------------------ */
 int wb_ctx_create(int device, wb_ctx **out);
-/* also frees every tree built on the context that is still alive: their
- * handles are invalid afterwards and must not be passed to wb_tree_free */
+/* also frees every tree, LAS file and cloud made on the context that is
+ * still alive: their handles are invalid afterwards and must not be passed
+ * to wb_tree_free, wb_las_close or wb_cloud_free */
 void wb_ctx_destroy(wb_ctx *ctx);
 const char *wb_last_error(wb_ctx *ctx);
 /* synchronise the context's stream */
@@ -132,14 +137,37 @@ int wb_size_fit(const double *corners, int ncorners, double center[3], double *s
 int wb_tree_build(wb_ctx *ctx, const int32_t *x, const int32_t *y, const int32_t *z, uint64_t n,
                   const wb_lasxform *xf, const double root_center[3], double root_side,
                   int flags, wb_tree **out);
+/* --- octree build from coordinates already decoded to FP64 (several LAS
+ *     files with different transforms, a length unit other than 1: the
+ *     location LasHeader::readPoint computes, las.cpp:896).  resolution: a
+ *     spacing every nonzero coordinate difference is at least (per axis;
+ *     combineScales of the inputs, las.cpp:913-944, times the unit); the
+ *     ground test checks the cloud extent against it. */
+int wb_tree_build_xyz(wb_ctx *ctx, const double *x, const double *y, const double *z, uint64_t n,
+                      const double resolution[3], const double root_center[3], double root_side,
+                      int flags, wb_tree **out);
+/* inputs merged by WB_BUILD_DEDUPE (the count wolkencli prints as duplicate
+ * points, wolkencli.cpp:114-115); pairs (may be NULL) receives ndup
+ * (merged input index, input index of the first point at its location) */
+int wb_tree_duplicates(wb_tree *tree, uint64_t *ndup, uint32_t *pairs);
 int wb_tree_free(wb_tree *tree);
 int wb_tree_info(wb_tree *tree, uint64_t *nleaves, uint64_t *nnodes, int *maxdepth,
                  int *sort_passes);
 /* leaves in DFS (Octree::dump) order: replaces Octree::dump / OctStore::dump
  * (octree.cpp:360-376, 888-891) as a structured record */
 int wb_tree_leaves(wb_tree *tree, wb_leaf *out, uint64_t cap, uint64_t *nleaves);
-/* input indices in canonical order (leaf DFS order, input order in a leaf) */
+/* input indices in canonical order (leaf DFS order, input order in a leaf);
+ * order has one entry per location (n minus the merged duplicates) */
 int wb_tree_order(wb_tree *tree, uint32_t *order);
+
+/* --- the octree as text: OctStore::dump (octree.cpp:360-376, 673-689,
+ *     888-891), the "dumpfile" of wolkencli (wolkencli.cpp:122-124): per
+ *     leaf in DFS order "(cx,cy,cz)±half N points[, K stray points]", then
+ *     "T total points", numbers by ldecimal.  Written to path (may be NULL)
+ *     and/or buf (cap bytes, NUL-terminated); len receives the length. */
+int wb_tree_dump(wb_tree *tree, const char *path, char *buf, uint64_t cap, uint64_t *len);
+/* ldecimal (ldecimal.cpp:32-132): returns the length, or a negative code */
+int wb_ldecimal(double x, double toler, char *buf, int cap);
 
 /* --- batched queries: OctStore::countPointsIn / pointsIn
  *     (octree.cpp:1214-1264) for nshapes shapes of one kind.
@@ -159,13 +187,21 @@ int wb_points_in(wb_tree *tree, int kind, const double *params, uint64_t nshapes
  *     the single-threaded flowsnake order deciding overlaps (last tile wins).
  *     bbmin/bbmax: the LAS header corners (they set the tiling as
  *     wolkencanvas.cpp:501-519 does).  class_out: n bytes in input order
- *     (0 never classified, 1 not ground, 2 ground).  tiles_out may be NULL,
+ *     (0 never classified, 1 not ground, 2 ground); a merged duplicate gets the class of
+ *     its location.  tiles_out may be NULL,
  *     otherwise it receives 7^k records in flowsnake counter order.
  *     With WB_DEVICE_INPUT the tree was built from device input and class_out
  *     may be a device pointer too (flags of this call). */
 int wb_classify(wb_tree *tree, const double bbmin[3], const double bbmax[3],
                 const wb_classify_params *params, uint8_t *class_out, int flags,
                 wb_tile *tiles_out, uint64_t tiles_cap, wb_stats *stats);
+
+/* the same pass with the tiling taken from the corners of several LAS
+ * headers (x, y, z each): the BoundRect of them all, as sizeToFit builds it
+ * (wolkencanvas.cpp:136-146, 501-519) */
+int wb_classify_corners(wb_tree *tree, const double *corners, int ncorners,
+                        const wb_classify_params *params, uint8_t *class_out, int flags,
+                        wb_tile *tiles_out, uint64_t tiles_cap, wb_stats *stats);
 
 /* --- SCAN and POSTSCAN alone (scanCylinder and postscanCylinder over every
  *     tile, scan.cpp:31-179, in the phase order of wolkencanvas.cpp:469-535):
@@ -178,6 +214,105 @@ int wb_scan_tiles(wb_tree *tree, const double bbmin[3], const double bbmax[3],
 /* number of flowsnake tiles the classification uses for these corners */
 int wb_tile_count(const double bbmin[3], const double bbmax[3], double tile_size,
                   uint64_t *ntiles);
+int wb_tile_count_corners(const double *corners, int ncorners, double tile_size,
+                          uint64_t *ntiles);
+
+/* --- LAS files (las.h:97-183) -------------------------------------------
+ * A wb_las is one LAS file opened for reading: its header, parsed as
+ * LasHeader::openRead does (las.cpp:272-428, including the repair of the
+ * point counts), and its point records copied to HBM as they lie in the
+ * file.  LASzip files are refused (no LASzip here). */
+typedef struct wb_las wb_las;
+typedef struct {
+  uint16_t source_id, global_encoding;
+  uint8_t guid[16];
+  uint8_t version_major, version_minor;
+  char system_id[33], software[33];
+  uint16_t creation_day, creation_year;
+  uint32_t header_size, point_offset, n_vlr;
+  uint16_t point_format, point_length;
+  double scale[3], offset[3], min[3], max[3];
+  uint64_t start_waveform, start_evlr;
+  uint32_t n_evlr;
+  uint64_t n_points[16];    /* [0] total, [r] return r, after the repair */
+  int32_t zipped;           /* LASzip file */
+  int32_t valid;            /* LasHeader::isValid (las.cpp:597-601) */
+} wb_las_header;
+
+/* the header alone (host only) */
+int wb_las_read_header(const char *path, wb_las_header *hdr);
+int wb_las_open(wb_ctx *ctx, const char *path, wb_las **out, wb_las_header *hdr);
+/* the same from a whole file image in host memory */
+int wb_las_open_mem(wb_ctx *ctx, const void *image, uint64_t len, wb_las **out,
+                    wb_las_header *hdr);
+int wb_las_close(wb_las *las);
+
+/* --- a cloud: the points of one or more LAS files in HBM, decoded to FP64
+ *     as LasHeader::readPoint does (las.cpp:808-904), in file order.
+ *     WB_ZEROS_READ applies the READ phase's rule for return number 0
+ *     (threads.cpp:479-545: when the first record has a return number,
+ *     records without one are dropped, otherwise they are read as first
+ *     returns); WB_ZEROS_KEEP keeps every record as read (wolkencli.cpp:
+ *     99-111).  The files must stay open while the cloud lives. */
+typedef struct wb_cloud wb_cloud;
+enum { WB_ZEROS_READ = 0, WB_ZEROS_KEEP = 1 };
+int wb_cloud_create(wb_ctx *ctx, wb_las *const *files, int nfiles, double unit, int zeros,
+                    wb_cloud **out);
+/* points, the coordinate lattice spacing (combineScales of the headers
+ * times the unit, las.cpp:934-944), joinPointFormat of the inputs
+ * (las.cpp:104-121), and the 2*nfiles header corners times the unit (each
+ * may be NULL) */
+int wb_cloud_info(wb_cloud *cloud, uint64_t *npoints, double resolution[3], int *join_format,
+                  double *corners);
+/* device pointers of the decoded coordinates */
+int wb_cloud_points(wb_cloud *cloud, const double **x, const double **y, const double **z);
+/* host copies of the decoded coordinates and of each point's record index
+ * in its file (any may be NULL) */
+int wb_cloud_read_points(wb_cloud *cloud, double *x, double *y, double *z, uint32_t *rec);
+/* the octree of the cloud, duplicates merged as OctBuffer::put does */
+int wb_cloud_build(wb_cloud *cloud, const double root_center[3], double root_side, int flags,
+                   wb_tree **out);
+/* wb_classify_corners with the tiling of the cloud's header corners */
+int wb_cloud_classify(wb_cloud *cloud, wb_tree *tree, const wb_classify_params *params,
+                      uint8_t *class_out, int flags, wb_tile *tiles_out, uint64_t tiles_cap,
+                      wb_stats *stats);
+int wb_cloud_free(wb_cloud *cloud);
+
+/* --- LAS output: CloudOutput::openFiles / writeFiles / closeFiles
+ *     (cloudoutput.cpp:119-246) with the octree's leaves in DFS order as its
+ *     blocks, and LasHeader::writePoint / writeHeader (las.cpp:549-595,
+ *     906-990).  Points whose class byte is 0 keep the class of their record;
+ *     a merged duplicate is written once, with the record of the last input
+ *     at its location.  Files are named
+ *       name-<class>-<k>.las (separate_classes) or name-<k>.las,
+ *     <k> only when points_per_file is nonzero. */
+enum { WB_WRITE_DRY = 4 };  /* flag: encode and plan, write no files */
+typedef struct {
+  const char *name;
+  int32_t separate_classes;
+  int32_t points_per_file;  /* 0: no limit; the GUI uses 10000000 */
+  int32_t point_format;     /* -1: joinPointFormat of the inputs */
+  int32_t creation_day, creation_year;  /* 0: today (UTC) */
+} wb_las_write_params;
+typedef struct {
+  uint64_t class_totals[256]; /* stored points per class (OctStore::countClasses) */
+  uint64_t npoints;           /* points written */
+  uint64_t bytes;             /* bytes written, headers included */
+  uint32_t nfiles;
+  int32_t point_format;
+  double scale[3], offset[3]; /* of the output files (LasHeader::setScale) */
+  double ms_classes;          /* device: class of each stored point, totals */
+  double ms_plan;             /* host: per-leaf counts, file plan, uploads */
+  double ms_encode;           /* device: the record-encode kernel */
+  double ms_copy;             /* device to host copies */
+  double ms_disk;             /* file writes */
+} wb_las_write_stats;
+/* cls: class bytes of wb_classify in cloud order (NULL: every point keeps
+ * its record's class; a device pointer with WB_DEVICE_INPUT); names (may be
+ * NULL) receives the file names, one per line */
+int wb_cloud_write(wb_cloud *cloud, wb_tree *tree, const uint8_t *cls, int flags,
+                   const wb_las_write_params *params, wb_las_write_stats *stats, char *names,
+                   uint64_t names_cap);
 
 #ifdef __cplusplus
 }

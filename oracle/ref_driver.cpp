@@ -1,6 +1,6 @@
 // ref_driver.cpp -- extern "C" entry points over the reference's own
 // geometry/angle/summation sources (angle.cpp, point.cpp, shape.cpp,
-// manysum.cpp, ldecimal.cpp), compiled from where they lie under the
+// manysum.cpp, ldecimal.cpp, manygcd.cpp), compiled from where they lie under the
 // reference tree by oracle/Makefile into oracle/_ref/libwbref.so.
 // Test infrastructure only: used to pin the oracle, never by the product.
 #include "angle.h"
@@ -8,9 +8,26 @@
 #include "shape.h"
 #include "manysum.h"
 #include "ldecimal.h"
+#include "manygcd.h"
+
+#include <cstring>
+#include <vector>
 
 extern "C" {
 void ref_fill_tables() { fillTanTables(); }
+int ref_ldecimal(double x, double toler, char *out, int cap)
+{
+  std::string s = ldecimal(x, toler);
+  if ((int)s.size() >= cap) return -1;
+  std::memcpy(out, s.c_str(), s.size() + 1);
+  return (int)s.size();
+}
+double ref_manygcd(const double *v, int n, double toler)
+{
+  return manygcd(std::vector<double>(v, v + n), toler);
+}
+int ref_degtobin(double a) { return degtobin(a); }
+double ref_bintodeg(int a) { return bintodeg(a); }
 int ref_atan2i(double y, double x) { return atan2i(y, x); }
 double ref_sin_bin(int a) { return sin(a); }
 double ref_cos_bin(int a) { return cos(a); }

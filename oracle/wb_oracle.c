@@ -717,14 +717,13 @@ typedef struct {
 /* BoundRect of the header corners at orientation 0 (boundrect.cpp:61-74,
  * point.cpp dirbound) followed by Flowsnake::setSize (flowsnake.cpp:208-229)
  * with the cube built in wolkencanvas.cpp:511-519. */
-void wbo_snake_setup(const double bbmin[3], const double bbmax[3], double tile_size, wbo_snake *sn)
+/* BoundRect::include of each corner at orientation 0 (boundrect.cpp:53-70);
+ * out: left, bottom, right, top, low, high (boundrect.h:41-64) */
+void wbo_bound_rect(const double *corners, int ncorners, double out[6])
 {
   double b[6] = {INFINITY, INFINITY, INFINITY, INFINITY, INFINITY, INFINITY};
-  const double *corner[2] = {bbmin, bbmax};
-  double side, cen[3], best = INFINITY;
-  int bi = 0;
-  for (int k = 0; k < 2; k++) {
-    const double *o = corner[k];
+  for (int k = 0; k < ncorners; k++) {
+    const double *o = corners + 3 * k;
     for (int i = 0; i < 4; i++) {
       int ang = i * DEG90;
       double nb = o[0] * bcos(ang) + o[1] * bsin(ang);
@@ -733,8 +732,26 @@ void wbo_snake_setup(const double bbmin[3], const double bbmax[3], double tile_s
     if (o[2] < b[4]) b[4] = o[2];
     if (-o[2] < b[5]) b[5] = -o[2];
   }
+  out[0] = b[0]; out[1] = b[1]; out[2] = -b[2]; out[3] = -b[3]; out[4] = b[4]; out[5] = -b[5];
+}
+
+void wbo_snake_setup_corners(const double *corners, int ncorners, double tile_size,
+                             wbo_snake *sn);
+void wbo_snake_setup(const double bbmin[3], const double bbmax[3], double tile_size, wbo_snake *sn)
+{
+  const double c[6] = {bbmin[0], bbmin[1], bbmin[2], bbmax[0], bbmax[1], bbmax[2]};
+  wbo_snake_setup_corners(c, 2, tile_size, sn);
+}
+
+void wbo_snake_setup_corners(const double *corners, int ncorners, double tile_size,
+                             wbo_snake *sn)
+{
+  double br[6];
+  double side, cen[3], best = INFINITY;
+  int bi = 0;
+  wbo_bound_rect(corners, ncorners, br);
   {
-    double left = b[0], bottom = b[1], right = -b[2], top = -b[3], low = b[4], high = -b[5];
+    double left = br[0], bottom = br[1], right = br[2], top = br[3], low = br[4], high = br[5];
     side = right - left;
     if (top - bottom > side) side = top - bottom;
     if (high - low > side) side = high - low;
@@ -927,16 +944,48 @@ static int tgrid_get(const tgrid *g, int ex, int ey)
  * las.cpp:808.  cls_out[n] receives 0 (never classified), 1 or 2 per input
  * point.  tiles_out (may be NULL) receives hi-lo+1 records in counter order.
  * Returns the number of tiles. */
+static long run_impl(const int32_t *xi, const int32_t *yi, const int32_t *zi, const double *fx,
+                     const double *fy, const double *fz, long n, const double scale[3],
+                     const double offset[3], const double res[3], const double *corners,
+                     int ncorners, const wbo_params *pr, uint8_t *cls_out,
+                     wbo_tile *tiles_out, wbo_stats *st, double *x_out, double *y_out,
+                     double *z_out);
+
 long wbo_run(const int32_t *xi, const int32_t *yi, const int32_t *zi, long n,
              const double scale[3], const double offset[3],
              const double bbmin[3], const double bbmax[3],
              const wbo_params *pr, uint8_t *cls_out, wbo_tile *tiles_out,
              wbo_stats *st, double *x_out, double *y_out, double *z_out)
 {
+  const double c[6] = {bbmin[0], bbmin[1], bbmin[2], bbmax[0], bbmax[1], bbmax[2]};
+  return run_impl(xi, yi, zi, NULL, NULL, NULL, n, scale, offset, scale, c, 2, pr,
+                  cls_out, tiles_out, st, x_out, y_out, z_out);
+}
+
+/* The same pass over coordinates already decoded (LasHeader::readPoint's
+ * location, las.cpp:896) with the root cube and tiling of several header
+ * corners (wolkencli.cpp:85-90, wolkencanvas.cpp:136-146); res: the
+ * per-axis spacing of the coordinate lattices (the extent check of fast
+ * mode uses it in place of the scale). */
+long wbo_run_xyz(const double *x, const double *y, const double *z, long n, const double res[3],
+                 const double *corners, int ncorners, const wbo_params *pr, uint8_t *cls_out,
+                 wbo_tile *tiles_out, wbo_stats *st)
+{
+  return run_impl(NULL, NULL, NULL, x, y, z, n, NULL, NULL, res, corners, ncorners, pr, cls_out,
+                  tiles_out, st, NULL, NULL, NULL);
+}
+
+static long run_impl(const int32_t *xi, const int32_t *yi, const int32_t *zi, const double *fx,
+                     const double *fy, const double *fz, long n, const double scale[3],
+                     const double offset[3], const double res[3], const double *corners,
+                     int ncorners, const wbo_params *pr, uint8_t *cls_out,
+                     wbo_tile *tiles_out, wbo_stats *st, double *x_out, double *y_out,
+                     double *z_out)
+{
   double *x = x_out ? x_out : malloc(sizeof(double) * (n ? n : 1));
   double *y = y_out ? y_out : malloc(sizeof(double) * (n ? n : 1));
   double *z = z_out ? z_out : malloc(sizeof(double) * (n ? n : 1));
-  double corners[6], center[3], side;
+  double center[3], side, ext[4];
   wbo_snake sn;
   wbo_tree *t;
   long ntiles, *pbuf = NULL, pcap = 0;
@@ -950,15 +999,24 @@ long wbo_run(const int32_t *xi, const int32_t *yi, const int32_t *zi, long n,
   memset(st, 0, sizeof *st);
   if (!tables_ready) wbo_fill_tables();
   for (long i = 0; i < n; i++) {
-    x[i] = (offset[0] + scale[0] * xi[i]) * 1.0;
-    y[i] = (offset[1] + scale[1] * yi[i]) * 1.0;
-    z[i] = (offset[2] + scale[2] * zi[i]) * 1.0;
+    if (fx) {
+      x[i] = fx[i]; y[i] = fy[i]; z[i] = fz[i];
+    } else {
+      x[i] = (offset[0] + scale[0] * xi[i]) * 1.0;
+      y[i] = (offset[1] + scale[1] * yi[i]) * 1.0;
+      z[i] = (offset[2] + scale[2] * zi[i]) * 1.0;
+    }
   }
-  memcpy(corners, bbmin, sizeof(double) * 3);
-  memcpy(corners + 3, bbmax, sizeof(double) * 3);
-  wbo_sizefit(corners, 2, center, &side);
+  wbo_sizefit(corners, ncorners, center, &side);
   t = wbo_build(x, y, z, n, center, side);
-  wbo_snake_setup(bbmin, bbmax, pr->tile_size, &sn);
+  wbo_snake_setup_corners(corners, ncorners, pr->tile_size, &sn);
+  ext[0] = ext[2] = INFINITY;
+  ext[1] = ext[3] = -INFINITY;
+  for (int k = 0; k < ncorners; k++)
+    for (int a = 0; a < 2; a++) {
+      ext[2 * a] = fmin(ext[2 * a], corners[3 * k + a]);
+      ext[2 * a + 1] = fmax(ext[2 * a + 1], corners[3 * k + a]);
+    }
   ntiles = (long)sn.hi - sn.lo + 1;
   ex = malloc(sizeof(int) * ntiles);
   ey = malloc(sizeof(int) * ntiles);
@@ -1075,7 +1133,7 @@ long wbo_run(const int32_t *xi, const int32_t *yi, const int32_t *zi, long n,
   }
   /* CLASSIFY (classify.cpp:96-173), last tile wins */
   memset(cls_out, 0, n);
-  fast = (bbmax[0] - bbmin[0]) < 1e7 * scale[1] && (bbmax[1] - bbmin[1]) < 1e7 * scale[0];
+  fast = (ext[1] - ext[0]) < 1e7 * res[1] && (ext[3] - ext[2]) < 1e7 * res[0];
   for (long k = 0; k < ntiles; k++) {
     double cx, cy, r, cp[3];
     wbo_shape cyl;

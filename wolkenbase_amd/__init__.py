@@ -21,6 +21,9 @@ GPU_SO = os.path.join(HERE, "libwb_gpu.so")
 WB_SHAPE_CYLINDER, WB_SHAPE_SPHERE, WB_SHAPE_HYPERBOLOID = 0, 1, 2
 WB_SHAPE_NPARAMS = 5
 WB_DEVICE_INPUT = 1
+WB_BUILD_DEDUPE = 2
+WB_WRITE_DRY = 4
+WB_ZEROS_READ, WB_ZEROS_KEEP = 0, 1
 WB_FIT_FP64, WB_FIT_FP32, WB_FIT_FP32_COMP = 0, 1, 2
 
 
@@ -67,6 +70,36 @@ class Stats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class LasHeader(C.Structure):
+    """wb_las_header: a LAS header after LasHeader::openRead (las.cpp:272-428)."""
+    _fields_ = [("source_id", C.c_uint16), ("global_encoding", C.c_uint16),
+                ("guid", C.c_uint8 * 16), ("version_major", C.c_uint8),
+                ("version_minor", C.c_uint8), ("system_id", C.c_char * 33),
+                ("software", C.c_char * 33), ("creation_day", C.c_uint16),
+                ("creation_year", C.c_uint16), ("header_size", C.c_uint32),
+                ("point_offset", C.c_uint32), ("n_vlr", C.c_uint32),
+                ("point_format", C.c_uint16), ("point_length", C.c_uint16),
+                ("scale", C.c_double * 3), ("offset", C.c_double * 3),
+                ("min", C.c_double * 3), ("max", C.c_double * 3),
+                ("start_waveform", C.c_uint64), ("start_evlr", C.c_uint64),
+                ("n_evlr", C.c_uint32), ("n_points", C.c_uint64 * 16),
+                ("zipped", C.c_int32), ("valid", C.c_int32)]
+
+
+class LasWriteParams(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("separate_classes", C.c_int32),
+                ("points_per_file", C.c_int32), ("point_format", C.c_int32),
+                ("creation_day", C.c_int32), ("creation_year", C.c_int32)]
+
+
+class LasWriteStats(C.Structure):
+    _fields_ = [("class_totals", C.c_uint64 * 256), ("npoints", C.c_uint64),
+                ("bytes", C.c_uint64), ("nfiles", C.c_uint32), ("point_format", C.c_int32),
+                ("scale", C.c_double * 3), ("offset", C.c_double * 3),
+                ("ms_classes", C.c_double), ("ms_plan", C.c_double),
+                ("ms_encode", C.c_double), ("ms_copy", C.c_double), ("ms_disk", C.c_double)]
+
+
 def default_params(tile_size=1.0, max_slope=1.0, thickness=0.0, min_hs=0.1,
                    fit_precision=0):
     """Reference defaults (mainwindow.cpp:403-410); fit_precision WB_FIT_FP64."""
@@ -88,6 +121,9 @@ SIGNATURES = {
     "wb_size_fit": (C.c_int, [_dp, C.c_int, _dp, _dp]),
     "wb_tree_build": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint64, C.POINTER(LasXform), _dp,
                                 C.c_double, C.c_int, C.POINTER(_vp)]),
+    "wb_tree_build_xyz": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint64, _dp, _dp, C.c_double,
+                                    C.c_int, C.POINTER(_vp)]),
+    "wb_tree_duplicates": (C.c_int, [_vp, _u64p, _u32p]),
     "wb_tree_free": (C.c_int, [_vp]),
     "wb_tree_info": (C.c_int, [_vp, _u64p, _u64p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "wb_tree_leaves": (C.c_int, [_vp, C.POINTER(Leaf), C.c_uint64, _u64p]),
@@ -99,6 +135,28 @@ SIGNATURES = {
     "wb_tile_count": (C.c_int, [_dp, _dp, C.c_double, _u64p]),
     "wb_scan_tiles": (C.c_int, [_vp, _dp, _dp, C.POINTER(ClassifyParams),
                                 C.POINTER(TileRecord), C.c_uint64, C.POINTER(Stats)]),
+    "wb_classify_corners": (C.c_int, [_vp, _dp, C.c_int, C.POINTER(ClassifyParams), _vp,
+                                      C.c_int, C.POINTER(TileRecord), C.c_uint64,
+                                      C.POINTER(Stats)]),
+    "wb_tile_count_corners": (C.c_int, [_dp, C.c_int, C.c_double, _u64p]),
+    "wb_tree_dump": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.c_uint64, _u64p]),
+    "wb_ldecimal": (C.c_int, [C.c_double, C.c_double, C.c_char_p, C.c_int]),
+    "wb_las_read_header": (C.c_int, [C.c_char_p, C.POINTER(LasHeader)]),
+    "wb_las_open": (C.c_int, [_vp, C.c_char_p, C.POINTER(_vp), C.POINTER(LasHeader)]),
+    "wb_las_open_mem": (C.c_int, [_vp, _vp, C.c_uint64, C.POINTER(_vp),
+                                  C.POINTER(LasHeader)]),
+    "wb_las_close": (C.c_int, [_vp]),
+    "wb_cloud_create": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.c_double, C.c_int,
+                                  C.POINTER(_vp)]),
+    "wb_cloud_info": (C.c_int, [_vp, _u64p, _dp, C.POINTER(C.c_int), _dp]),
+    "wb_cloud_points": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "wb_cloud_read_points": (C.c_int, [_vp, _dp, _dp, _dp, _u32p]),
+    "wb_cloud_build": (C.c_int, [_vp, _dp, C.c_double, C.c_int, C.POINTER(_vp)]),
+    "wb_cloud_classify": (C.c_int, [_vp, _vp, C.POINTER(ClassifyParams), _vp, C.c_int,
+                                    C.POINTER(TileRecord), C.c_uint64, C.POINTER(Stats)]),
+    "wb_cloud_free": (C.c_int, [_vp]),
+    "wb_cloud_write": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(LasWriteParams),
+                                 C.POINTER(LasWriteStats), C.c_char_p, C.c_uint64]),
 }
 
 _lib = None
@@ -137,6 +195,7 @@ class Context:
             raise WbError(f"wb_ctx_create({device}) failed with {rc}")
         self.h = h
         self.trees = weakref.WeakSet()
+        self.las = weakref.WeakSet()  # LasFile and Cloud objects
 
     def check(self, rc, what):
         if rc != 0:
@@ -146,8 +205,12 @@ class Context:
         self.check(lib().wb_ctx_sync(self.h), "wb_ctx_sync")
 
     def close(self):
-        """Frees the trees still alive on this context, then the context."""
+        """Frees the clouds, LAS files and trees still alive on this context,
+        then the context."""
         if getattr(self, "h", None):
+            objs = list(getattr(self, "las", ()))
+            for o in sorted(objs, key=lambda o: not hasattr(o, "files")):
+                o.close()
             for t in list(getattr(self, "trees", ())):
                 t.free()
             lib().wb_ctx_destroy(self.h)
@@ -194,6 +257,17 @@ class Octree:
     """Canonical octree resident in HBM (Octree + OctStore of octree.h)."""
 
     @staticmethod
+    def size_fit_corners(corners):
+        """Octree::sizeFit over a list of corners (octree.cpp:268-310)."""
+        pts = _arr(np.asarray(corners).reshape(-1), np.float64)
+        c = np.zeros(3)
+        s = np.zeros(1)
+        rc = lib().wb_size_fit(_dptr(pts), len(pts) // 3, _dptr(c), _dptr(s))
+        if rc != 0:
+            raise WbError("wb_size_fit failed")
+        return c, float(s[0])
+
+    @staticmethod
     def size_fit(bbmin, bbmax):
         """Octree::sizeFit over the two header corners (octree.cpp:268-310)."""
         pts = _arr(np.concatenate([bbmin, bbmax]), np.float64)
@@ -205,9 +279,11 @@ class Octree:
         return c, float(s[0])
 
     @classmethod
-    def build(cls, ctx, xi, yi, zi, scale, offset, center, side, unit=1.0, sort_passes=None):
+    def build(cls, ctx, xi, yi, zi, scale, offset, center, side, unit=1.0, sort_passes=None,
+              dedupe=False):
         """Build from int32 LAS coordinates (numpy arrays, or device tensors
-        exposing data_ptr() for input already resident in HBM)."""
+        exposing data_ptr() for input already resident in HBM).  dedupe merges
+        points at one location as OctBuffer::put does (octree.cpp:620-660)."""
         self = cls.__new__(cls)
         self.ctx = ctx
         xf = LasXform((C.c_double * 3)(*scale), (C.c_double * 3)(*offset), unit)
@@ -216,22 +292,72 @@ class Octree:
         if dev:
             n = int(xi.numel())
             px, py, pz = xi.data_ptr(), yi.data_ptr(), zi.data_ptr()
-            self._keep = None
         else:
             xi, yi, zi = (_arr(a, np.int32) for a in (xi, yi, zi))
             n = len(xi)
             px, py, pz = xi.ctypes.data, yi.ctypes.data, zi.ctypes.data
-            self._keep = None
         h = _vp()
+        flags = (WB_DEVICE_INPUT if dev else 0) | (WB_BUILD_DEDUPE if dedupe else 0)
         rc = lib().wb_tree_build(ctx.h, px, py, pz, n, C.byref(xf), _dptr(cen), float(side),
-                                 WB_DEVICE_INPUT if dev else 0, C.byref(h))
+                                 flags, C.byref(h))
         ctx.check(rc, "wb_tree_build")
+        return self._adopt(ctx, h, n, cen, side)
+
+    @classmethod
+    def build_xyz(cls, ctx, x, y, z, resolution, center, side, dedupe=False):
+        """Build from coordinates already decoded to FP64 (numpy arrays or
+        device tensors).  resolution: per-axis spacing every nonzero
+        coordinate difference is at least."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        cen = _arr(center, np.float64)
+        res = _arr(resolution, np.float64)
+        dev = hasattr(x, "data_ptr")
+        if dev:
+            n = int(x.numel())
+            px, py, pz = x.data_ptr(), y.data_ptr(), z.data_ptr()
+        else:
+            x, y, z = (_arr(a, np.float64) for a in (x, y, z))
+            n = len(x)
+            px, py, pz = x.ctypes.data, y.ctypes.data, z.ctypes.data
+        h = _vp()
+        flags = (WB_DEVICE_INPUT if dev else 0) | (WB_BUILD_DEDUPE if dedupe else 0)
+        rc = lib().wb_tree_build_xyz(ctx.h, px, py, pz, n, _dptr(res), _dptr(cen), float(side),
+                                     flags, C.byref(h))
+        ctx.check(rc, "wb_tree_build_xyz")
+        return self._adopt(ctx, h, n, cen, side)
+
+    def _adopt(self, ctx, h, n, cen, side):
         self.h = h
         ctx.trees.add(self)
-        self.n = n
+        nd = C.c_uint64()
+        ctx.check(lib().wb_tree_duplicates(h, C.byref(nd), None), "wb_tree_duplicates")
+        self.n_input = n
+        self.ndup = nd.value
+        self.n = n - nd.value
         self.center = cen
         self.side = float(side)
         return self
+
+    def dump(self, path=None):
+        """OctStore::dump (octree.cpp:360-376, 673-689, 888-891) as text;
+        also written to path when given."""
+        n = C.c_uint64()
+        L = lib()
+        self.ctx.check(L.wb_tree_dump(self.h, None, None, 0, C.byref(n)), "wb_tree_dump")
+        buf = C.create_string_buffer(n.value + 1)
+        self.ctx.check(L.wb_tree_dump(self.h, path.encode() if path else None, buf, n.value + 1,
+                                      C.byref(n)), "wb_tree_dump")
+        return buf.raw[:n.value].decode("utf-8")
+
+    def duplicates(self):
+        """(merged input index, input index of the first point at its
+        location) pairs, shape (ndup, 2)."""
+        out = np.zeros((max(self.ndup, 1), 2), dtype=np.uint32)
+        nd = C.c_uint64()
+        self.ctx.check(lib().wb_tree_duplicates(self.h, C.byref(nd), out.ctypes.data_as(_u32p)),
+                       "wb_tree_duplicates")
+        return out[:self.ndup]
 
     def free(self):
         # a closed context has already freed its trees
@@ -293,6 +419,23 @@ class Octree:
         return [idx[int(offs[s]):int(offs[s + 1])].copy() for s in range(ns)]
 
 
+def ldecimal(x, toler=0.0):
+    """ldecimal (ldecimal.cpp:32-132)."""
+    buf = C.create_string_buffer(64)
+    n = lib().wb_ldecimal(float(x), float(toler), buf, 64)
+    if n < 0:
+        raise WbError("wb_ldecimal failed")
+    return buf.value.decode()
+
+
+def tile_count_corners(corners, tile_size=1.0):
+    a = _arr(np.asarray(corners).reshape(-1), np.float64)
+    n = C.c_uint64()
+    if lib().wb_tile_count_corners(_dptr(a), len(a) // 3, float(tile_size), C.byref(n)) != 0:
+        raise WbError("wb_tile_count_corners failed")
+    return n.value
+
+
 def device_memory(device=0):
     """(free, total) bytes of device memory."""
     f, t = C.c_uint64(), C.c_uint64()
@@ -323,7 +466,7 @@ def classify(tree, bbmin, bbmax, params=None, want_tiles=True, out=None):
         cls = out
         ptr, flags = out.data_ptr(), WB_DEVICE_INPUT
     else:
-        cls = np.zeros(tree.n, dtype=np.uint8)
+        cls = np.zeros(tree.n_input, dtype=np.uint8)
         ptr, flags = cls.ctypes.data, 0
     rc = lib().wb_classify(tree.h, _dptr(a), _dptr(b), C.byref(params), ptr, flags,
                            tiles if want_tiles else None, ntiles, C.byref(st))

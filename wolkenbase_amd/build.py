@@ -25,7 +25,7 @@ HIP_FLAGS = [
     "-I", CSRC, "-I", os.path.join(ROOT, "include"),
     "-Wno-unused-result", "-Wno-unused-value",
 ]
-GPU_SOURCES = ["wb_build.hip", "wb_classify.hip", "wb_api.hip"]
+GPU_SOURCES = ["wb_build.hip", "wb_classify.hip", "wb_api.hip", "wb_las.hip", "wb_dump.hip"]
 HOST_SOURCES = ["wb_host.cpp"]
 
 

@@ -103,7 +103,8 @@ void wb_ctx_destroy(wb_ctx *c)
   if (!c) return;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
-  // trees still alive go with their context
+  // clouds, LAS files and trees still alive go with their context
+  wb_release_las(c);
   for (wb_tree *t : c->trees) {
     for (auto &o : t->owned) hipFree(o.first);
     delete t;
@@ -153,10 +154,11 @@ int wb_tree_build(wb_ctx *c, const int32_t *x, const int32_t *y, const int32_t *
   if (!(root_side > 0) || !isfinite(root_side)) return fail(c, "root side must be positive", WB_EINVAL);
   if ((root_side / (1 << 21)) * (double)(1 << 21) != root_side)
     return fail(c, "root side is not representable", WB_ERANGE);
+  if (flags & ~(WB_DEVICE_INPUT | WB_BUILD_DEDUPE)) return fail(c, "unknown flags", WB_EINVAL);
   *out = nullptr;
   wb_tree *t = new wb_tree();
   t->ctx = c;
-  t->n = n;
+  t->n = t->n_input = n;
   for (int a = 0; a < 3; a++) {
     t->center[a] = root_center[a];
     t->scale[a] = xf->scale[a] * xf->unit;
@@ -187,7 +189,7 @@ int wb_tree_build(wb_ctx *c, const int32_t *x, const int32_t *y, const int32_t *
     dz = tmp + 2 * n;
   }
   try {
-    wb_build_tree(t, dx, dy, dz);
+    wb_build_tree(t, dx, dy, dz, nullptr, nullptr, nullptr, (flags & WB_BUILD_DEDUPE) != 0);
   } catch (...) {
     if (tmp) wb_free(c, tmp, 12 * n);
     for (auto &o : t->owned) wb_free(c, o.first, o.second);
@@ -198,6 +200,75 @@ int wb_tree_build(wb_ctx *c, const int32_t *x, const int32_t *y, const int32_t *
   WB_CHECK(hipStreamSynchronize(c->stream));
   c->trees.insert(t);
   *out = t;
+  return WB_OK;
+  API_CATCH(c)
+}
+
+int wb_tree_build_xyz(wb_ctx *c, const double *x, const double *y, const double *z, uint64_t n,
+                      const double resolution[3], const double root_center[3], double root_side,
+                      int flags, wb_tree **out)
+{
+  if (!c || !out || !root_center || !resolution) return fail(c, "null argument", WB_EINVAL);
+  if (n > 0xffffffffull) return fail(c, "more than 2^32-1 points", WB_ERANGE);
+  if (n > 0 && (!x || !y || !z)) return fail(c, "null coordinate array", WB_EINVAL);
+  if (!(root_side > 0) || !isfinite(root_side)) return fail(c, "root side must be positive", WB_EINVAL);
+  if ((root_side / (1 << 21)) * (double)(1 << 21) != root_side)
+    return fail(c, "root side is not representable", WB_ERANGE);
+  for (int a = 0; a < 3; a++)
+    if (!(resolution[a] > 0) || !isfinite(resolution[a]))
+      return fail(c, "resolution must be positive", WB_EINVAL);
+  if (flags & ~(WB_DEVICE_INPUT | WB_BUILD_DEDUPE)) return fail(c, "unknown flags", WB_EINVAL);
+  *out = nullptr;
+  wb_tree *t = new wb_tree();
+  t->ctx = c;
+  t->n = t->n_input = n;
+  for (int a = 0; a < 3; a++) {
+    t->center[a] = root_center[a];
+    t->scale[a] = resolution[a];  // only the extent check of the ground test reads it
+    t->offset[a] = 0;
+  }
+  t->side = root_side;
+  t->sort_passes = 0;
+  t->ms_build = t->ms_sort = 0;
+  API_TRY(c)
+  WB_CHECK(hipSetDevice(c->device));
+  const double *dx = x, *dy = y, *dz = z;
+  double *tmp = nullptr;
+  if (!(flags & WB_DEVICE_INPUT) && n > 0) {
+    tmp = (double *)wb_alloc(c, 24 * n);
+    WB_CHECK(hipMemcpyAsync(tmp, x, 8 * n, hipMemcpyHostToDevice, c->stream));
+    WB_CHECK(hipMemcpyAsync(tmp + n, y, 8 * n, hipMemcpyHostToDevice, c->stream));
+    WB_CHECK(hipMemcpyAsync(tmp + 2 * n, z, 8 * n, hipMemcpyHostToDevice, c->stream));
+    dx = tmp;
+    dy = tmp + n;
+    dz = tmp + 2 * n;
+  }
+  try {
+    wb_build_tree(t, nullptr, nullptr, nullptr, dx, dy, dz, (flags & WB_BUILD_DEDUPE) != 0);
+  } catch (...) {
+    if (tmp) wb_free(c, tmp, 24 * n);
+    for (auto &o : t->owned) wb_free(c, o.first, o.second);
+    delete t;
+    throw;
+  }
+  if (tmp) wb_free(c, tmp, 24 * n);
+  WB_CHECK(hipStreamSynchronize(c->stream));
+  c->trees.insert(t);
+  *out = t;
+  return WB_OK;
+  API_CATCH(c)
+}
+
+int wb_tree_duplicates(wb_tree *t, uint64_t *ndup, uint32_t *pairs)
+{
+  if (!t || !ndup) return WB_EINVAL;
+  wb_ctx *c = t->ctx;
+  *ndup = t->ndup;
+  if (!pairs || t->ndup == 0) return WB_OK;
+  API_TRY(c)
+  WB_CHECK(hipSetDevice(c->device));
+  WB_CHECK(hipMemcpyAsync(pairs, t->dup_pairs, 8 * t->ndup, hipMemcpyDeviceToHost, c->stream));
+  WB_CHECK(hipStreamSynchronize(c->stream));
   return WB_OK;
   API_CATCH(c)
 }
@@ -337,9 +408,17 @@ int wb_tile_count(const double bbmin[3], const double bbmax[3], double tile_size
   return WB_OK;
 }
 
+int wb_tile_count_corners(const double *corners, int ncorners, double tile_size, uint64_t *ntiles)
+{
+  if (!corners || ncorners <= 0 || !ntiles || !(tile_size > 0)) return WB_EINVAL;
+  wb::SnakeSetup s = wb::snake_setup_corners(corners, ncorners, tile_size);
+  *ntiles = (uint64_t)((int64_t)s.hi - s.lo + 1);
+  return WB_OK;
+}
+
 // wb_classify and wb_scan_tiles: the pass up to POSTSCAN, and CLASSIFY unless
 // scan_only
-static int run_pass(wb_tree *t, const double bbmin[3], const double bbmax[3],
+static int run_pass(wb_tree *t, const double *corners, int ncorners,
                     const wb_classify_params *p, uint8_t *class_out, int flags,
                     wb_tile *tiles_out, uint64_t tiles_cap, wb_stats *stats, bool scan_only)
 {
@@ -349,11 +428,17 @@ static int run_pass(wb_tree *t, const double bbmin[3], const double bbmax[3],
   if (p->fit_precision < WB_FIT_FP64 || p->fit_precision > WB_FIT_FP32_COMP)
     return fail(c, "unknown fit precision", WB_EINVAL);
   // the early-exit argument needs every direction >= 30 units below +180 deg
-  for (int a = 0; a < 2; a++)
-    if (!(bbmax[a] - bbmin[a] < 1e7 * t->scale[1 - a]))
+  for (int a = 0; a < 2; a++) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int k = 0; k < ncorners; k++) {
+      lo = fmin(lo, corners[3 * k + a]);
+      hi = fmax(hi, corners[3 * k + a]);
+    }
+    if (!(hi - lo < 1e7 * t->scale[1 - a]))
       return fail(c, "cloud extent above 1e7 coordinate units is not supported", WB_ERANGE);
+  }
   uint64_t ntiles;
-  wb_tile_count(bbmin, bbmax, p->tile_size, &ntiles);
+  wb_tile_count_corners(corners, ncorners, p->tile_size, &ntiles);
   if (ntiles > 0xffffffffull) return fail(c, "too many tiles", WB_ERANGE);
   if (tiles_out && tiles_cap < ntiles) return fail(c, "tile buffer too small", WB_EINVAL);
   API_TRY(c)
@@ -361,13 +446,14 @@ static int run_pass(wb_tree *t, const double bbmin[3], const double bbmax[3],
   auto t0 = std::chrono::steady_clock::now();
   uint8_t *dcls = class_out;
   bool host_out = !(flags & WB_DEVICE_INPUT) && !scan_only;
-  if (host_out) dcls = (uint8_t *)wb_alloc(c, t->n + 16);
+  if (host_out) dcls = (uint8_t *)wb_alloc(c, t->n_input + 16);
   ClassifyOut co;
-  wb_run_classify(t, bbmin, bbmax, p, dcls, tiles_out != nullptr, scan_only, &co);
+  wb_run_classify(t, corners, ncorners, p, dcls, tiles_out != nullptr, scan_only, &co);
   if (host_out) {
-    if (t->n) WB_CHECK(hipMemcpyAsync(class_out, dcls, t->n, hipMemcpyDeviceToHost, c->stream));
+    if (t->n_input)
+      WB_CHECK(hipMemcpyAsync(class_out, dcls, t->n_input, hipMemcpyDeviceToHost, c->stream));
     WB_CHECK(hipStreamSynchronize(c->stream));
-    wb_free(c, dcls, t->n + 16);
+    wb_free(c, dcls, t->n_input + 16);
   }
   auto t1 = std::chrono::steady_clock::now();
   if (tiles_out) {
@@ -422,7 +508,16 @@ int wb_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
                 uint64_t tiles_cap, wb_stats *stats)
 {
   if (!t || !bbmin || !bbmax || !p || !class_out) return WB_EINVAL;
-  return run_pass(t, bbmin, bbmax, p, class_out, flags, tiles_out, tiles_cap, stats, false);
+  const double corners[6] = {bbmin[0], bbmin[1], bbmin[2], bbmax[0], bbmax[1], bbmax[2]};
+  return run_pass(t, corners, 2, p, class_out, flags, tiles_out, tiles_cap, stats, false);
+}
+
+int wb_classify_corners(wb_tree *t, const double *corners, int ncorners,
+                        const wb_classify_params *p, uint8_t *class_out, int flags,
+                        wb_tile *tiles_out, uint64_t tiles_cap, wb_stats *stats)
+{
+  if (!t || !corners || ncorners <= 0 || !p || !class_out) return WB_EINVAL;
+  return run_pass(t, corners, ncorners, p, class_out, flags, tiles_out, tiles_cap, stats, false);
 }
 
 int wb_scan_tiles(wb_tree *t, const double bbmin[3], const double bbmax[3],
@@ -430,7 +525,8 @@ int wb_scan_tiles(wb_tree *t, const double bbmin[3], const double bbmax[3],
                   wb_stats *stats)
 {
   if (!t || !bbmin || !bbmax || !p || !tiles_out) return WB_EINVAL;
-  return run_pass(t, bbmin, bbmax, p, nullptr, 0, tiles_out, tiles_cap, stats, true);
+  const double corners[6] = {bbmin[0], bbmin[1], bbmin[2], bbmax[0], bbmax[1], bbmax[2]};
+  return run_pass(t, corners, 2, p, nullptr, 0, tiles_out, tiles_cap, stats, true);
 }
 
 }  // extern "C"

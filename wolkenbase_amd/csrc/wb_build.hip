@@ -275,12 +275,31 @@ __device__ inline uint64_t spread3(uint32_t v)
 
 __device__ inline double decode(int32_t v, double s, double o) { return o + s * (double)v; }
 
-__global__ void k_keygen(const int32_t *xi, const int32_t *yi, const int32_t *zi, uint64_t n,
-                         Xform f, uint64_t *key, uint32_t *val)
+// the input points: int32 LAS coordinates with the transform of Xform, or
+// coordinates already decoded to FP64
+struct Src {
+  const int32_t *xi, *yi, *zi;
+  const double *fx, *fy, *fz;
+};
+
+template <bool F64>
+__device__ inline void load_point(const Src &s, const Xform &f, uint64_t i, double &x, double &y,
+                                  double &z)
+{
+  if (F64) {
+    x = s.fx[i]; y = s.fy[i]; z = s.fz[i];
+  } else {
+    x = decode(s.xi[i], f.sx, f.ox); y = decode(s.yi[i], f.sy, f.oy); z = decode(s.zi[i], f.sz, f.oz);
+  }
+}
+
+template <bool F64>
+__global__ void k_keygen(Src src, uint64_t n, Xform f, uint64_t *key, uint32_t *val)
 {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double x = decode(xi[i], f.sx, f.ox), y = decode(yi[i], f.sy, f.oy), z = decode(zi[i], f.sz, f.oz);
+  double x, y, z;
+  load_point<F64>(src, f, i, x, y, z);
   uint32_t ix = cell_index(x, f.lox, f.cell, f.inv_cell);
   uint32_t iy = cell_index(y, f.loy, f.cell, f.inv_cell);
   uint32_t iz = cell_index(z, f.loz, f.cell, f.inv_cell);
@@ -443,16 +462,93 @@ __global__ void k_fill_pt_leaf(const uint32_t *leaf_start, uint32_t nl, uint32_t
 // ---------------------------------------------------------------------------
 // K4: SoA permute (decode in sorted order) and 32-slot bounding boxes
 // ---------------------------------------------------------------------------
-__global__ void k_permute(const int32_t *xi, const int32_t *yi, const int32_t *zi,
-                          const uint32_t *sidx, uint64_t n, Xform f, double *px, double *py,
-                          double *pz)
+template <bool F64>
+__global__ void k_permute(Src src, const uint32_t *sidx, uint64_t n, Xform f, double *px,
+                          double *py, double *pz)
 {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint32_t s = sidx[i];
-  px[i] = decode(xi[s], f.sx, f.ox);
-  py[i] = decode(yi[s], f.sy, f.oy);
-  pz[i] = decode(zi[s], f.sz, f.oz);
+  double x, y, z;
+  load_point<F64>(src, f, sidx[i], x, y, z);
+  px[i] = x;
+  py[i] = y;
+  pz[i] = z;
+}
+
+// ---------------------------------------------------------------------------
+// Duplicate locations (OctBuffer::put, octree.cpp:620-660): a point put at
+// the location of a stored point replaces that point's record in place, so
+// the store holds each location once, at the place of its first occurrence,
+// with the record of its last.  After a full-key sort (stable, so equal keys
+// are in input order) equal locations share a key and lie in one run of
+// equal keys; runs are a single slot except where points share a level-21
+// cell.  For each slot: grp = slot of the first equal location in its run,
+// last = input index of the last one.
+// ---------------------------------------------------------------------------
+template <bool F64>
+__global__ void k_dup_mark(const uint64_t *key, const uint32_t *sidx, uint64_t n, Src src, Xform f,
+                           uint32_t *grp, uint32_t *last)
+{
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint64_t k = key[s];
+  const bool left = s > 0 && key[s - 1] == k, right = s + 1 < n && key[s + 1] == k;
+  if (!left && !right) {
+    grp[s] = (uint32_t)s;
+    last[s] = sidx[s];
+    return;
+  }
+  double x, y, z;
+  load_point<F64>(src, f, sidx[s], x, y, z);
+  uint64_t a = s, b = s + 1;
+  while (a > 0 && key[a - 1] == k) a--;
+  while (b < n && key[b] == k) b++;
+  uint64_t first = s, lastj = s;
+  for (uint64_t j = a; j < s; j++) {
+    double qx, qy, qz;
+    load_point<F64>(src, f, sidx[j], qx, qy, qz);
+    if (qx == x && qy == y && qz == z) {
+      first = j;
+      break;
+    }
+  }
+  for (uint64_t j = b - 1; j > s; j--) {
+    double qx, qy, qz;
+    load_point<F64>(src, f, sidx[j], qx, qy, qz);
+    if (qx == x && qy == y && qz == z) {
+      lastj = j;
+      break;
+    }
+  }
+  grp[s] = (uint32_t)first;
+  last[s] = sidx[lastj];
+}
+
+__global__ void k_dup_flag(const uint32_t *grp, uint64_t n, uint32_t *flag)
+{
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) flag[s] = grp[s] == (uint32_t)s ? 1u : 0u;
+}
+
+// firsts move to their compact slot; the others become (input index, input
+// index of the first) pairs.  pos is the exclusive scan of the first flags.
+__global__ void k_dup_compact(const uint64_t *key, const uint32_t *sidx, const uint32_t *grp,
+                              const uint32_t *last, const uint32_t *pos, uint64_t n,
+                              uint64_t *key2, uint32_t *sidx2, uint32_t *last2, uint32_t *pairs)
+{
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t g = grp[s];
+  if (g == (uint32_t)s) {
+    const uint32_t p = pos[s];
+    key2[p] = key[s];
+    sidx2[p] = sidx[s];
+    last2[p] = last[s];
+  } else {
+    const uint64_t d = s - pos[s];
+    pairs[2 * d] = sidx[s];
+    pairs[2 * d + 1] = sidx[g];
+  }
 }
 
 __global__ void k_chunk_box(const double *px, const double *py, const double *pz, uint64_t n,
@@ -543,10 +639,18 @@ static int default_passes(double side)
   return std::min(8, std::max(2, p));
 }
 
-void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32_t *dz)
+void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32_t *dz,
+                   const double *fx, const double *fy, const double *fz, bool dedupe)
 {
   wb_ctx *c = t->ctx;
-  const uint64_t n = t->n;
+  const uint64_t n_input = t->n_input;
+  uint64_t n = n_input;
+  const bool f64 = fx != nullptr;
+  Src src = {dx, dy, dz, fx, fy, fz};
+  t->n = n;
+  t->ndup = 0;
+  t->dup_pairs = nullptr;
+  t->last_of = nullptr;
   Xform f;
   f.sx = t->scale[0]; f.sy = t->scale[1]; f.sz = t->scale[2];
   f.ox = t->offset[0]; f.oy = t->offset[1]; f.oz = t->offset[2];
@@ -560,7 +664,8 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
   t->sidx = wb_talloc<uint32_t>(t, n);
   hipEvent_t evb[4];
   for (int k = 0; k < 4; k++) WB_CHECK(hipEventCreate(&evb[k]));
-  int passes = t->sort_passes > 0 ? t->sort_passes : default_passes(t->side);
+  // merging duplicates needs equal locations adjacent: sort the full key
+  int passes = dedupe ? 8 : t->sort_passes > 0 ? t->sort_passes : default_passes(t->side);
   const uint32_t node_cap = (uint32_t)std::min<uint64_t>(n / 64 + 1024, 0x7fffffff);
   t->node_cube = wb_talloc<double>(t, 4 * (size_t)node_cap);
   t->node_child = wb_talloc<int32_t>(t, 8 * (size_t)node_cap);
@@ -574,11 +679,45 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
   for (;;) {
     if (n > 0) {
       WB_CHECK(hipEventRecord(evb[0], c->stream));
-      hipLaunchKernelGGL(k_keygen, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, dx, dy, dz,
-                         n, f, t->key, t->sidx);
+      if (f64)
+        hipLaunchKernelGGL(k_keygen<true>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, src,
+                           n, f, t->key, t->sidx);
+      else
+        hipLaunchKernelGGL(k_keygen<false>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, src,
+                           n, f, t->key, t->sidx);
       WB_CHECK(hipEventRecord(evb[1], c->stream));
       wb_radix_sort_pairs(c, t->key, t->sidx, n, passes);
       WB_CHECK(hipEventRecord(evb[2], c->stream));
+      if (dedupe && t->ndup == 0 && n == n_input) {
+        uint32_t *grp = (uint32_t *)wb_alloc(c, 4 * n), *last = (uint32_t *)wb_alloc(c, 4 * n);
+        uint32_t *pos = (uint32_t *)wb_alloc(c, 4 * n);
+        if (f64)
+          hipLaunchKernelGGL(k_dup_mark<true>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream,
+                             t->key, t->sidx, n, src, f, grp, last);
+        else
+          hipLaunchKernelGGL(k_dup_mark<false>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream,
+                             t->key, t->sidx, n, src, f, grp, last);
+        hipLaunchKernelGGL(k_dup_flag, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, grp, n,
+                           pos);
+        const uint64_t nd = wb_exclusive_scan_u32(c, pos, n);
+        if (nd < n) {
+          uint64_t *key2 = wb_talloc<uint64_t>(t, nd);
+          uint32_t *sidx2 = wb_talloc<uint32_t>(t, nd);
+          t->last_of = wb_talloc<uint32_t>(t, nd);
+          t->dup_pairs = wb_talloc<uint32_t>(t, 2 * (n - nd));
+          hipLaunchKernelGGL(k_dup_compact, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream,
+                             t->key, t->sidx, grp, last, pos, n, key2, sidx2, t->last_of,
+                             t->dup_pairs);
+          t->key = key2;
+          t->sidx = sidx2;
+          t->ndup = n - nd;
+          n = nd;
+          t->n = n;
+        }
+        wb_free(c, grp, 4 * n_input);
+        wb_free(c, last, 4 * n_input);
+        wb_free(c, pos, 4 * n_input);
+      }
     }
     // root node 0
     double root[4] = {t->center[0], t->center[1], t->center[2], t->side};
@@ -662,8 +801,12 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
   if (n > 0) {
     hipLaunchKernelGGL(k_fill_pt_leaf, dim3(nl), dim3(256), 0, c->stream, t->leaf_start, nl,
                        t->pt_leaf);
-    hipLaunchKernelGGL(k_permute, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, dx, dy, dz,
-                       t->sidx, n, f, t->px, t->py, t->pz);
+    if (f64)
+      hipLaunchKernelGGL(k_permute<true>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, src,
+                         t->sidx, n, f, t->px, t->py, t->pz);
+    else
+      hipLaunchKernelGGL(k_permute<false>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, src,
+                         t->sidx, n, f, t->px, t->py, t->pz);
     hipLaunchKernelGGL(k_chunk_box, dim3(wb_blocks(nchunks * WB_CHUNK, 256)), dim3(256), 0,
                        c->stream, t->px, t->py, t->pz, n, t->chunk_box, t->chunk_low);
     hipLaunchKernelGGL(k_leaf_canon, dim3(nl), dim3(256), 0, c->stream, t->leaf_start, nl,

@@ -1681,6 +1681,14 @@ __global__ void k_scatter_class(uint64_t n, const uint8_t *cls, const uint32_t *
   out[sidx[i]] = cls[i];
 }
 
+// an input merged into an earlier point at its location gets that point's class
+__global__ void k_dup_class(uint64_t ndup, const uint32_t *pairs, uint8_t *out)
+{
+  uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndup) return;
+  out[pairs[2 * d]] = out[pairs[2 * d + 1]];
+}
+
 // class histogram (per-block partials, summed by k_sum_hist) and, when tile
 // records are wanted, the number of ground points each deciding tile
 // classified
@@ -1738,14 +1746,14 @@ void wb_run_points(wb_tree *t, int kind, const double *dparams, uint64_t ns, con
                      dparams, ns, doffs, dout);
 }
 
-void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
+void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
                      const wb_classify_params *p, uint8_t *d_class_out, bool want_tiles,
                      bool scan_only, ClassifyOut *out)
 {
   wb_ctx *c = t->ctx;
   hipStream_t st = c->stream;
   const uint64_t n = t->n;
-  wb::SnakeSetup ss = wb::snake_setup(bbmin, bbmax, p->tile_size);
+  wb::SnakeSetup ss = wb::snake_setup_corners(corners, ncorners, p->tile_size);
   TileGeo g;
   g.scx = ss.cx; g.scy = ss.cy; g.spacing = ss.spacing; g.r = ss.radius;
   g.lo = ss.lo; g.hi = ss.hi;
@@ -1875,6 +1883,9 @@ void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
       wb_free(c, partial, 12ull * nbk);
       hipLaunchKernelGGL(k_scatter_class, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, cls,
                          t->sidx, d_class_out);
+      if (t->ndup)
+        hipLaunchKernelGGL(k_dup_class, dim3(wb_blocks(t->ndup, 256)), dim3(256), 0, st, t->ndup,
+                           t->dup_pairs, d_class_out);
     }
   }
   WB_CHECK(hipEventRecord(ev[3], st));

@@ -84,14 +84,13 @@ static const int kLoLim[12] = {0, -4, -18, -214, -900, -10504, -44118, -514714,
 static const int kHiLim[12] = {0, 2, 30, 128, 1500, 6302, 73530, 308828, 3603000,
                                15132602, 176547030, 741497528};
 
-SnakeSetup snake_setup(const double bbmin[3], const double bbmax[3], double tile_size)
+void bound_rect(const double *corners, int ncorners, double lbrthl[6])
 {
-  // BoundRect::include for both corners: bounds = left, bottom, -right,
-  // -top, low, -high with dirbound(i*90deg) = x*cos + y*sin in binary angles
+  // BoundRect::include of each corner: bounds = left, bottom, -right, -top,
+  // low, -high with dirbound(i*90deg) = x*cos + y*sin in binary angles
   double b[6] = {INFINITY, INFINITY, INFINITY, INFINITY, INFINITY, INFINITY};
-  const double *corner[2] = {bbmin, bbmax};
-  for (int k = 0; k < 2; k++) {
-    const double *o = corner[k];
+  for (int k = 0; k < ncorners; k++) {
+    const double *o = corners + 3 * k;
     for (int i = 0; i < 4; i++) {
       int ang = i * DEG90;
       double nb = o[0] * bin_cos(ang) + o[1] * bin_sin(ang);
@@ -100,7 +99,25 @@ SnakeSetup snake_setup(const double bbmin[3], const double bbmax[3], double tile
     if (o[2] < b[4]) b[4] = o[2];
     if (-o[2] < b[5]) b[5] = -o[2];
   }
-  double left = b[0], bottom = b[1], right = -b[2], top = -b[3], low = b[4], high = -b[5];
+  lbrthl[0] = b[0];
+  lbrthl[1] = b[1];
+  lbrthl[2] = -b[2];
+  lbrthl[3] = -b[3];
+  lbrthl[4] = b[4];
+  lbrthl[5] = -b[5];
+}
+
+SnakeSetup snake_setup(const double bbmin[3], const double bbmax[3], double tile_size)
+{
+  const double corners[6] = {bbmin[0], bbmin[1], bbmin[2], bbmax[0], bbmax[1], bbmax[2]};
+  return snake_setup_corners(corners, 2, tile_size);
+}
+
+SnakeSetup snake_setup_corners(const double *corners, int ncorners, double tile_size)
+{
+  double br[6];
+  bound_rect(corners, ncorners, br);
+  double left = br[0], bottom = br[1], right = br[2], top = br[3], low = br[4], high = br[5];
   double side = right - left;
   if (top - bottom > side) side = top - bottom;
   if (high - low > side) side = high - low;

@@ -27,6 +27,11 @@ struct SnakeSetup {
   int lo, hi, index;
 };
 SnakeSetup snake_setup(const double bbmin[3], const double bbmax[3], double tile_size);
+// the same over the corners of several LAS headers
+SnakeSetup snake_setup_corners(const double *corners, int ncorners, double tile_size);
+// BoundRect at orientation 0 of a list of corners (x, y, z each):
+// left, bottom, right, top, low, high (boundrect.cpp:53-70, boundrect.h:41-64)
+void bound_rect(const double *corners, int ncorners, double lbrthl[6]);
 
 // toFlowsnake(n) (flowsnake.cpp:92-136) -> Eisenstein (x, y)
 void to_flowsnake(int n, int *ex, int *ey);

@@ -44,13 +44,21 @@ struct wb_ctx {
   // size-keyed cache of device buffers
   std::multimap<size_t, void *> cache;
   size_t cached_bytes;
-  // trees built on this context and not yet freed
+  // trees, LAS files and clouds made on this context and not yet freed
   std::set<struct wb_tree *> trees;
+  std::set<struct wb_las *> las;
+  std::set<struct wb_cloud *> clouds;
 };
+
+// ---- implemented in wb_las.hip ----
+// frees the clouds and LAS files still alive on a context being destroyed
+void wb_release_las(wb_ctx *c);
 
 struct wb_tree {
   wb_ctx *ctx;
-  uint64_t n;
+  uint64_t n;        // distinct locations in the tree
+  uint64_t n_input;  // input points (n plus the duplicates merged away)
+  uint64_t ndup;     // inputs merged into an earlier point at the same location
   double center[3], side;
   double scale[3], offset[3];
   int sort_passes, maxdepth;
@@ -68,6 +76,11 @@ struct wb_tree {
   int32_t *leaf_parent;
   double *node_cube;
   int32_t *node_child, *node_parent;
+  // with duplicates merged: (merged input index, input index of the first
+  // point at its location) pairs, and per sorted slot the input index of
+  // the last point at the slot's location (whose record a writer keeps)
+  uint32_t *dup_pairs;
+  uint32_t *last_of;
   std::vector<std::pair<void *, size_t>> owned;
 };
 
@@ -111,7 +124,7 @@ void wb_run_points(wb_tree *t, int kind, const double *dparams, uint64_t ns, con
                    uint32_t *dout);
 // the classification pass (tile geometry, membership, SCAN, POSTSCAN and,
 // unless scan_only, CLASSIFY); class bytes to d_class_out in input order
-void wb_run_classify(wb_tree *t, const double bbmin[3], const double bbmax[3],
+void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
                      const wb_classify_params *p, uint8_t *d_class_out, bool want_tiles,
                      bool scan_only, ClassifyOut *out);
 
@@ -123,7 +136,11 @@ void wb_radix_sort_pairs(wb_ctx *c, uint64_t *key, uint32_t *val, uint64_t n, in
 // LSD radix sort of (u32 key, u32 val) pairs on all 32 bits
 void wb_radix_sort_u32_pairs(wb_ctx *c, uint32_t *key, uint32_t *val, uint64_t n);
 
-// builds the canonical octree of n points (decoded from int32 LAS coordinates)
-void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32_t *dz);
+// builds the canonical octree of t->n_input points, decoded from int32 LAS
+// coordinates (dx, dy, dz with t->scale / t->offset) or given in FP64 (fx,
+// fy, fz; the int32 pointers are then null); with dedupe, points at the
+// location of an earlier point are merged into it (sets t->n, t->ndup)
+void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32_t *dz,
+                   const double *fx, const double *fy, const double *fz, bool dedupe);
 
 inline unsigned wb_blocks(uint64_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
