@@ -28,6 +28,11 @@ WORKLOADS = {
                 "box buildings and cone trees (1-3 returns), int32 at 0.001 m"),
     "tile1m": ("tile1m: 200 m x 200 m, 25 pulses/m^2 (~1.2M points), seed 1000, "
                "6-sinusoid ground, 40 box buildings, 300 cone trees, int32 at 0.001 m"),
+    "las20m": ("las20m: the tile20m points as one LAS 1.4 point-format-1 file (28-byte "
+               "records) resident in HBM; a step decodes the records (READ-phase zero rule), "
+               "builds the octree with the store's duplicate merge (full-key sort), runs "
+               "SCAN/POSTSCAN/CLASSIFY and plans and encodes the class-separated output "
+               "files (10M points per file) in HBM"),
 }
 
 
@@ -74,6 +79,176 @@ def cpu_baseline(cloud, window):
                        f"{dt:.1f} s single-threaded")
 
 
+def cpu_baseline_las(cloud, window):
+    """The oracle's LAS pipeline (decode, duplicate merge, build, scan,
+    postscan, classify, class-separated encode) on the points of a window x
+    window m square at the tile centre, written as a LAS file first."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_las as OL
+    from wolkenbase_amd import synth
+    x = cloud["offset"][0] + cloud["scale"][0] * cloud["xi"].astype(np.float64)
+    y = cloud["offset"][1] + cloud["scale"][1] * cloud["yi"].astype(np.float64)
+    cx = 0.5 * (cloud["bbmin"][0] + cloud["bbmax"][0])
+    cy = 0.5 * (cloud["bbmin"][1] + cloud["bbmax"][1])
+    m = (np.abs(x - cx) < window / 2) & (np.abs(y - cy) < window / 2)
+    sub = {k: cloud[k][m] for k in ("xi", "yi", "zi", "intensity", "returns")}
+    for k in ("scale", "offset"):
+        sub[k] = cloud[k]
+    z = cloud["offset"][2] + cloud["scale"][2] * sub["zi"].astype(np.float64)
+    sub["bbmin"] = np.array([x[m].min(), y[m].min(), z.min()])
+    sub["bbmax"] = np.array([x[m].max(), y[m].max(), z.max()])
+    img = synth.las_image(sub, 1)
+    n = int(m.sum())
+    t0 = time.perf_counter()
+    OL.pipeline([img])
+    dt = time.perf_counter() - t0
+    return dict(value=n / dt, unit="points/s", cores=1, kind="port",
+                sample=f"oracle LAS pipeline (record decode, duplicate merge, build, scan, "
+                       f"postscan, classify, class-separated encode) over the {n} points of a "
+                       f"{window:g} m x {window:g} m window at the tile centre, {dt:.1f} s "
+                       f"single-threaded")
+
+
+def main_las(args):
+    """The las20m workload: the LAS path around the classification."""
+    import numpy as np
+    import torch
+    import wolkenbase_amd as W
+    from wolkenbase_amd import las as WL
+    from wolkenbase_amd import synth
+
+    t0 = time.perf_counter()
+    cloud = synth.preset("tile20m")
+    img = synth.las_image(cloud, 1)
+    log(f"[bench] las20m image: {len(img) / 2**20:.0f} MiB in {time.perf_counter() - t0:.1f} s")
+    dev = torch.device("cuda:0")
+    torch.cuda.synchronize()
+    ctx = W.Context(0)
+    las = WL.LasFile(ctx, image=img)
+    n = las.number_points()
+    rlen = las.header["point_length"]
+    cls = torch.empty(n, dtype=torch.uint8, device=dev)
+
+    def step(dry=True, name="/tmp/wb_las20m"):
+        cl = WL.Cloud(ctx, [las], zeros="read")
+        tree = cl.build()
+        _, _, st = cl.classify(tree, want_tiles=False, out=cls)
+        ws, names = cl.write(tree, cls, name, separate_classes=True, points_per_file=10000000,
+                             dry=dry)
+        d = dict(ms_decode=cl.ms_decode, ms_build=st["ms_build"], ms_sort=st["ms_sort"],
+                 ms_tiles=st["ms_tiles"], ms_scan=st["ms_scan"], ms_classify=st["ms_classify"],
+                 ms_classify_kernel=st["ms_classify_kernel"],
+                 ms_heavy_kernel=st["ms_heavy_kernel"], ms_out_classes=ws["ms_classes"],
+                 ms_plan=ws["ms_plan"], ms_encode=ws["ms_encode"], ms_copy=ws["ms_copy"],
+                 ms_disk=ws["ms_disk"], ndup=tree.ndup, nfiles=ws["nfiles"],
+                 bytes_out=ws["bytes"], classes=ws["class_totals"], names=names)
+        tree.free()
+        cl.close()
+        return d
+
+    for w in range(args.warmup):
+        d = step()
+        log(f"[bench] warmup {w + 1}/{args.warmup}: decode {d['ms_decode']:.2f} ms, "
+            f"encode {d['ms_encode']:.2f} ms")
+    torch.cuda.synchronize()
+    lat, stats = [], []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        stats.append(step())
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+        log(f"[bench] step {k + 1}/{args.steps}: {lat[-1]:.2f} ms")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    value = n * args.steps / elapsed
+
+    def avg(f):
+        return sum(s[f] for s in stats) / len(stats)
+
+    # outside the timed region: the file read from disk (disk + PCIe), and the
+    # class-separated files copied back and written (PCIe + disk)
+    path = "/tmp/wb_las20m_in.las"
+    with open(path, "wb") as f:
+        f.write(img)
+    t1 = time.perf_counter()
+    las_disk = WL.LasFile(ctx, path=path)
+    ms_open_disk = (time.perf_counter() - t1) * 1e3
+    las_disk.close()
+    os.remove(path)
+    t1 = time.perf_counter()
+    las_mem = WL.LasFile(ctx, image=img)
+    ms_open_mem = (time.perf_counter() - t1) * 1e3
+    las_mem.close()
+    full = step(dry=False)
+    for nm in full["names"]:
+        os.remove(nm)
+    st = stats[-1]
+    kern = {"k_ground": avg("ms_classify_kernel"), "k_ground_heavy": avg("ms_heavy_kernel"),
+            "radix_sort": avg("ms_sort"), "k_las_decode": avg("ms_decode"),
+            "k_las_encode": avg("ms_encode")}
+    dominant = max(kern, key=kern.get)
+    # algorithmic bytes: decode reads each record and writes x, y, z and the
+    # record index; encode reads the stored point's record and class and
+    # writes its output record; the ground test 13 B per point; the sort 24 B
+    # per point and pass plus the 8-B key read of each histogram (8 passes)
+    nd = n - st["ndup"]
+    alg = {"k_ground": 13.0 * nd, "k_ground_heavy": 13.0 * nd, "radix_sort": 32.0 * 8 * n,
+           "k_las_decode": (rlen + 28.0) * n, "k_las_encode": (rlen + 1.0 + 28.0) * nd}
+    achieved = alg[dominant] / (kern[dominant] * 1e-3) / 1e9
+
+    def kgbs(k):
+        return dict(ms=round(kern[k], 3), algorithmic_bytes=alg[k],
+                    gbs=round(alg[k] / (kern[k] * 1e-3) / 1e9, 1),
+                    frac=round(alg[k] / (kern[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS["las20m"], "n_points": n, "record_bytes": rlen,
+                   "input": "LAS records resident in HBM; output file images left in HBM"},
+        "p50_ms": round(percentile(lat, 50), 3),
+        "p99_ms": round(percentile(lat, 99), 3),
+        "roofline": dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                         frac=round(achieved / HBM_PEAK_GBS, 6), traffic=None, kernel=dominant,
+                         kernel_ms=round(kern[dominant], 3),
+                         algorithmic_bytes_per_launch=alg[dominant]),
+        "las_kernels": {"k_las_decode": kgbs("k_las_decode"),
+                        "k_las_encode": kgbs("k_las_encode")},
+        "breakdown_ms": {k: round(avg(k), 3) for k in
+                         ("ms_decode", "ms_build", "ms_sort", "ms_tiles", "ms_scan",
+                          "ms_classify", "ms_classify_kernel", "ms_heavy_kernel",
+                          "ms_out_classes", "ms_plan", "ms_encode")},
+        "io_outside_timed_ms": {"open_from_disk": round(ms_open_disk, 1),
+                                "open_from_host_memory": round(ms_open_mem, 1),
+                                "write_copy_d2h": round(full["ms_copy"], 1),
+                                "write_disk": round(full["ms_disk"], 1),
+                                "bytes_in": len(img), "bytes_out": full["bytes_out"]},
+        "duplicates": st["ndup"],
+        "files": st["nfiles"],
+        "classes": {str(k): v for k, v in st["classes"].items()},
+    }
+    if not args.no_cpu_baseline:
+        log(f"[bench] cpu baseline on a {args.cpu_window:g} m window ...")
+        out["cpu_baseline"] = cpu_baseline_las(cloud, args.cpu_window)
+    else:
+        out["cpu_baseline"] = None
+    las.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +266,8 @@ def main():
             print("bench.py runs on one GPU only: the classification of one tile is not "
                   "sharded across GPUs in this build (see DESIGN.md, 'Sharding').", flush=True)
         sys.exit(2)
+    if args.workload == "las20m":
+        return main_las(args)
 
     import numpy as np
     import torch

@@ -264,6 +264,8 @@ int wb_cloud_create(wb_ctx *ctx, wb_las *const *files, int nfiles, double unit, 
  * may be NULL) */
 int wb_cloud_info(wb_cloud *cloud, uint64_t *npoints, double resolution[3], int *join_format,
                   double *corners);
+/* device time (ms) of the cloud's record decode (keep flags, scan, decode) */
+int wb_cloud_decode_ms(wb_cloud *cloud, double *ms);
 /* device pointers of the decoded coordinates */
 int wb_cloud_points(wb_cloud *cloud, const double **x, const double **y, const double **z);
 /* host copies of the decoded coordinates and of each point's record index
@@ -286,7 +288,8 @@ int wb_cloud_free(wb_cloud *cloud);
  *     at its location.  Files are named
  *       name-<class>-<k>.las (separate_classes) or name-<k>.las,
  *     <k> only when points_per_file is nonzero. */
-enum { WB_WRITE_DRY = 4 };  /* flag: encode and plan, write no files */
+enum { WB_WRITE_DRY = 4 };  /* flag: plan and encode the files in HBM, copy and write
+                               nothing (names and statistics are still returned) */
 typedef struct {
   const char *name;
   int32_t separate_classes;

@@ -151,6 +151,7 @@ SIGNATURES = {
     "wb_cloud_info": (C.c_int, [_vp, _u64p, _dp, C.POINTER(C.c_int), _dp]),
     "wb_cloud_points": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "wb_cloud_read_points": (C.c_int, [_vp, _dp, _dp, _dp, _u32p]),
+    "wb_cloud_decode_ms": (C.c_int, [_vp, _dp]),
     "wb_cloud_build": (C.c_int, [_vp, _dp, C.c_double, C.c_int, C.POINTER(_vp)]),
     "wb_cloud_classify": (C.c_int, [_vp, _vp, C.POINTER(ClassifyParams), _vp, C.c_int,
                                     C.POINTER(TileRecord), C.c_uint64, C.POINTER(Stats)]),

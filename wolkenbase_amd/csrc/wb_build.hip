@@ -479,20 +479,22 @@ __global__ void k_permute(Src src, const uint32_t *sidx, uint64_t n, Xform f, do
 // Duplicate locations (OctBuffer::put, octree.cpp:620-660): a point put at
 // the location of a stored point replaces that point's record in place, so
 // the store holds each location once, at the place of its first occurrence,
-// with the record of its last.  After a full-key sort (stable, so equal keys
-// are in input order) equal locations share a key and lie in one run of
-// equal keys; runs are a single slot except where points share a level-21
-// cell.  For each slot: grp = slot of the first equal location in its run,
-// last = input index of the last one.
+// with the record of its last.  Equal locations have equal keys, so after
+// the (stable, LSD) sort on the key bits above `shift` they lie in one run
+// of equal sorted bits, in input order; such runs hold the points of one
+// cell of the deepest sorted level and are short.  For each slot: grp =
+// slot of the first equal location in its run, last = input index of the
+// last one.
 // ---------------------------------------------------------------------------
 template <bool F64>
-__global__ void k_dup_mark(const uint64_t *key, const uint32_t *sidx, uint64_t n, Src src, Xform f,
-                           uint32_t *grp, uint32_t *last)
+__global__ void k_dup_mark(const uint64_t *key, const uint32_t *sidx, uint64_t n, int shift,
+                           Src src, Xform f, uint32_t *grp, uint32_t *last)
 {
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const uint64_t k = key[s];
-  const bool left = s > 0 && key[s - 1] == k, right = s + 1 < n && key[s + 1] == k;
+  const uint64_t k = key[s] >> shift;
+  const bool left = s > 0 && (key[s - 1] >> shift) == k;
+  const bool right = s + 1 < n && (key[s + 1] >> shift) == k;
   if (!left && !right) {
     grp[s] = (uint32_t)s;
     last[s] = sidx[s];
@@ -501,8 +503,8 @@ __global__ void k_dup_mark(const uint64_t *key, const uint32_t *sidx, uint64_t n
   double x, y, z;
   load_point<F64>(src, f, sidx[s], x, y, z);
   uint64_t a = s, b = s + 1;
-  while (a > 0 && key[a - 1] == k) a--;
-  while (b < n && key[b] == k) b++;
+  while (a > 0 && (key[a - 1] >> shift) == k) a--;
+  while (b < n && (key[b] >> shift) == k) b++;
   uint64_t first = s, lastj = s;
   for (uint64_t j = a; j < s; j++) {
     double qx, qy, qz;
@@ -664,8 +666,7 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
   t->sidx = wb_talloc<uint32_t>(t, n);
   hipEvent_t evb[4];
   for (int k = 0; k < 4; k++) WB_CHECK(hipEventCreate(&evb[k]));
-  // merging duplicates needs equal locations adjacent: sort the full key
-  int passes = dedupe ? 8 : t->sort_passes > 0 ? t->sort_passes : default_passes(t->side);
+  int passes = t->sort_passes > 0 ? t->sort_passes : default_passes(t->side);
   const uint32_t node_cap = (uint32_t)std::min<uint64_t>(n / 64 + 1024, 0x7fffffff);
   t->node_cube = wb_talloc<double>(t, 4 * (size_t)node_cap);
   t->node_child = wb_talloc<int32_t>(t, 8 * (size_t)node_cap);
@@ -676,7 +677,19 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
   SplitOut h_out;
   Active *act = (Active *)wb_alloc(c, sizeof(Active) * node_cap);
   Active *nxt = (Active *)wb_alloc(c, sizeof(Active) * node_cap);
+  // the arrays of all input points; with duplicates merged, the tree's key
+  // and slot arrays are compact copies
+  uint64_t *key0 = t->key, *key2 = nullptr;
+  uint32_t *sidx0 = t->sidx, *sidx2 = nullptr, *last2 = nullptr, *pairs2 = nullptr;
   for (;;) {
+    // every attempt starts from all input points
+    n = n_input;
+    t->n = n;
+    t->key = key0;
+    t->sidx = sidx0;
+    t->ndup = 0;
+    t->last_of = nullptr;
+    t->dup_pairs = nullptr;
     if (n > 0) {
       WB_CHECK(hipEventRecord(evb[0], c->stream));
       if (f64)
@@ -688,28 +701,32 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
       WB_CHECK(hipEventRecord(evb[1], c->stream));
       wb_radix_sort_pairs(c, t->key, t->sidx, n, passes);
       WB_CHECK(hipEventRecord(evb[2], c->stream));
-      if (dedupe && t->ndup == 0 && n == n_input) {
+      if (dedupe) {
+        const int shift = 64 - 8 * passes;
         uint32_t *grp = (uint32_t *)wb_alloc(c, 4 * n), *last = (uint32_t *)wb_alloc(c, 4 * n);
         uint32_t *pos = (uint32_t *)wb_alloc(c, 4 * n);
         if (f64)
           hipLaunchKernelGGL(k_dup_mark<true>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream,
-                             t->key, t->sidx, n, src, f, grp, last);
+                             t->key, t->sidx, n, shift, src, f, grp, last);
         else
           hipLaunchKernelGGL(k_dup_mark<false>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream,
-                             t->key, t->sidx, n, src, f, grp, last);
+                             t->key, t->sidx, n, shift, src, f, grp, last);
         hipLaunchKernelGGL(k_dup_flag, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, grp, n,
                            pos);
         const uint64_t nd = wb_exclusive_scan_u32(c, pos, n);
         if (nd < n) {
-          uint64_t *key2 = wb_talloc<uint64_t>(t, nd);
-          uint32_t *sidx2 = wb_talloc<uint32_t>(t, nd);
-          t->last_of = wb_talloc<uint32_t>(t, nd);
-          t->dup_pairs = wb_talloc<uint32_t>(t, 2 * (n - nd));
+          if (!key2) {
+            key2 = wb_talloc<uint64_t>(t, n_input);
+            sidx2 = wb_talloc<uint32_t>(t, n_input);
+            last2 = wb_talloc<uint32_t>(t, n_input);
+            pairs2 = wb_talloc<uint32_t>(t, 2 * n_input);
+          }
           hipLaunchKernelGGL(k_dup_compact, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream,
-                             t->key, t->sidx, grp, last, pos, n, key2, sidx2, t->last_of,
-                             t->dup_pairs);
+                             t->key, t->sidx, grp, last, pos, n, key2, sidx2, last2, pairs2);
           t->key = key2;
           t->sidx = sidx2;
+          t->last_of = last2;
+          t->dup_pairs = pairs2;
           t->ndup = n - nd;
           n = nd;
           t->n = n;

@@ -71,6 +71,7 @@ struct wb_cloud {
   std::vector<double> corners;  // min and max corner (times unit) of each file
   double resolution[3];
   int join_format;
+  double ms_decode;  // device time of the keep flags and the decode
 };
 
 static int fail(wb_ctx *c, const std::string &msg, int code)
@@ -318,6 +319,11 @@ struct OutXf {
   double scale[3], offset[3], unit;
 };
 
+// Counters that many waves update are kept in WB_NCOPY copies, each block
+// using copy blockIdx % WB_NCOPY, and summed on the host: one address per
+// counter would serialise every wave's atomic in one L2 channel.
+#define WB_NCOPY 64
+
 // per output file: point counts by return, and the written coordinates'
 // extremes as order-preserving integer keys
 struct FileStats {
@@ -373,7 +379,8 @@ __global__ void k_out_class(uint64_t n, const uint32_t *canon, const uint32_t *s
     const int leader = __ffsll((long long)rem) - 1;
     const int cv = __shfl(v, leader, 64);
     const uint64_t m = __ballot(on && v == cv);
-    if ((int)__lane_id() == leader) atomicAdd(&totals[cv], (unsigned long long)__popcll(m));
+    if ((int)__lane_id() == leader)
+      atomicAdd(&totals[256 * (blockIdx.x % WB_NCOPY) + cv], (unsigned long long)__popcll(m));
     rem &= ~m;
   }
 }
@@ -382,13 +389,15 @@ __global__ void k_out_class(uint64_t n, const uint32_t *canon, const uint32_t *s
 __global__ void k_leaf_class_count(const uint32_t *leaf_start, uint32_t nl, const uint8_t *ocls,
                                    const int16_t *cid, int K, uint32_t *counts)
 {
+  __shared__ uint32_t c[256];
   const uint32_t l = blockIdx.x;
   if (l >= nl) return;
   const uint32_t b = leaf_start[l], e = leaf_start[l + 1];
-  for (int k = threadIdx.x; k < K; k += blockDim.x) counts[(uint64_t)l * K + k] = 0;
+  for (int k = threadIdx.x; k < K; k += blockDim.x) c[k] = 0;
   __syncthreads();
-  for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x)
-    atomicAdd(&counts[(uint64_t)l * K + cid[ocls[i]]], 1u);
+  for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) atomicAdd(&c[cid[ocls[i]]], 1u);
+  __syncthreads();
+  for (int k = threadIdx.x; k < K; k += blockDim.x) counts[(uint64_t)l * K + k] = c[k];
 }
 
 template <int OFF, int NB>
@@ -533,11 +542,12 @@ __global__ void __launch_bounds__(64) k_las_encode(
       const int gv = __shfl(fg, leader, 64);
       const uint64_t m = __ballot(on && fg == gv);
       const bool mine = on && fg == gv;
+      FileStats &fsc = fst[(uint64_t)gv * WB_NCOPY + blockIdx.x % WB_NCOPY];
       for (int r = 0; r < 16; r++) {
         const uint64_t mr = __ballot(mine && ret == r);
-        if (mr && r > 0 && lane == leader) atomicAdd(&fst[gv].npoints[r], (unsigned long long)__popcll(mr));
+        if (mr && r > 0 && lane == leader) atomicAdd(&fsc.npoints[r], (unsigned long long)__popcll(mr));
       }
-      if (lane == leader) atomicAdd(&fst[gv].npoints[0], (unsigned long long)__popcll(m));
+      if (lane == leader) atomicAdd(&fsc.npoints[0], (unsigned long long)__popcll(m));
       unsigned long long kmn[3], kmx[3];
       kmn[0] = mine ? order_key(wx) : ~0ull;
       kmn[1] = mine ? order_key(wy) : ~0ull;
@@ -551,8 +561,8 @@ __global__ void __launch_bounds__(64) k_las_encode(
         }
       if (lane == leader)
         for (int a = 0; a < 3; a++) {
-          atomicMin(&fst[gv].mn[a], kmn[a]);
-          atomicMax(&fst[gv].mx[a], kmx[a]);
+          atomicMin(&fsc.mn[a], kmn[a]);
+          atomicMax(&fsc.mx[a], kmx[a]);
         }
       rem &= ~m;
     }
@@ -858,6 +868,9 @@ int wb_cloud_create(wb_ctx *c, wb_las *const *files, int nfiles, double unit, in
   std::vector<uint32_t *> pos(nfiles, nullptr);
   uint64_t total = 0;
   std::vector<int> formats;
+  hipEvent_t ev[2];
+  for (int k = 0; k < 2; k++) WB_CHECK(hipEventCreate(&ev[k]));
+  WB_CHECK(hipEventRecord(ev[0], c->stream));
   for (int f = 0; f < nfiles; f++) {
     const wb_las *L = files[f];
     LasSrc s;
@@ -908,10 +921,15 @@ int wb_cloud_create(wb_ctx *c, wb_las *const *files, int nfiles, double unit, in
       hipLaunchKernelGGL(k_las_decode, dim3(wb_blocks(L->n, 256)), dim3(256), 0, c->stream,
                          cl->src[f], L->n, unit, pos[f], cl->x, cl->y, cl->z, cl->rec);
   }
+  WB_CHECK(hipEventRecord(ev[1], c->stream));
   cl->d_src = (LasSrc *)wb_alloc(c, sizeof(LasSrc) * nfiles);
   WB_CHECK(hipMemcpyAsync(cl->d_src, cl->src.data(), sizeof(LasSrc) * nfiles,
                           hipMemcpyHostToDevice, c->stream));
   WB_CHECK(hipStreamSynchronize(c->stream));
+  float ems = 0;
+  hipEventElapsedTime(&ems, ev[0], ev[1]);
+  cl->ms_decode = ems;
+  for (int k = 0; k < 2; k++) hipEventDestroy(ev[k]);
   for (int f = 0; f < nfiles; f++)
     if (pos[f]) wb_free(c, pos[f], 4 * files[f]->n);
   // the lattice spacing of the decoded coordinates: combineScales of the
@@ -929,6 +947,13 @@ int wb_cloud_create(wb_ctx *c, wb_las *const *files, int nfiles, double unit, in
   *out = cl;
   return WB_OK;
   API_CATCH(c)
+}
+
+int wb_cloud_decode_ms(wb_cloud *cl, double *ms)
+{
+  if (!cl || !ms) return WB_EINVAL;
+  *ms = cl->ms_decode;
+  return WB_OK;
 }
 
 int wb_cloud_info(wb_cloud *cl, uint64_t *npoints, double resolution[3], int *join_format,
@@ -1040,16 +1065,19 @@ int wb_cloud_write(wb_cloud *cl, wb_tree *t, const uint8_t *cls, int flags,
   for (int k = 0; k < 4; k++) WB_CHECK(hipEventCreate(&ev[k]));
   // 1. class of every stored point, class totals (countClasses)
   uint8_t *ocls = (uint8_t *)wb_alloc(c, n + 16);
-  unsigned long long *dtot = (unsigned long long *)wb_alloc(c, 256 * 8);
-  WB_CHECK(hipMemsetAsync(dtot, 0, 256 * 8, c->stream));
+  unsigned long long *dtot = (unsigned long long *)wb_alloc(c, 256 * 8 * WB_NCOPY);
+  WB_CHECK(hipMemsetAsync(dtot, 0, 256 * 8 * WB_NCOPY, c->stream));
   WB_CHECK(hipEventRecord(ev[0], c->stream));
   if (n)
     hipLaunchKernelGGL(k_out_class, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, n, t->canon,
                        t->sidx, t->last_of, dcls, cl->d_src, nfin, cl->rec, cl->unit, ocls, dtot);
   WB_CHECK(hipEventRecord(ev[1], c->stream));
-  unsigned long long tot[256];
-  WB_CHECK(hipMemcpyAsync(tot, dtot, sizeof tot, hipMemcpyDeviceToHost, c->stream));
+  std::vector<unsigned long long> totc(256 * WB_NCOPY);
+  WB_CHECK(hipMemcpyAsync(totc.data(), dtot, 8 * totc.size(), hipMemcpyDeviceToHost, c->stream));
   WB_CHECK(hipStreamSynchronize(c->stream));
+  unsigned long long tot[256] = {};
+  for (int q = 0; q < WB_NCOPY; q++)
+    for (int k = 0; k < 256; k++) tot[k] += totc[256 * q + k];
   auto t_plan = now();
   for (int k = 0; k < 256; k++) S.class_totals[k] = tot[k];
   // 2. the file plan (CloudOutput::openFiles, cloudoutput.cpp:119-178)
@@ -1149,8 +1177,9 @@ int wb_cloud_write(wb_cloud *cl, wb_tree *t, const uint8_t *cls, int flags,
   uint64_t *dbase = (uint64_t *)wb_alloc(c, 8 * LK + 16);
   int32_t *dfile = (int32_t *)wb_alloc(c, 4 * LK + 16);
   uint64_t *doff = (uint64_t *)wb_alloc(c, 8 * (nout + 1));
-  FileStats *dfs = (FileStats *)wb_alloc(c, sizeof(FileStats) * (nout + 1));
-  std::vector<FileStats> fs(nout + 1);
+  const size_t nfs = (size_t)(nout + 1) * WB_NCOPY;
+  FileStats *dfs = (FileStats *)wb_alloc(c, sizeof(FileStats) * nfs);
+  std::vector<FileStats> fs(nfs);
   for (auto &f : fs) {
     memset(&f, 0, sizeof f);
     for (int a = 0; a < 3; a++) f.mn[a] = ~0ull;
@@ -1162,7 +1191,7 @@ int wb_cloud_write(wb_cloud *cl, wb_tree *t, const uint8_t *cls, int flags,
     WB_CHECK(hipMemcpyAsync(dfile, fileof.data(), 4 * LK, hipMemcpyHostToDevice, c->stream));
   }
   WB_CHECK(hipMemcpyAsync(doff, data_off.data(), 8 * (nout + 1), hipMemcpyHostToDevice, c->stream));
-  WB_CHECK(hipMemcpyAsync(dfs, fs.data(), sizeof(FileStats) * (nout + 1), hipMemcpyHostToDevice,
+  WB_CHECK(hipMemcpyAsync(dfs, fs.data(), sizeof(FileStats) * nfs, hipMemcpyHostToDevice,
                           c->stream));
   S.ms_plan = ms(t_plan, now());
   // 3. encode every stored record into its file image
@@ -1181,9 +1210,21 @@ int wb_cloud_write(wb_cloud *cl, wb_tree *t, const uint8_t *cls, int flags,
 #undef WB_ENC
   }
   WB_CHECK(hipEventRecord(ev[3], c->stream));
-  WB_CHECK(hipMemcpyAsync(fs.data(), dfs, sizeof(FileStats) * (nout + 1), hipMemcpyDeviceToHost,
+  WB_CHECK(hipMemcpyAsync(fs.data(), dfs, sizeof(FileStats) * nfs, hipMemcpyDeviceToHost,
                           c->stream));
   WB_CHECK(hipStreamSynchronize(c->stream));
+  // fold the copies into copy 0 of each file
+  for (int g = 0; g < nout; g++) {
+    FileStats &f0 = fs[(size_t)g * WB_NCOPY];
+    for (int q = 1; q < WB_NCOPY; q++) {
+      const FileStats &fq = fs[(size_t)g * WB_NCOPY + q];
+      for (int i = 0; i < 16; i++) f0.npoints[i] += fq.npoints[i];
+      for (int a = 0; a < 3; a++) {
+        f0.mn[a] = std::min(f0.mn[a], fq.mn[a]);
+        f0.mx[a] = std::max(f0.mx[a], fq.mx[a]);
+      }
+    }
+  }
   float e0 = 0, e1 = 0;
   hipEventElapsedTime(&e0, ev[0], ev[1]);
   hipEventElapsedTime(&e1, ev[2], ev[3]);
@@ -1201,7 +1242,7 @@ int wb_cloud_write(wb_cloud *cl, wb_tree *t, const uint8_t *cls, int flags,
   const bool dry = (flags & WB_WRITE_DRY) != 0;
   size_t pin_bytes = 0;
   for (int g = 0; g < nout; g++) pin_bytes = std::max<size_t>(pin_bytes, npts[g] * LEN);
-  if (pin_bytes) WB_CHECK(hipHostMalloc((void **)&pin, pin_bytes, hipHostMallocDefault));
+  if (pin_bytes && !dry) WB_CHECK(hipHostMalloc((void **)&pin, pin_bytes, hipHostMallocDefault));
   for (int id = 0; id < K; id++)
     for (int j = 0; j < nfiles[id]; j++) {
       const int g = first_file[id] + j;
@@ -1222,7 +1263,7 @@ int wb_cloud_write(wb_cloud *cl, wb_tree *t, const uint8_t *cls, int flags,
       put_le(hd, 96, WB_LAS_HEADER, 4);
       hd[104] = (uint8_t)fo;
       put_le(hd, 105, (uint16_t)LEN, 2);
-      const FileStats &f = fs[g];
+      const FileStats &f = fs[(size_t)g * WB_NCOPY];
       bool legacy = true;
       for (int i = 0; i < 6; i++)
         if (f.npoints[i] > 4294967295ull) legacy = false;
@@ -1239,7 +1280,7 @@ int wb_cloud_write(wb_cloud *cl, wb_tree *t, const uint8_t *cls, int flags,
       for (int i = 0; i < 16; i++) put_le(hd, 247 + 8 * i, f.npoints[i], 8);
       const size_t nb = (size_t)(npts[g] * LEN);
       auto tc = now();
-      if (nb) WB_CHECK(hipMemcpy(pin, dimg + data_off[g], nb, hipMemcpyDeviceToHost));
+      if (nb && !dry) WB_CHECK(hipMemcpy(pin, dimg + data_off[g], nb, hipMemcpyDeviceToHost));
       auto td = now();
       S.ms_copy += ms(tc, td);
       if (!dry) {
@@ -1256,13 +1297,13 @@ int wb_cloud_write(wb_cloud *cl, wb_tree *t, const uint8_t *cls, int flags,
   S.point_format = fo;
   if (pin) hipHostFree(pin);
   wb_free(c, ocls, n + 16);
-  wb_free(c, dtot, 256 * 8);
+  wb_free(c, dtot, 256 * 8 * WB_NCOPY);
   wb_free(c, dcid, sizeof cidh);
   wb_free(c, dcnt, 4 * LK + 16);
   wb_free(c, dbase, 8 * LK + 16);
   wb_free(c, dfile, 4 * LK + 16);
   wb_free(c, doff, 8 * (nout + 1));
-  wb_free(c, dfs, sizeof(FileStats) * (nout + 1));
+  wb_free(c, dfs, sizeof(FileStats) * nfs);
   wb_free(c, dimg, img_bytes + 16);
   if (dcls_own) wb_free(c, dcls_own, cl->n + 16);
   for (int k = 0; k < 4; k++) hipEventDestroy(ev[k]);

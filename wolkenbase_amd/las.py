@@ -116,6 +116,9 @@ class Cloud:
         corners = (C.c_double * (6 * len(self.files)))()
         lib().wb_cloud_info(p, C.byref(n), res, C.byref(jf), corners)
         self.n = n.value
+        ms = C.c_double()
+        lib().wb_cloud_decode_ms(p, C.byref(ms))
+        self.ms_decode = ms.value
         self.resolution = np.array(list(res))
         self.join_format = jf.value
         self.corners = np.array(list(corners)).reshape(-1, 3)
@@ -147,16 +150,21 @@ class Cloud:
         t.ctx = self.ctx
         return t._adopt(self.ctx, h, self.n, cen, side)
 
-    def classify(self, tree, params=None, want_tiles=True):
+    def classify(self, tree, params=None, want_tiles=True, out=None):
         """SCAN + POSTSCAN + CLASSIFY with the tiling of every file's header
-        corners.  Returns (class bytes in cloud order, tiles or None, stats)."""
+        corners.  Returns (class bytes in cloud order, tiles or None, stats);
+        with `out` a device tensor (uint8, n) the classes stay in HBM there."""
         from . import tile_count_corners
         params = params or default_params()
         ntiles = tile_count_corners(self.corners, params.tile_size) if want_tiles else 0
         tiles = (TileRecord * max(ntiles, 1))() if want_tiles else None
         st = Stats()
-        cls = np.zeros(self.n, dtype=np.uint8)
-        rc = lib().wb_cloud_classify(self.h, tree.h, C.byref(params), cls.ctypes.data, 0,
+        if out is not None:
+            cls, ptr, flags = out, out.data_ptr(), WB_DEVICE_INPUT
+        else:
+            cls = np.zeros(self.n, dtype=np.uint8)
+            ptr, flags = cls.ctypes.data, 0
+        rc = lib().wb_cloud_classify(self.h, tree.h, C.byref(params), ptr, flags,
                                      tiles if want_tiles else None, ntiles, C.byref(st))
         self.ctx.check(rc, "wb_cloud_classify")
         tarr = np.ctypeslib.as_array(tiles)[:ntiles].copy() if want_tiles else None

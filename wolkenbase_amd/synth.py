@@ -2,6 +2,7 @@
 libwb_synth.so, built from csrc/wb_synth.c)."""
 import ctypes as C
 import math
+import struct
 import os
 
 import numpy as np
@@ -93,3 +94,43 @@ def serving_tile(k, rng):
     extent = math.sqrt(n / dens)
     return dict(seed=3000000 + k, extent=extent, density=dens, buildings=40, trees=300,
                 embankments=0, outliers=0.0, linear=0.0)
+
+
+def las_image(cloud, fmt=1, system_id=b"SYNTH"):
+    """A LAS 1.4 file image of a synthetic tile in point format 1 or 6: the
+    int32 coordinates with the tile's scale and offset, intensity, return
+    number and count, class 0, GPS time rising with the point index; header
+    counts by return and bounds as the tile gives them."""
+    n = len(cloud["xi"])
+    ret = (cloud["returns"] >> 4).astype(np.uint8)
+    nret = (cloud["returns"] & 15).astype(np.uint8)
+    if fmt == 1:
+        dt = np.dtype([("x", "<i4"), ("y", "<i4"), ("z", "<i4"), ("i", "<u2"), ("rb", "u1"),
+                       ("c", "u1"), ("sa", "i1"), ("ud", "u1"), ("ps", "<u2"), ("t", "<f8")])
+        rec = np.zeros(n, dtype=dt)
+        rec["rb"] = (ret & 7) | ((nret & 7) << 3)
+    elif fmt == 6:
+        dt = np.dtype([("x", "<i4"), ("y", "<i4"), ("z", "<i4"), ("i", "<u2"), ("rb", "u1"),
+                       ("fl", "u1"), ("c", "u1"), ("ud", "u1"), ("sa", "<i2"), ("ps", "<u2"),
+                       ("t", "<f8")])
+        rec = np.zeros(n, dtype=dt)
+        rec["rb"] = (ret & 15) | ((nret & 15) << 4)
+    else:
+        raise ValueError("point format 1 or 6")
+    rec["x"], rec["y"], rec["z"] = cloud["xi"], cloud["yi"], cloud["zi"]
+    rec["i"] = cloud["intensity"]
+    rec["t"] = 1e5 + np.arange(n) * 1e-5
+    by_ret = np.bincount(ret, minlength=16)[1:16].astype(np.uint64)
+    legacy = fmt < 6 and not by_ret[5:].any()
+    h = bytearray(375)
+    h[0:4] = b"LASF"
+    h[24:26] = bytes([1, 4])
+    h[26:26 + len(system_id)] = system_id
+    h[58:69] = b"wolkenbase "
+    struct.pack_into("<HHHIIBH", h, 90, 1, 2020, 375, 375, 0, fmt, dt.itemsize)
+    struct.pack_into("<I5I", h, 107, n if legacy else 0, *(by_ret[:5] if legacy else [0] * 5))
+    struct.pack_into("<3d3d", h, 131, *cloud["scale"], *cloud["offset"])
+    mn, mx = cloud["bbmin"], cloud["bbmax"]
+    struct.pack_into("<6d", h, 179, mx[0], mn[0], mx[1], mn[1], mx[2], mn[2])
+    struct.pack_into("<QQIQ15Q", h, 227, 0, 0, 0, n, *by_ret)
+    return bytes(h) + rec.tobytes()
