@@ -318,3 +318,85 @@ def test_snake_setup_and_size_fit_match_oracle(hm):
         hm.wbh_size_fit(pts.ctypes.data_as(D), 2, c.ctypes.data_as(D), s.ctypes.data_as(D))
         oc, os_ = O.sizefit(lo, hi)
         assert np.array_equal(c, oc) and s[0] == os_
+
+
+def border_care_bins(occ):
+    """Runs of 24 or more empty bins with the occupied bin on each side."""
+    care = set()
+    for st in range(64):
+        if not (occ >> st) & 1 and (occ >> ((st - 1) % 64)) & 1:
+            n = 0
+            while not (occ >> ((st + n) % 64)) & 1:
+                n += 1
+            if n >= 24:
+                care |= {(st + k) % 64 for k in range(-1, n + 1)}
+    return care
+
+
+def test_border_wedge_pruning_is_conservative(hm):
+    # the exact border pass (a longest empty run of 24 or 25 bins) skips a box
+    # only if it holds no direction in the run or its two border bins
+    hm.wbh_border_wedge_hit.restype = C.c_int
+    hm.wbh_border_wedge_hit.argtypes = [C.c_uint64] + [C.c_double] * 6
+    hm.wbh_dir_bin.restype = C.c_int
+    hm.wbh_dir_bin.argtypes = [C.c_double, C.c_double]
+    rng = np.random.default_rng(23)
+    masks = []
+    for it in range(400):
+        runs = [int(rng.choice([24, 25]))] + ([int(rng.choice([24, 25]))] if it % 3 == 0 else [])
+        occ = (1 << 64) - 1
+        s = int(rng.integers(0, 64))
+        for r in runs:
+            occ &= ~sum(1 << ((s + k) % 64) for k in range(r))
+            s = (s + r + 1 + int(rng.integers(0, 6))) % 64
+        # thin out the occupied bins away from the runs' borders
+        care = border_care_bins(occ)
+        for b in range(64):
+            if b not in care and rng.random() < 0.3:
+                occ &= ~(1 << b)
+        # the exact pass sees a longest run of 24 or 25 bins, at most two of them
+        lens = []
+        for st in range(64):
+            if not (occ >> st) & 1 and (occ >> ((st - 1) % 64)) & 1:
+                n = 0
+                while not (occ >> ((st + n) % 64)) & 1:
+                    n += 1
+                lens.append(n)
+        big = [n for n in lens if n >= 24]
+        if not big or max(big) > 25 or len(big) > 2:
+            continue
+        masks.append(occ)
+    rejected = 0
+    for it, occ in enumerate(masks):
+        care = border_care_bins(occ)
+        for rep in range(12):
+            px, py = rng.normal(size=2) * 50 + np.array([500000.0, 4400000.0])
+            w, h = rng.uniform(0.01, 40, 2)
+            cx, cy = px + rng.normal() * 60, py + rng.normal() * 60
+            x0, x1, y0, y1 = cx - w, cx + w, cy - h, cy + h
+            if hm.wbh_border_wedge_hit(occ, px, py, x0, x1, y0, y1):
+                continue
+            rejected += 1
+            us = np.concatenate([[0, 1, 0, 1, 0.5, 0.5, 0, 1], rng.random(40)])
+            vs = np.concatenate([[0, 0, 1, 1, 0, 1, 0.5, 0.5], rng.random(40)])
+            for u, v in zip(us, vs):
+                qx, qy = x0 + u * (x1 - x0), y0 + v * (y1 - y0)
+                if qx == px and qy == py:
+                    continue
+                assert hm.wbh_dir_bin(qy - py, qx - px) not in care, (it, hex(occ), u, v)
+        # tiny boxes at every care edge and its opposite, sub-margin offsets
+        starts = [b for b in care if (b - 1) % 64 not in care]
+        ends = [(b + 1) % 64 for b in care if (b + 1) % 64 not in care]
+        edges = [-np.pi + k * np.pi / 32 for k in starts + ends]
+        edges += [e + np.pi for e in edges]
+        for e in edges:
+            for off in (-2e-3, -1e-3, -5e-4, -1e-5, 0.0, 1e-5, 5e-4, 1e-3, 2e-3):
+                r = rng.uniform(5, 80)
+                cx, cy = r * np.cos(e + off), r * np.sin(e + off)
+                d = 1e-5 * r
+                x0, x1, y0, y1 = cx - d, cx + d, cy - d, cy + d
+                if hm.wbh_border_wedge_hit(occ, 0.0, 0.0, x0, x1, y0, y1):
+                    continue
+                for qx, qy in ((x0, y0), (x0, y1), (x1, y0), (x1, y1), (cx, cy)):
+                    assert hm.wbh_dir_bin(qy, qx) not in care, (hex(occ), e, off)
+    assert len(masks) > 200 and rejected > 500

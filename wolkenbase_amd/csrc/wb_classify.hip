@@ -1500,7 +1500,7 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
     }
     if (lane == 0) {
       if (phase < 2) wb::care_wedges(checked, wg, A.edges);
-      else wg.live = -1;
+      else wb::border_wedges(before, afterb, nr, wg, A.edges);
     }
     __syncthreads();
     bool done = false;

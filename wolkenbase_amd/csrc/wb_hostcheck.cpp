@@ -64,6 +64,34 @@ int wbh_wedge_hit(uint64_t occ, double px, double py, double x0, double x1, doub
   return wb::wedge_hit(w, px, py, x0, x1, y0, y1) ? 1 : 0;
 }
 
+// the border wedges of the runs of >= 24 bins of occ (before/after as the
+// exact pass finds them) and the box test against them
+int wbh_border_wedge_hit(uint64_t occ, double px, double py, double x0, double x1, double y0,
+                         double y1)
+{
+  static float tab[3 * 128];
+  static int filled;
+  if (!filled) {
+    wb::fill_edge_table(tab);
+    filled = 1;
+  }
+  int before[2] = {-1, -1}, after[2] = {-1, -1}, nr = 0;
+  for (int st = 0; st < 64; st++) {
+    if ((occ >> st) & 1) continue;
+    if (!((occ >> ((st + 63) & 63)) & 1)) continue;
+    int len = 0;
+    while (len < 64 && !((occ >> ((st + len) & 63)) & 1)) len++;
+    if (len >= 24 && nr < 2) {
+      before[nr] = (st + 63) & 63;
+      after[nr] = (st + len) & 63;
+      nr++;
+    }
+  }
+  wb::Wedges w;
+  wb::border_wedges(before, after, nr, w, tab);
+  return wb::wedge_hit(w, px, py, x0, x1, y0, y1) ? 1 : 0;
+}
+
 int wbh_atan2i_forced(double y, double x)
 {
   ready();

@@ -710,6 +710,31 @@ WB_HD void care_wedges(uint64_t occ, Wedges &w, const float *tab)
   w.live = live;
 }
 
+// The care wedges of the exact border pass: with a longest empty run of 24
+// or 25 bins, only directions in the run or in the two occupied bins that
+// border it can change the exact gap (a direction in the run breaks it, the
+// extreme directions of the border bins are its ends).  Run r (r < nr <= 2)
+// spans bins before[r]+1 .. after[r]-1; its wedge runs from the start edge of
+// bin before[r] turned back by 1e-3 rad to the end edge of bin after[r]
+// turned on by 1e-3 rad: at most 27 bins, so convex.  Slots 0 and 2.
+WB_HD void border_wedges(const int before[2], const int after[2], int nr, Wedges &w,
+                         const float *tab)
+{
+  int live = 0;
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    if (r < nr) {
+      const int s = before[r] & 63, e = (after[r] + 1) & 63;
+      w.ax[2 * r] = tab[2 * s];
+      w.ay[2 * r] = tab[2 * s + 1];
+      w.bx[2 * r] = tab[256 + 2 * e];
+      w.by[2 * r] = tab[256 + 2 * e + 1];
+      live |= 1 << (2 * r);
+    }
+  }
+  w.live = nr > 0 ? live : -1;
+}
+
 // false only if the box [x0,x1] x [y0,y1] lies outside every wedge as seen
 // from (px, py): all four corners strictly right of a wedge's first edge, or
 // all strictly left of its second edge (the box is convex)
