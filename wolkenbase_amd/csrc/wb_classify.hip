@@ -295,19 +295,77 @@ __device__ inline uint32_t wave_count_add(uint32_t *cnt, bool has, int key)
   return slot;
 }
 
+// The (leaf, tile) part of the membership test, once per leaf for the
+// lattice cells near it: a window of cells (ex, ey), ex in [x0, x0+w), ey in
+// [y0, y0+h), w*h <= LEAF_WIN, covering every cell whose centre can lie
+// within the candidate bound of a point inside the leaf's square (one cell
+// of slack on each side).  Flag bit 0: a tile whose cylinder findBlocks
+// reaches this leaf through (every ancestor cube and the leaf cube intersect
+// it); bit 1: all eight leaf corners in the cylinder.  w = 0: the leaf's
+// window is too large and its points take the full test.
+#define LEAF_WIN 256
+struct LeafWin {
+  int x0, y0, w, h;
+};
+
+__global__ void __launch_bounds__(64) k_leaf_tiles(TreeView T, TileGeo g, LeafWin *win,
+                                                   uint8_t *flags)
+{
+  const uint32_t l = blockIdx.x;
+  if (l >= T.nleaves) return;
+  const double *lc = T.leaf_cube + 4 * (size_t)l;
+  const double h = lc[3] / 2, lim = g.r * (1 + 1e-9) + 1e-9;
+  const double fy0 = ((lc[1] - h - lim) - g.scy) / g.spacing / kSqrt34;
+  const double fy1 = ((lc[1] + h + lim) - g.scy) / g.spacing / kSqrt34;
+  const double u0 = ((lc[0] - h - lim) - g.scx) / g.spacing;
+  const double u1 = ((lc[0] + h + lim) - g.scx) / g.spacing;
+  LeafWin wl = {0, 0, 0, 0};
+  if (fabs(fy0) < 1e9 && fabs(fy1) < 1e9 && fabs(u0) < 1e9 && fabs(u1) < 1e9) {
+    wl.y0 = (int)floor(fy0) - 1;
+    const int y1 = (int)ceil(fy1) + 1;
+    wl.x0 = (int)floor(u0 + wl.y0 / 2.0) - 1;
+    const int x1 = (int)ceil(u1 + y1 / 2.0) + 1;
+    wl.w = x1 - wl.x0 + 1;
+    wl.h = y1 - wl.y0 + 1;
+    if (wl.w <= 0 || wl.h <= 0 || wl.w * wl.h > LEAF_WIN) wl.w = wl.h = 0;
+  }
+  if (threadIdx.x == 0) win[l] = wl;
+  const int ncell = wl.w * wl.h;
+  for (int c = threadIdx.x; c < ncell; c += 64) {
+    const int ex = wl.x0 + c % wl.w, ey = wl.y0 + c / wl.w;
+    uint8_t f = 0;
+    int cnt;
+    if (tile_at(g, ex, ey, &cnt)) {
+      double cx, cy;
+      tile_center(g, ex, ey, &cx, &cy);
+      const wb::Shape cyl = wb::make_cylinder(cx, cy, g.r);
+      bool reach = wb::shape_intersect(cyl, lc[0], lc[1], lc[2], lc[3]);
+      for (int nd = T.leaf_parent[l]; reach && nd >= 0; nd = T.node_parent[nd]) {
+        const double *q = T.node_cube + 4 * nd;
+        reach = wb::shape_intersect(cyl, q[0], q[1], q[2], q[3]);
+      }
+      if (reach) f = 1 | (wb::shape_in_cube(cyl, lc[0], lc[1], lc[2], lc[3]) ? 2 : 0);
+    }
+    flags[(size_t)l * LEAF_WIN + c] = f;
+  }
+}
+
 // pointsIn(cylinder) membership of every point, evaluated from the point's
 // side: the candidates are the lattice tiles near it; a tile holds the point
 // iff findBlocks reaches its leaf (every ancestor cube and the leaf cube
 // intersect the cylinder) and either all eight leaf corners are in or the
-// point itself is.
-__global__ void k_membership(TreeView T, TileGeo g, int32_t *memb, uint32_t *nmemb,
-                             uint32_t *tile_cnt, uint32_t *err)
+// point itself is.  The leaf's part comes from its k_leaf_tiles window;
+// candidates outside the window (points outside their leaf's cube) take the
+// whole test.
+__global__ void k_membership(TreeView T, TileGeo g, const LeafWin *win, const uint8_t *flags,
+                             int32_t *memb, uint32_t *nmemb, uint32_t *tile_cnt, uint32_t *err)
 {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= T.n) return;
   const double x = T.px[i], y = T.py[i];
   const uint32_t leaf = T.pt_leaf[i];
   const double *lc = T.leaf_cube + 4 * leaf;
+  const LeafWin wl = win[leaf];
   double u = (x - g.scx) / g.spacing, v = (y - g.scy) / g.spacing;
   double fy = v / kSqrt34;
   int Y = (int)__builtin_rint(fy);
@@ -326,15 +384,22 @@ __global__ void k_membership(TreeView T, TileGeo g, int32_t *memb, uint32_t *nme
       int cnt;
       if (!tile_at(g, ex, ey, &cnt)) continue;
       wb::Shape cyl = wb::make_cylinder(cx, cy, g.r);
-      bool in = wb::shape_in_cube(cyl, lc[0], lc[1], lc[2], lc[3]) || wb::cyl_in(cyl, x, y);
-      if (!in) continue;
-      if (!wb::shape_intersect(cyl, lc[0], lc[1], lc[2], lc[3])) continue;
-      bool reach = true;
-      for (int nd = T.leaf_parent[leaf]; nd >= 0; nd = T.node_parent[nd]) {
-        const double *q = T.node_cube + 4 * nd;
-        if (!wb::shape_intersect(cyl, q[0], q[1], q[2], q[3])) { reach = false; break; }
+      const int gx = ex - wl.x0, gy = ey - wl.y0;
+      if (gx >= 0 && gy >= 0 && gx < wl.w && gy < wl.h) {
+        const uint8_t f = flags[(size_t)leaf * LEAF_WIN + gy * wl.w + gx];
+        if (!(f & 1)) continue;
+        if (!(f & 2) && !wb::cyl_in(cyl, x, y)) continue;
+      } else {
+        bool in = wb::shape_in_cube(cyl, lc[0], lc[1], lc[2], lc[3]) || wb::cyl_in(cyl, x, y);
+        if (!in) continue;
+        if (!wb::shape_intersect(cyl, lc[0], lc[1], lc[2], lc[3])) continue;
+        bool reach = true;
+        for (int nd = T.leaf_parent[leaf]; nd >= 0; nd = T.node_parent[nd]) {
+          const double *q = T.node_cube + 4 * nd;
+          if (!wb::shape_intersect(cyl, q[0], q[1], q[2], q[3])) { reach = false; break; }
+        }
+        if (!reach) continue;
       }
-      if (!reach) continue;
       if (nm < MAX_MEMB) mem[nm] = cnt;
       nm++;
     }
@@ -1793,9 +1858,15 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
   hipLaunchKernelGGL(k_fill_i32, dim3(wb_blocks(gcells, 256)), dim3(256), 0, st, grid, gcells, -1);
   g.grid = grid;
   hipLaunchKernelGGL(k_tile_grid, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey, grid);
+  LeafWin *lwin = (LeafWin *)wb_alloc(c, sizeof(LeafWin) * ((size_t)t->nleaves + 1));
+  uint8_t *lflags = (uint8_t *)wb_alloc(c, (size_t)LEAF_WIN * t->nleaves + 16);
+  if (t->nleaves)
+    hipLaunchKernelGGL(k_leaf_tiles, dim3(t->nleaves), dim3(64), 0, st, T, g, lwin, lflags);
   if (n > 0)
-    hipLaunchKernelGGL(k_membership, dim3(wb_blocks(n, 256)), dim3(256), 0, st, T, g, memb, nmemb,
-                       tile_cnt, err);
+    hipLaunchKernelGGL(k_membership, dim3(wb_blocks(n, 256)), dim3(256), 0, st, T, g, lwin,
+                       lflags, memb, nmemb, tile_cnt, err);
+  wb_free(c, lwin, sizeof(LeafWin) * ((size_t)t->nleaves + 1));
+  wb_free(c, lflags, (size_t)LEAF_WIN * t->nleaves + 16);
   // tile lists in canonical order
   uint32_t *tile_off = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
   WB_CHECK(hipMemcpyAsync(tile_off, tile_cnt, 4ull * g.ntiles, hipMemcpyDeviceToDevice, st));
