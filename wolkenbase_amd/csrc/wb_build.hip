@@ -113,6 +113,9 @@ uint64_t wb_exclusive_scan_u32(wb_ctx *c, uint32_t *data, uint64_t n)
 // a device-wide exclusive scan of the digit-major histogram table, then a
 // stable scatter in which each wave ranks its keys with 8 ballots (peer
 // mask of equal digits) and one LDS atomic per distinct digit per round.
+// The block's keys are then placed in LDS in digit order, and consecutive
+// threads write each digit's run to its consecutive output slots, so every
+// store instruction writes whole runs rather than 64 scattered buckets.
 // ---------------------------------------------------------------------------
 #define RS_THREADS 256
 #define RS_ITEMS 16
@@ -141,6 +144,9 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K *kin, const u
                                                              unsigned nblocks)
 {
   __shared__ uint32_t wcnt[4][256];
+  __shared__ uint32_t dstart[256];
+  __shared__ K skey[RS_TILE];
+  __shared__ uint32_t sval[RS_TILE];
   const int w = threadIdx.x / WAVE;
   const unsigned lane = lane_id();
   for (int d = threadIdx.x; d < 4 * 256; d += RS_THREADS) (&wcnt[0][0])[d] = 0;
@@ -172,11 +178,23 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K *kin, const u
     rank[r] = old + before;
   }
   __syncthreads();
-  // per-digit prefix over the four waves
+  // per-digit prefix over the four waves, and the block's digit starts (an
+  // exclusive scan of the digit totals across the 256 threads)
   {
-    int d = threadIdx.x;
-    uint32_t a0 = wcnt[0][d], a1 = wcnt[1][d], a2 = wcnt[2][d];
+    const int d = threadIdx.x;
+    const uint32_t a0 = wcnt[0][d], a1 = wcnt[1][d], a2 = wcnt[2][d], a3 = wcnt[3][d];
+    const uint32_t tot = a0 + a1 + a2 + a3;
+    uint32_t inc = tot;  // inclusive scan within the wave
+    for (int s = 1; s < WAVE; s <<= 1) {
+      const uint32_t o = __shfl_up(inc, s, WAVE);
+      if ((int)lane >= s) inc += o;
+    }
+    __shared__ uint32_t wsum[4];
+    if ((int)lane == WAVE - 1) wsum[w] = inc;
     __syncthreads();
+    uint32_t woff = 0;
+    for (int k = 0; k < w; k++) woff += wsum[k];
+    dstart[d] = woff + inc - tot;
     wcnt[0][d] = 0;
     wcnt[1][d] = a0;
     wcnt[2][d] = a0 + a1;
@@ -186,10 +204,20 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K *kin, const u
   for (int r = 0; r < RS_ITEMS; r++) {
     uint64_t i = base + (uint64_t)r * WAVE + lane;
     if (i >= n) continue;
-    uint32_t d = dig[r];
-    uint64_t pos = (uint64_t)offs[(uint64_t)d * nblocks + blockIdx.x] + wcnt[w][d] + rank[r];
-    kout[pos] = key[r];
-    vout[pos] = val[r];
+    const uint32_t d = dig[r];
+    const uint32_t lpos = dstart[d] + wcnt[w][d] + rank[r];
+    skey[lpos] = key[r];
+    sval[lpos] = val[r];
+  }
+  __syncthreads();
+  const uint64_t b0 = (uint64_t)blockIdx.x * RS_TILE;
+  const uint32_t cnt = (uint32_t)(n - b0 < (uint64_t)RS_TILE ? n - b0 : RS_TILE);
+  for (uint32_t j = threadIdx.x; j < cnt; j += RS_THREADS) {
+    const K k = skey[j];
+    const uint32_t d = (unsigned)(k >> shift) & 255u;
+    const uint64_t pos = (uint64_t)offs[(uint64_t)d * nblocks + blockIdx.x] + (j - dstart[d]);
+    kout[pos] = k;
+    vout[pos] = sval[j];
   }
 }
 
