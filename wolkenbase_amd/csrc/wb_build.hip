@@ -304,10 +304,14 @@ __device__ inline uint64_t spread3(uint32_t v)
 __device__ inline double decode(int32_t v, double s, double o) { return o + s * (double)v; }
 
 // the input points: int32 LAS coordinates with the transform of Xform, or
-// coordinates already decoded to FP64
+// coordinates already decoded to FP64; `packed` is a copy k_keygen writes
+// with x, y, z of a point side by side (int4, or double4 for FP64), so the
+// later gathers by sorted slot fetch one vector per point instead of three
+// scattered words
 struct Src {
   const int32_t *xi, *yi, *zi;
   const double *fx, *fy, *fz;
+  void *packed;
 };
 
 template <bool F64>
@@ -321,6 +325,20 @@ __device__ inline void load_point(const Src &s, const Xform &f, uint64_t i, doub
   }
 }
 
+// the same point from the packed copy
+template <bool F64>
+__device__ inline void load_packed(const Src &s, const Xform &f, uint64_t i, double &x, double &y,
+                                   double &z)
+{
+  if (F64) {
+    const double4 v = ((const double4 *)s.packed)[i];
+    x = v.x; y = v.y; z = v.z;
+  } else {
+    const int4 v = ((const int4 *)s.packed)[i];
+    x = decode(v.x, f.sx, f.ox); y = decode(v.y, f.sy, f.oy); z = decode(v.z, f.sz, f.oz);
+  }
+}
+
 template <bool F64>
 __global__ void k_keygen(Src src, uint64_t n, Xform f, uint64_t *key, uint32_t *val)
 {
@@ -328,6 +346,8 @@ __global__ void k_keygen(Src src, uint64_t n, Xform f, uint64_t *key, uint32_t *
   if (i >= n) return;
   double x, y, z;
   load_point<F64>(src, f, i, x, y, z);
+  if (F64) ((double4 *)src.packed)[i] = make_double4(x, y, z, 0.0);
+  else ((int4 *)src.packed)[i] = make_int4(src.xi[i], src.yi[i], src.zi[i], 0);
   uint32_t ix = cell_index(x, f.lox, f.cell, f.inv_cell);
   uint32_t iy = cell_index(y, f.loy, f.cell, f.inv_cell);
   uint32_t iz = cell_index(z, f.loz, f.cell, f.inv_cell);
@@ -497,7 +517,7 @@ __global__ void k_permute(Src src, const uint32_t *sidx, uint64_t n, Xform f, do
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double x, y, z;
-  load_point<F64>(src, f, sidx[i], x, y, z);
+  load_packed<F64>(src, f, sidx[i], x, y, z);
   px[i] = x;
   py[i] = y;
   pz[i] = z;
@@ -529,14 +549,14 @@ __global__ void k_dup_mark(const uint64_t *key, const uint32_t *sidx, uint64_t n
     return;
   }
   double x, y, z;
-  load_point<F64>(src, f, sidx[s], x, y, z);
+  load_packed<F64>(src, f, sidx[s], x, y, z);
   uint64_t a = s, b = s + 1;
   while (a > 0 && (key[a - 1] >> shift) == k) a--;
   while (b < n && (key[b] >> shift) == k) b++;
   uint64_t first = s, lastj = s;
   for (uint64_t j = a; j < s; j++) {
     double qx, qy, qz;
-    load_point<F64>(src, f, sidx[j], qx, qy, qz);
+    load_packed<F64>(src, f, sidx[j], qx, qy, qz);
     if (qx == x && qy == y && qz == z) {
       first = j;
       break;
@@ -544,7 +564,7 @@ __global__ void k_dup_mark(const uint64_t *key, const uint32_t *sidx, uint64_t n
   }
   for (uint64_t j = b - 1; j > s; j--) {
     double qx, qy, qz;
-    load_point<F64>(src, f, sidx[j], qx, qy, qz);
+    load_packed<F64>(src, f, sidx[j], qx, qy, qz);
     if (qx == x && qy == y && qz == z) {
       lastj = j;
       break;
@@ -676,7 +696,8 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
   const uint64_t n_input = t->n_input;
   uint64_t n = n_input;
   const bool f64 = fx != nullptr;
-  Src src = {dx, dy, dz, fx, fy, fz};
+  const size_t pk_bytes = (f64 ? 32 : 16) * (size_t)(n_input ? n_input : 1);
+  Src src = {dx, dy, dz, fx, fy, fz, wb_alloc(c, pk_bytes)};
   t->n = n;
   t->ndup = 0;
   t->dup_pairs = nullptr;
@@ -796,11 +817,17 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
       std::swap(act, nxt);
     }
     if (h_out.overflow & 1u) {
-      if (passes >= 8) throw std::string("octree deeper than 21 levels (more than 537 points in a level-21 cell)");
+      if (passes >= 8) {
+        wb_free(c, src.packed, pk_bytes);
+        throw std::string("octree deeper than 21 levels (more than 537 points in a level-21 cell)");
+      }
       passes++;
       continue;  // the sort did not reach deep enough: redo with one more digit
     }
-    if (h_out.overflow) throw std::string("octree node/leaf capacity exceeded");
+    if (h_out.overflow) {
+      wb_free(c, src.packed, pk_bytes);
+      throw std::string("octree node/leaf capacity exceeded");
+    }
     break;
   }
   wb_free(c, act, sizeof(Active) * node_cap);
@@ -852,6 +879,8 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
     else
       hipLaunchKernelGGL(k_permute<false>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, src,
                          t->sidx, n, f, t->px, t->py, t->pz);
+    wb_free(c, src.packed, pk_bytes);
+    src.packed = nullptr;
     hipLaunchKernelGGL(k_chunk_box, dim3(wb_blocks(nchunks * WB_CHUNK, 256)), dim3(256), 0,
                        c->stream, t->px, t->py, t->pz, n, t->chunk_box, t->chunk_low);
     hipLaunchKernelGGL(k_leaf_canon, dim3(nl), dim3(256), 0, c->stream, t->leaf_start, nl,
@@ -864,5 +893,6 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
     t->ms_build = a;
     t->ms_sort = b;
   }
+  if (src.packed) wb_free(c, src.packed, pk_bytes);
   for (int k = 0; k < 4; k++) hipEventDestroy(evb[k]);
 }
