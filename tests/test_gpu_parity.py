@@ -70,6 +70,34 @@ def test_tree_small_sizes(ctx):
         assert np.array_equal(tree.order().astype(np.int64), ref.order()), n
 
 
+def test_tree_deep_cluster_more_sort_digits(ctx):
+    # 1000 distinct points in a 1 cm cube inside a 30 m tile need leaves far
+    # below the depth the default sort covers: the build sorts again with
+    # more digits (and, with duplicates, merges again) until it reaches them
+    cloud = synth.generate(seed=61, extent=30.0, density=20.0, buildings=0, trees=10)
+    g = np.arange(10)
+    gx, gy, gz = np.meshgrid(g, g, g, indexing="ij")
+    c0 = (int(np.median(cloud["xi"])), int(np.median(cloud["yi"])), int(np.median(cloud["zi"])))
+    for key, gk, c in (("xi", gx, c0[0]), ("yi", gy, c0[1]), ("zi", gz, c0[2])):
+        cloud[key] = np.concatenate([cloud[key], (gk.ravel() + c).astype(np.int32)])
+    tree, ref, (x, y, z) = build_both(ctx, cloud)
+    assert tree.info()["maxdepth"] == ref.maxdepth == 12
+    assert tree.info()["sort_passes"] == 5  # the default for a 32 m root is 3
+    assert np.array_equal(tree.leaves()["count"], ref.leaves()["count"])
+    assert np.array_equal(tree.order().astype(np.int64), ref.order())
+    # the same with every cluster point given twice, merged
+    xi, yi, zi = (np.concatenate([cloud[k], cloud[k][-1000:]]) for k in ("xi", "yi", "zi"))
+    center, side = W.Octree.size_fit(cloud["bbmin"], cloud["bbmax"])
+    t2 = W.Octree.build(ctx, xi, yi, zi, cloud["scale"], cloud["offset"], center, side,
+                        dedupe=True)
+    assert t2.ndup == 1000
+    assert np.array_equal(t2.order().astype(np.int64), ref.order())
+    pairs = t2.duplicates()
+    n0 = len(cloud["xi"])
+    assert sorted(pairs[:, 0].tolist()) == list(range(n0, n0 + 1000))
+    assert np.array_equal(pairs[np.argsort(pairs[:, 0]), 1], np.arange(n0 - 1000, n0))
+
+
 def test_degenerate_root_rejected(ctx):
     # Octree::sizeFit gives side 0 for a single location; the reference then
     # asserts side>0 on the first findBlock (octree.cpp:208).  The ABI

@@ -380,11 +380,17 @@ struct SplitOut {
 
 __device__ inline uint32_t digit_at(uint64_t k, int level) { return (uint32_t)(k >> (64 - 3 * level)) & 7u; }
 
-__global__ void k_split_level(const uint64_t *key, const Active *act, uint32_t nact, int level,
-                              int max_level, Active *next, uint32_t next_cap, LeafTmp *leaves,
+// The levels are launched back to back: the number of active nodes of a
+// level is the counter the previous level incremented (act_cnt), and a level
+// after a failed one does nothing.
+__global__ void k_split_level(const uint64_t *key, const Active *act, const uint32_t *act_cnt,
+                              uint32_t act_cap, int level, int max_level, Active *next,
+                              uint32_t *next_cnt, uint32_t next_cap, LeafTmp *leaves,
                               uint32_t leaf_cap, double *node_cube, int32_t *node_child,
                               int32_t *node_parent, uint32_t node_cap, SplitOut *out)
 {
+  if (out->overflow) return;
+  const uint32_t nact = min(*act_cnt, act_cap);
   uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= nact) return;
   Active A = act[a];
@@ -420,7 +426,7 @@ __global__ void k_split_level(const uint64_t *key, const Active *act, uint32_t n
           atomicOr(&out->overflow, 1u);
         } else {
           uint32_t nid = atomicAdd(&out->n_node, 1u);
-          uint32_t slot = atomicAdd(&out->n_next, 1u);
+          uint32_t slot = atomicAdd(next_cnt, 1u);
           if (nid < node_cap && slot < next_cap) {
             node_cube[4 * nid] = ncx;
             node_cube[4 * nid + 1] = ncy;
@@ -723,6 +729,7 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
   uint32_t leaf_cap = (uint32_t)std::min<uint64_t>(n / 8 + 1024, 0x7fffffff);
   LeafTmp *lt = nullptr;
   SplitOut *d_out = (SplitOut *)wb_alloc(c, sizeof(SplitOut));
+  uint32_t *d_cnt = (uint32_t *)wb_alloc(c, 4 * (WB_MAX_LEVEL + 2));
   SplitOut h_out;
   Active *act = (Active *)wb_alloc(c, sizeof(Active) * node_cap);
   Active *nxt = (Active *)wb_alloc(c, sizeof(Active) * node_cap);
@@ -798,24 +805,24 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
     if (lt) wb_free(c, lt, sizeof(LeafTmp) * leaf_cap);
     lt = (LeafTmp *)wb_alloc(c, sizeof(LeafTmp) * leaf_cap);
     int max_level = std::min(WB_MAX_LEVEL, (8 * passes) / 3);
-    uint32_t nact = 0;
+    // active nodes of one level are disjoint and hold more than 537 points
+    const uint32_t act_ub = (uint32_t)std::min<uint64_t>(node_cap, n / (WB_RECORDS + 1) + 1);
+    uint32_t hcnt[WB_MAX_LEVEL + 2] = {};
     if (n > 0) {
       Active r0 = {0u, (uint32_t)n, 0, 0};
       WB_CHECK(hipMemcpyAsync(act, &r0, sizeof r0, hipMemcpyHostToDevice, c->stream));
-      nact = 1;
+      hcnt[0] = 1;
     }
-    for (int level = 0; nact > 0; level++) {
-      uint32_t zero = 0;
-      WB_CHECK(hipMemcpyAsync(&d_out->n_next, &zero, 4, hipMemcpyHostToDevice, c->stream));
-      hipLaunchKernelGGL(k_split_level, dim3(wb_blocks(nact, 128)), dim3(128), 0, c->stream,
-                         t->key, act, nact, level, max_level, nxt, node_cap, lt, leaf_cap,
-                         t->node_cube, t->node_child, t->node_parent, node_cap, d_out);
-      WB_CHECK(hipMemcpyAsync(&h_out, d_out, sizeof h_out, hipMemcpyDeviceToHost, c->stream));
-      WB_CHECK(hipStreamSynchronize(c->stream));
-      if (h_out.overflow) break;
-      nact = h_out.n_next;
+    WB_CHECK(hipMemcpyAsync(d_cnt, hcnt, sizeof hcnt, hipMemcpyHostToDevice, c->stream));
+    for (int level = 0; n > 0 && level < max_level; level++) {
+      hipLaunchKernelGGL(k_split_level, dim3(wb_blocks(act_ub, 128)), dim3(128), 0, c->stream,
+                         t->key, act, d_cnt + level, node_cap, level, max_level, nxt,
+                         d_cnt + level + 1, node_cap, lt, leaf_cap, t->node_cube, t->node_child,
+                         t->node_parent, node_cap, d_out);
       std::swap(act, nxt);
     }
+    WB_CHECK(hipMemcpyAsync(&h_out, d_out, sizeof h_out, hipMemcpyDeviceToHost, c->stream));
+    WB_CHECK(hipStreamSynchronize(c->stream));
     if (h_out.overflow & 1u) {
       if (passes >= 8) {
         wb_free(c, src.packed, pk_bytes);
@@ -837,6 +844,7 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
   t->nnodes = h_out.n_node;
   t->maxdepth = (int)h_out.maxdepth;
   wb_free(c, d_out, sizeof(SplitOut));
+  wb_free(c, d_cnt, 4 * (WB_MAX_LEVEL + 2));
 
   // leaves in DFS order = ascending start
   const uint32_t nl = t->nleaves;
