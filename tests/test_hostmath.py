@@ -109,14 +109,21 @@ def test_dir_bin_matches_exact_bin(hm):
     L = O.lib()
     rng = np.random.default_rng(19)
     n = 100000
-    # random, LAS-like, axis-aligned, and directions placed on bin edges
-    edges = (np.arange(64) * 2 ** 25 - 2 ** 30)[rng.integers(0, 64, n)] + rng.integers(-70000, 70000, n)
+    # random, LAS-like, axis-aligned, diagonal, and directions placed near bin
+    # edges: within 70000 units, and within 4096 units (the float path's
+    # margin is 2048 units, its error a few hundred)
+    base = (np.arange(64) * 2 ** 25 - 2 ** 30)
+    edges = np.concatenate([base[rng.integers(0, 64, n)] + rng.integers(-70000, 70000, n),
+                            base[rng.integers(0, 64, n)] + rng.integers(-4096, 4097, n)])
     ang = edges / 2 ** 31 * 2 * np.pi
-    r = rng.uniform(0.001, 500, n)
+    r = rng.uniform(0.001, 500, 2 * n)
+    d = rng.integers(1, 4000, 20) * 0.001
+    sx = rng.choice([-1.0, 1.0], 20)
+    sy = rng.choice([-1.0, 1.0], 20)
     ys = np.concatenate([rng.normal(size=n), rng.integers(-4000, 4000, n) * 0.001, r * np.sin(ang),
-                         np.zeros(10), rng.normal(size=10)])
+                         np.zeros(10), rng.normal(size=10), sy * d])
     xs = np.concatenate([rng.normal(size=n), rng.integers(-4000, 4000, n) * 0.001, r * np.cos(ang),
-                         rng.normal(size=10), np.zeros(10)])
+                         rng.normal(size=10), np.zeros(10), sx * d])
     bad = 0
     for y, x in zip(ys, xs):
         if x == 0 and y == 0:
@@ -197,7 +204,7 @@ def test_wedge_pruning_is_conservative(hm):
     # runs of exactly 24, 31, 32 (more than pi with the margins), 33, 40 and
     # two runs, with small boxes all round p at every angle
     specials = []
-    for run in (24, 31, 32, 33, 40, 62):
+    for run in (24, 31, 32, 33, 40, 62, 63):
         s = int(rng.integers(0, 64))
         specials.append(sum(1 << ((s + run + k) % 64) for k in range(64 - run)))
     specials.append(((1 << 64) - 1) & ~(((1 << 24) - 1) | (((1 << 25) - 1) << 30)))
@@ -206,7 +213,15 @@ def test_wedge_pruning_is_conservative(hm):
         # run edges, the edges half a turn away, and sub-margin offsets
         starts = [b for b in care if (b - 1) % 64 not in care]
         ends = [(b + 1) % 64 for b in care if (b + 1) % 64 not in care]
-        edges = [-np.pi + k * np.pi / 32 for k in starts + ends]
+        # and the middle edge of every run of 32 bins or more, where it is split
+        mids = []
+        for b in starts:
+            n = 0
+            while (b + n) % 64 in care and n < 64:
+                n += 1
+            if n >= 32:
+                mids.append((b + n // 2) % 64)
+        edges = [-np.pi + k * np.pi / 32 for k in starts + ends + mids]
         edges += [e + np.pi for e in edges]
         for e in edges:
             for off in (-3e-3, -1e-3, -6e-4, -2e-4, -1e-5, 0.0, 1e-5, 2e-4, 6e-4, 1e-3, 3e-3):
@@ -387,7 +402,15 @@ def test_border_wedge_pruning_is_conservative(hm):
         # tiny boxes at every care edge and its opposite, sub-margin offsets
         starts = [b for b in care if (b - 1) % 64 not in care]
         ends = [(b + 1) % 64 for b in care if (b + 1) % 64 not in care]
-        edges = [-np.pi + k * np.pi / 32 for k in starts + ends]
+        # and the middle edge of every run of 32 bins or more, where it is split
+        mids = []
+        for b in starts:
+            n = 0
+            while (b + n) % 64 in care and n < 64:
+                n += 1
+            if n >= 32:
+                mids.append((b + n // 2) % 64)
+        edges = [-np.pi + k * np.pi / 32 for k in starts + ends + mids]
         edges += [e + np.pi for e in edges]
         for e in edges:
             for off in (-2e-3, -1e-3, -5e-4, -1e-5, 0.0, 1e-5, 5e-4, 1e-3, 2e-3):

@@ -1518,7 +1518,7 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
   __shared__ wb::Wedges wg;  // the care wedges of walks A and B (uniform)
   if (blockIdx.x >= nheavy) return;
   const uint64_t t_start = COUNT ? wall_clock64() : 0;
-  uint64_t c_leaf = 0, c_pts = 0, c_exp = 0;
+  uint64_t c_leaf = 0, c_pts = 0, c_exp = 0, c_test = 0, c_in = 0;
   const TreeView &T = A.T;
   const HeavyRec hr = heavy[blockIdx.x];
   const uint32_t i = hr.i;
@@ -1611,7 +1611,8 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
       lim = min(4 * lim, (uint32_t)HB_LEAVES);
       if (COUNT) c_leaf += nq;
       // per-leaf bounds, inside flag (bit 31 of lmask) and group mask, and
-      // the prefix of the leaves' sample or slot counts
+      // the prefix of the leaves' sample counts (walk A) or selected-group
+      // counts (walks B and C)
       uint32_t carry = 0;
       for (uint32_t k0 = 0; k0 < nq; k0 += WAVE) {
         const uint32_t k = k0 + lane;
@@ -1640,7 +1641,8 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
           lb[k] = b;
           le[k] = e;
           lmask[k] = mask;
-          cnt = !seen ? 0 : phase == 0 ? ng : e - b;
+          cnt = !seen ? 0 : phase == 0 ? ng
+                                       : __popc(mask & 0x7fffffffu) + (ng > 31 ? ng - 31 : 0);
         }
         uint32_t incl = cnt;
         for (int s = 1; s < WAVE; s <<= 1) {
@@ -1653,27 +1655,44 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
       if (lane == 0) lpre[0] = 0;
       __syncthreads();
       const uint32_t S = carry;
-      if (COUNT) c_pts += S;
+      if (COUNT) c_pts += phase == 0 ? S : WB_CHUNK * S;
       uint32_t kc = 0;  // this lane's leaf in the batch
-      for (uint32_t f0 = 0; f0 < S && !done; f0 += WAVE) {
-        const uint32_t f = f0 + lane;
+      // walk A: a sample per lane; walks B and C: two selected groups per
+      // step, lanes 0-31 the slots of the first, 32-63 of the second
+      const uint32_t per = phase == 0 ? WAVE : WAVE / WB_CHUNK;
+      for (uint32_t f0 = 0; f0 < S && !done; f0 += per) {
+        const uint32_t f = phase == 0 ? f0 + lane : f0 + lane / WB_CHUNK;
         while (kc + 1 < nq && lpre[kc + 1] <= f) kc++;
         if (f < S) {
           const uint32_t b = lb[kc], e = le[kc], mask = lmask[kc], o = f - lpre[kc];
+          const uint32_t g0 = b / WB_CHUNK;
           uint32_t j;
           bool test = true;
           if (phase == 0) {
-            const uint32_t g0 = b / WB_CHUNK;
             j = T.chunk_low[g0 + o];
             if (j < b || j >= e) j = o == 0 ? b : (g0 + o) * WB_CHUNK;
           } else {
-            j = b + o;
-            const uint32_t g = j / WB_CHUNK - b / WB_CHUNK;
-            if (g < 31 && !((mask >> g) & 1)) test = false;
+            // the o-th selected group of the leaf: a set bit of the mask,
+            // or one of the groups past the 31st
+            const uint32_t m31 = mask & 0x7fffffffu, pc = __popc(m31);
+            uint32_t g;
+            if (o < pc) {
+              uint32_t m = m31;
+              for (uint32_t q = 0; q < o; q++) m &= m - 1;
+              g = __ffs(m) - 1;
+            } else {
+              g = 31 + (o - pc);
+            }
+            j = (g0 + g) * WB_CHUNK + (lane & (WB_CHUNK - 1));
+            test = j >= b && j < e;
           }
           if (test) {
             const double qx = T.px[j], qy = T.py[j], qz = T.pz[j];
             const bool in = (phase != 0 && (mask >> 31)) || wb::hyp_in(H, qx, qy, qz);
+            if (COUNT) {
+              c_test++;
+              if (in) c_in++;
+            }
             if (in && (qx != x || qy != y)) {
               const int bin = wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
               if (phase < 2) {
@@ -1723,18 +1742,26 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
     if (lane == 0) atomicAdd(A.nslow, 1u);
   }
   for (int s = 1; s < 64; s <<= 1) nexact += __shfl_xor(nexact, s, 64);
+  if (COUNT)
+    for (int s = 1; s < 64; s <<= 1) {
+      c_test += __shfl_xor(c_test, s, 64);
+      c_in += __shfl_xor(c_in, s, 64);
+    }
   if (lane == 0) {
     A.cls[i] = (uint8_t)verdict;
     if (nexact) atomicAdd(A.atan_exact, (unsigned long long)nexact);
     if (COUNT) {
-      // per start walk: points, passes, leaves, slots, expansions, ticks
-      unsigned long long *h = A.counters + CT_N + 6 * hr.start;
+      // per start walk: points, passes, leaves, slots, expansions, ticks,
+      // slots tested, slots inside the shape
+      unsigned long long *h = A.counters + CT_N + 8 * hr.start;
       atomicAdd(&h[0], 1ull);
       if (verdict == 1) atomicAdd(&h[1], 1ull);
       atomicAdd(&h[2], (unsigned long long)c_leaf);
       atomicAdd(&h[3], (unsigned long long)c_pts);
       atomicAdd(&h[4], (unsigned long long)c_exp);
       atomicAdd(&h[5], (unsigned long long)(wall_clock64() - t_start));
+      atomicAdd(&h[6], (unsigned long long)c_test);
+      atomicAdd(&h[7], (unsigned long long)c_in);
     }
   }
 }
@@ -1915,8 +1942,8 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
     WB_CHECK(hipEventRecord(ev[4], st));
     static const bool count_work = getenv("WB_CLASSIFY_COUNTERS") != nullptr;
     if (count_work) {
-      dcount = (unsigned long long *)wb_alloc(c, (CT_N + 18) * 8);
-      WB_CHECK(hipMemsetAsync(dcount, 0, (CT_N + 18) * 8, st));
+      dcount = (unsigned long long *)wb_alloc(c, (CT_N + 24) * 8);
+      WB_CHECK(hipMemsetAsync(dcount, 0, (CT_N + 24) * 8, st));
     }
     A.counters = dcount;
     heavy = (HeavyRec *)wb_alloc(c, sizeof(HeavyRec) * (n + 1));
@@ -1995,14 +2022,15 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
   out->fallbacks = hc[7];
   out->nslow = h_nslow;
   if (dcount) {
-    unsigned long long h[CT_N + 18];
+    unsigned long long h[CT_N + 24];
     WB_CHECK(hipMemcpy(h, dcount, sizeof h, hipMemcpyDeviceToHost));
     for (int s = 0; s < 3; s++)
       fprintf(stderr,
               "[wb counters] heavy from walk %c: points %llu passed %llu leaves %llu slots %llu "
-              "expansions %llu ticks %llu\n",
-              "ABC"[s], h[CT_N + 6 * s], h[CT_N + 6 * s + 1], h[CT_N + 6 * s + 2],
-              h[CT_N + 6 * s + 3], h[CT_N + 6 * s + 4], h[CT_N + 6 * s + 5]);
+              "expansions %llu ticks %llu tested %llu inside %llu\n",
+              "ABC"[s], h[CT_N + 8 * s], h[CT_N + 8 * s + 1], h[CT_N + 8 * s + 2],
+              h[CT_N + 8 * s + 3], h[CT_N + 8 * s + 4], h[CT_N + 8 * s + 5],
+              h[CT_N + 8 * s + 6], h[CT_N + 8 * s + 7]);
     fprintf(stderr,
             "[wb counters] lanes A %llu B %llu open after B %llu heavy %llu | A: leaf-lanes %llu "
             "group-lanes %llu dirs %llu | B: leaf-lanes %llu point-lanes %llu dirs %llu | per "
@@ -2018,7 +2046,7 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
             h[CT_TSUM], h[CT_TMAX], rate, h[CT_THIST], h[CT_THIST + 1], h[CT_THIST + 2],
             h[CT_THIST + 3], h[CT_THIST + 4], h[CT_TSUMS], h[CT_TSUMS + 1], h[CT_TSUMS + 2],
             h[CT_TSUMS + 3], h[CT_TSUMS + 4]);
-    wb_free(c, dcount, (CT_N + 18) * 8);
+    wb_free(c, dcount, (CT_N + 24) * 8);
   }
   out->ms_tiles = ms1; out->ms_scan = ms2; out->ms_classify = ms3;
   wb_free(c, grid, 4 * gcells);

@@ -382,16 +382,27 @@ WB_HD int atan2i(double y, double x, TabPtr tab_tan, TabPtr tab_cos, TabPtr tab_
 
 // ---------------------------------------------------------------------------
 // 64-bin direction index of atan2i(dy, dx): the bin of the circle position
-// (a + 2^30) mod 2^31, 2^25 units per bin.  A float atan2 decides it unless
-// the direction lies within DIR_EPS units of a bin edge.  The float path's
-// error is bounded by the two float conversions of dy and dx (2^-24 relative
-// each, at most 2^-23 rad), the atan2f result (a few ulp of at most pi) and
-// the scaling to binary units -- together well under 2^10 units against
-// DIR_EPS = 2^16 -- and atan2i itself is within one unit of the true angle,
-// so outside the edge band both give the same bin.  Inside it, or for
-// operands a float cannot carry, atan2i decides.
+// (a + 2^30) mod 2^31, 2^25 units per bin, i.e. floor((theta + pi) * 32/pi)
+// for the angle theta of (dx, dy).  A float path decides it unless the
+// direction lies within DIR_MARGIN bins (2^-14, 2048 units, 6e-6 rad) of a
+// bin edge:
+//   t   = min(|dx|, |dy|) / max(|dx|, |dy|) in [0, 1] (the octant ratio),
+//   psi = t * P(t^2) ~ atan(t) in [0, pi/4], P of degree 6 fitted by
+//         minimax (Lawson's iteratively reweighted least squares) to atan
+//         on [0, 1]: 2.5e-7 rad, 3.2e-7 rad evaluated in float with fma,
+//   s   = psi * 32/pi, the position in the octant's eight bins,
+// and the octant folds s onto the circle: the bin in the quadrant is k or
+// 15 - k (k = floor(s); |dy| > |dx| mirrors), the quadrants of opposite signs
+// mirror again (31 - b) and dy > 0 adds 32.  The float path's error: the
+// conversions of dx, dy (2^-24 relative each), the quotient (the device's
+// v_rcp_f32 is within 1 ulp; the host divides) and the products add at most
+// 3e-7 rad to the fit's 3.2e-7 -- against a 6e-6 rad margin -- and atan2i
+// itself is within one unit (3e-9 rad) of the true angle, so outside the
+// edge band both give the same bin.  Inside it (which holds every direction
+// on an axis or a diagonal), or for operands a float cannot carry, atan2i
+// decides.
 // ---------------------------------------------------------------------------
-constexpr int DIR_EPS = 1 << 16;
+constexpr float DIR_MARGIN = 0x1p-14f;
 
 // atan2i out of line; returns the angle in the low 32 bits and the number of
 // exact last steps taken (0 or 1) in bit 32
@@ -419,13 +430,32 @@ WB_HD int dir_bin(double dy, double dx, const double *tab_tan, const double *tab
 {
   const double m = dabs(dy) > dabs(dx) ? dabs(dy) : dabs(dx);
   if (m > 1e-30 && m < 1e30) {
-    const float t = atan2f((float)dy, (float)dx);          // [-pi, pi]
-    double pos = (double)t * (1073741824.0 / 3.14159265358979323846) + 1073741824.0;
-    if (pos >= 2147483648.0) pos -= 2147483648.0;
-    if (pos < 0) pos += 2147483648.0;
-    const int64_t ip = (int64_t)pos;
-    const int64_t frac = ip & ((1 << 25) - 1);
-    if (frac > DIR_EPS && frac < (1 << 25) - DIR_EPS) return (int)((ip >> 25) & 63);
+    const float fx = (float)dx, fy = (float)dy;
+    const float ax = fabsf(fx), ay = fabsf(fy);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+#ifdef __HIP_DEVICE_COMPILE__
+    const float t = mn * __builtin_amdgcn_rcpf(mx);
+#else
+    const float t = mn / mx;
+#endif
+    const float u = t * t;
+    float p = 0.006811794359236956f;
+    p = fmaf(p, u, -0.033604223281145096f);
+    p = fmaf(p, u, 0.07962367683649063f);
+    p = fmaf(p, u, -0.1323334276676178f);
+    p = fmaf(p, u, 0.19807815551757812f);
+    p = fmaf(p, u, -0.3331736922264099f);
+    p = fmaf(p, u, 0.9999961256980896f);
+    const float s = (p * t) * 10.185916357881302f;  // 32/pi
+    const float fl = floorf(s);
+    const float fr = s - fl;  // exact
+    if (fr > DIR_MARGIN && fr < 1.0f - DIR_MARGIN && s < 8.0f) {
+      const uint32_t ux = __builtin_bit_cast(uint32_t, fx), uy = __builtin_bit_cast(uint32_t, fy);
+      int b = (int)fl;
+      if (ay > ax) b ^= 15;
+      if ((ux ^ uy) >> 31) b ^= 31;
+      return b + (fy > 0 ? 32 : 0);
+    }
   }
   const int a = atan2i_ext(dy, dx, tab_tan, tab_cos, tab_sin, K, exact_used);
   return (int)((((uint32_t)a + 0x40000000u) & 0x7fffffffu) >> 25);
@@ -680,14 +710,17 @@ WB_HD void fill_edge_table(float *tab)
 // The care wedges of occ from the edge table: a run of bins [s, s+len) is
 // the wedge from edge s turned back by 1e-3 rad to edge s+len turned on by
 // 1e-3 rad; a run of 32 bins or more spans more than pi with the margins and
-// is split at its middle edge into two convex wedges.  Run r fills slots 2r
-// and, when split, 2r+1 (fixed slots keep every index a constant).
+// is split at its middle edge m into two wedges, [s, m] and [m, s+len], each
+// with the margin on both sides: at most 31 bins plus 2e-3 rad, so convex
+// (wedge_hit's test needs that).  A run of 63 bins (one occupied bin) has no
+// such split and is not pruned.  Run r fills slots 2r and, when split, 2r+1
+// (fixed slots keep every index a constant).
 WB_HD void care_wedges(uint64_t occ, Wedges &w, const float *tab)
 {
   w.live = -1;
   int rs[2], rl[2];
   const int nr = care_runs(occ, rs, rl);
-  if (nr < 0) return;
+  if (nr < 0 || (nr > 0 && rl[0] >= 63)) return;
   int live = 0;
 #pragma unroll
   for (int r = 0; r < 2; r++) {
@@ -695,13 +728,12 @@ WB_HD void care_wedges(uint64_t occ, Wedges &w, const float *tab)
       const int s = rs[r], len = rl[r], e = (s + len) & 63;
       const bool split = len >= 32;
       const int m = split ? (s + len / 2) & 63 : e;
-      const int sm = split ? 0 : 1;
       w.ax[2 * r] = tab[2 * s];
       w.ay[2 * r] = tab[2 * s + 1];
-      w.bx[2 * r] = tab[(sm + 1) * 128 + 2 * m];
-      w.by[2 * r] = tab[(sm + 1) * 128 + 2 * m + 1];
-      w.ax[2 * r + 1] = tab[128 + 2 * m];
-      w.ay[2 * r + 1] = tab[128 + 2 * m + 1];
+      w.bx[2 * r] = tab[256 + 2 * m];
+      w.by[2 * r] = tab[256 + 2 * m + 1];
+      w.ax[2 * r + 1] = tab[2 * m];
+      w.ay[2 * r + 1] = tab[2 * m + 1];
       w.bx[2 * r + 1] = tab[256 + 2 * e];
       w.by[2 * r + 1] = tab[256 + 2 * e + 1];
       live |= (split ? 3 : 1) << (2 * r);
