@@ -41,6 +41,7 @@
 #include <stdio.h>
 
 #define WBO_RECORDS 537            /* octree.h:40-41 */
+#define WBO_MAX_LEVEL 21           /* deepest level of the canonical tree */
 #define DEG30  0x0aaaaaab           /* angle.h:105 */
 #define DEG45  0x10000000
 #define DEG90  0x20000000
@@ -469,6 +470,7 @@ typedef struct wbo_tree {
   oleaf *leaves; int nleaves, capleaves;
   long *order;          /* point indices in DFS order, input order in leaf */
   int maxdepth;
+  int failed;           /* a level-21 cell holds more than RECORDS points */
 } wbo_tree;
 
 static int new_node(wbo_tree *t, const double *c, double side)
@@ -526,6 +528,15 @@ static void build_rec(wbo_tree *t, int nd, long *idx, long cnt, long *tmp, int d
     double cc[3], cs;
     if (m == 0) continue;
     child_cube(c, side, i, cc, &cs);
+    if (m > WBO_RECORDS && depth + 1 >= WBO_MAX_LEVEL) {
+      /* The reference splits once more and drops the point it was putting
+       * (OctStore::put, octree.cpp:849-876, ignores the second put's
+       * result), once per further point: a tree that depends on the order
+       * of arrival.  The canonical tree stops at level 21 (the cell index
+       * of the GPU build has 21 bits per axis) and the build fails. */
+      t->failed = 1;
+      continue;
+    }
     if (m > WBO_RECORDS) {
       int ch = new_node(t, cc, cs);
       t->nodes[nd].child[i] = ch;
@@ -575,6 +586,7 @@ void wbo_free(wbo_tree *t)
 int wbo_nleaves(const wbo_tree *t) { return t->nleaves; }
 int wbo_nnodes(const wbo_tree *t) { return t->nnodes; }
 int wbo_maxdepth(const wbo_tree *t) { return t->maxdepth; }
+int wbo_failed(const wbo_tree *t) { return t->failed; }
 
 void wbo_leaves(const wbo_tree *t, double *cen, double *side, int *depth,
                 long *first, long *count)
@@ -1009,6 +1021,13 @@ static long run_impl(const int32_t *xi, const int32_t *yi, const int32_t *zi, co
   }
   wbo_sizefit(corners, ncorners, center, &side);
   t = wbo_build(x, y, z, n, center, side);
+  if (t->failed) {
+    wbo_free(t);
+    if (!x_out) free(x);
+    if (!y_out) free(y);
+    if (!z_out) free(z);
+    return -1;
+  }
   wbo_snake_setup_corners(corners, ncorners, pr->tile_size, &sn);
   ext[0] = ext[2] = INFINITY;
   ext[1] = ext[3] = -INFINITY;

@@ -218,10 +218,12 @@ def pipeline(images, params=None, unit=1.0, zeros="read", name="out", separate=T
                                         params.tile_size, C.byref(sn))
         tiles = (O.Tile * (sn.hi - sn.lo + 1))()
         st = O.Stats()
-        L.wbo_run_xyz(xd.ctypes.data_as(_dp), yd.ctypes.data_as(_dp), zd.ctypes.data_as(_dp),
-                      len(xd), res.ctypes.data_as(_dp), corners.ctypes.data_as(_dp),
-                      len(hdrs) * 2, C.byref(params), cls_d.ctypes.data_as(_u8p), tiles,
-                      C.byref(st))
+        rc = L.wbo_run_xyz(xd.ctypes.data_as(_dp), yd.ctypes.data_as(_dp),
+                           zd.ctypes.data_as(_dp), len(xd), res.ctypes.data_as(_dp),
+                           corners.ctypes.data_as(_dp), len(hdrs) * 2, C.byref(params),
+                           cls_d.ctypes.data_as(_u8p), tiles, C.byref(st))
+        if rc < 0:
+            raise ValueError("more than 537 points in a level-21 cell")
     # class bytes in cloud order: a merged point takes its location's class
     rank = np.full(n, -1, dtype=np.int64)
     rank[distinct] = np.arange(len(distinct))

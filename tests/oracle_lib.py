@@ -80,6 +80,8 @@ def lib():
         L.wbo_intersect_flat.argtypes = [C.c_int, _dp, _dp, C.c_double]
         L.wbo_sizefit.argtypes = [_dp, C.c_int, _dp, _dp]
         L.wbo_build.restype = C.c_void_p
+        L.wbo_failed.argtypes = [C.c_void_p]
+        L.wbo_failed.restype = C.c_int
         L.wbo_build.argtypes = [_dp, _dp, _dp, C.c_long, _dp, C.c_double]
         L.wbo_free.argtypes = [C.c_void_p]
         L.wbo_nleaves.argtypes = [C.c_void_p]
@@ -250,6 +252,8 @@ class Tree:
         self.h = lib().wbo_build(_ptr(self.x, _dp), _ptr(self.y, _dp), _ptr(self.z, _dp),
                                  len(self.x), _ptr(c, _dp), float(side))
         self.n = len(self.x)
+        if lib().wbo_failed(self.h):
+            raise ValueError("more than 537 points in a level-21 cell")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -323,11 +327,13 @@ def run(cloud, params=None, want_xyz=False):
     x = np.zeros(n) if want_xyz else None
     y = np.zeros(n) if want_xyz else None
     z = np.zeros(n) if want_xyz else None
-    lib().wbo_run(xi.ctypes.data_as(_i32p), yi.ctypes.data_as(_i32p), zi.ctypes.data_as(_i32p),
+    rc = lib().wbo_run(xi.ctypes.data_as(_i32p), yi.ctypes.data_as(_i32p), zi.ctypes.data_as(_i32p),
                   n, _ptr(sc, _dp), _ptr(of, _dp), _ptr(bmin, _dp), _ptr(bmax, _dp),
                   C.byref(params), cls.ctypes.data_as(_u8p), tiles, C.byref(st),
                   _ptr(x, _dp) if want_xyz else None, _ptr(y, _dp) if want_xyz else None,
                   _ptr(z, _dp) if want_xyz else None)
+    if rc < 0:
+        raise ValueError("more than 537 points in a level-21 cell")
     tarr = np.ctypeslib.as_array(tiles)
     out = (cls, tarr.copy(), st)
     if want_xyz:

@@ -157,6 +157,54 @@ def test_classify_matches_oracle(ctx, cfg):
         assert ((m > 64) & (m <= 128)).any() and ((m > 128) & (m <= 256)).any() and (m > 256).any()
 
 
+def header_box(cloud, lo_frac, hi_frac):
+    """A header box over the middle of the cloud in x and y and the 20th to
+    90th z percentiles: a LAS header whose bounds do not cover its records."""
+    lo, hi = np.array(cloud["bbmin"], float), np.array(cloud["bbmax"], float)
+    ext = hi - lo
+    _, _, z = decode(cloud)
+    c = dict(cloud)
+    c["bbmin"] = [lo[0] + lo_frac * ext[0], lo[1] + lo_frac * ext[1], float(np.percentile(z, 20))]
+    c["bbmax"] = [lo[0] + hi_frac * ext[0], lo[1] + hi_frac * ext[1], float(np.percentile(z, 90))]
+    return c
+
+
+@pytest.mark.parametrize("seed", [31, 32])
+def test_classify_points_outside_header_box(ctx, seed):
+    # the root cube and the tiling come from the box; the points outside the
+    # root cube sit in its extreme cells, far outside their leaves' cubes,
+    # and still count as neighbours of the points inside
+    cloud = header_box(synth.generate(seed=seed, extent=40.0, density=20.0, embankments=1,
+                                      outliers=0.01), 0.2, 0.75)
+    tree, _, (x, y, z) = build_both(ctx, cloud)
+    center, side = W.Octree.size_fit(cloud["bbmin"], cloud["bbmax"])
+    out = ((np.abs(x - center[0]) > side / 2) | (np.abs(y - center[1]) > side / 2) |
+           (np.abs(z - center[2]) > side / 2))
+    assert out.sum() > 10000
+    cls, tiles, st = W.classify(tree, cloud["bbmin"], cloud["bbmax"])
+    rcls, rtiles, rst = O.run(cloud)
+    print("outside", int(out.sum()), "stats", st, "oracle", rst.n_class1, rst.n_class2,
+          rst.n_unclassified)
+    assert np.array_equal(tiles["npoints"], rtiles["npoints"])
+    assert np.array_equal(tiles["hs"], rtiles["hs"])
+    assert int((cls != rcls).sum()) == 0
+    assert rst.n_class2 > 0 and rst.n_class1 > 0
+
+
+def test_points_outside_root_cube_pile_up(ctx):
+    # a smaller box: more than 537 points outside one corner of the root
+    # cube follow the same extreme path to level 21; neither tree exists
+    cloud = header_box(synth.generate(seed=32, extent=40.0, density=20.0, embankments=1,
+                                      outliers=0.01), 0.3, 0.65)
+    center, side = W.Octree.size_fit(cloud["bbmin"], cloud["bbmax"])
+    with pytest.raises(W.WbError, match="21 levels"):
+        W.Octree.build(ctx, cloud["xi"], cloud["yi"], cloud["zi"], cloud["scale"],
+                       cloud["offset"], center, side)
+    x, y, z = decode(cloud)
+    with pytest.raises(ValueError):
+        O.Tree(x, y, z, center, side)
+
+
 def test_context_close_frees_live_trees():
     # a tree still alive when its context closes is freed with it; freeing
     # it afterwards is a no-op, not a use of the closed context

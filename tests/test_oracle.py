@@ -314,6 +314,31 @@ def test_octree_split_boundaries():
         assert lv["count"].sum() == n
 
 
+def test_octree_points_outside_root_cube():
+    # points outside the root cube follow the compares to its extreme cells:
+    # 537 of them beyond one corner are one leaf; 538 would need a level-21
+    # cell of more than 537 points, which the canonical tree refuses; points
+    # beyond one face but spread along it split as usual
+    rng = np.random.default_rng(8)
+    for n in (537, 538):
+        x, y, z = -50 - rng.random(n), -50 - rng.random(n), -50 - rng.random(n)
+        if n == 537:
+            t = O.Tree(x, y, z, [0.0, 0.0, 0.0], 64.0)
+            lv = t.leaves()
+            assert t.nleaves == 1 and lv["depth"][0] == 1 and lv["count"][0] == 537
+        else:
+            with pytest.raises(ValueError):
+                O.Tree(x, y, z, [0.0, 0.0, 0.0], 64.0)
+    n = 5000
+    x = -50 - rng.random(n)
+    y, z = rng.uniform(-32, 32, n), rng.uniform(-32, 32, n)
+    t = O.Tree(x, y, z, [0.0, 0.0, 0.0], 64.0)
+    lv = t.leaves()
+    assert lv["count"].sum() == n and lv["count"].max() <= 537
+    # every leaf lies on the root's -x face
+    assert np.all(lv["center"][:, 0] - lv["side"] / 2 == -32.0)
+
+
 # ------------------------------------------------------- ground test ----
 def dir_deg(deg):
     return int(round(deg / 360 * 2 ** 31)) - (2 ** 31 if deg >= 180 else 0)

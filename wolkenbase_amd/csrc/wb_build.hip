@@ -340,12 +340,20 @@ __device__ inline void load_packed(const Src &s, const Xform &f, uint64_t i, dou
 }
 
 template <bool F64>
-__global__ void k_keygen(Src src, uint64_t n, Xform f, uint64_t *key, uint32_t *val)
+__global__ void k_keygen(Src src, uint64_t n, Xform f, uint64_t *key, uint32_t *val,
+                         uint32_t *outside)
 {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double x, y, z;
   load_point<F64>(src, f, i, x, y, z);
+  // a point outside the closed root cube (or not a number) lands in an
+  // extreme cell, possibly far outside its leaf's cube
+  const double side = f.cell * (double)(1 << WB_MAX_LEVEL);
+  const bool out = !(x >= f.lox && x <= f.lox + side && y >= f.loy && y <= f.loy + side &&
+                     z >= f.loz && z <= f.loz + side);
+  const uint64_t ob = __ballot(out);
+  if (ob && (int)__lane_id() == __ffsll((long long)ob) - 1) atomicOr(outside, 1u);
   if (F64) ((double4 *)src.packed)[i] = make_double4(x, y, z, 0.0);
   else ((int4 *)src.packed)[i] = make_int4(src.xi[i], src.yi[i], src.zi[i], 0);
   uint32_t ix = cell_index(x, f.lox, f.cell, f.inv_cell);
@@ -730,6 +738,9 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
   LeafTmp *lt = nullptr;
   SplitOut *d_out = (SplitOut *)wb_alloc(c, sizeof(SplitOut));
   uint32_t *d_cnt = (uint32_t *)wb_alloc(c, 4 * (WB_MAX_LEVEL + 2));
+  uint32_t *d_outside = (uint32_t *)wb_alloc(c, 4);
+  static const uint32_t zero_word = 0;
+  uint32_t h_outside = 0;
   SplitOut h_out;
   Active *act = (Active *)wb_alloc(c, sizeof(Active) * node_cap);
   Active *nxt = (Active *)wb_alloc(c, sizeof(Active) * node_cap);
@@ -748,12 +759,13 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
     t->dup_pairs = nullptr;
     if (n > 0) {
       WB_CHECK(hipEventRecord(evb[0], c->stream));
+      WB_CHECK(hipMemcpyAsync(d_outside, &zero_word, 4, hipMemcpyHostToDevice, c->stream));
       if (f64)
         hipLaunchKernelGGL(k_keygen<true>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, src,
-                           n, f, t->key, t->sidx);
+                           n, f, t->key, t->sidx, d_outside);
       else
         hipLaunchKernelGGL(k_keygen<false>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, src,
-                           n, f, t->key, t->sidx);
+                           n, f, t->key, t->sidx, d_outside);
       WB_CHECK(hipEventRecord(evb[1], c->stream));
       wb_radix_sort_pairs(c, t->key, t->sidx, n, passes);
       WB_CHECK(hipEventRecord(evb[2], c->stream));
@@ -822,17 +834,29 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
       std::swap(act, nxt);
     }
     WB_CHECK(hipMemcpyAsync(&h_out, d_out, sizeof h_out, hipMemcpyDeviceToHost, c->stream));
+    WB_CHECK(hipMemcpyAsync(&h_outside, d_outside, 4, hipMemcpyDeviceToHost, c->stream));
     WB_CHECK(hipStreamSynchronize(c->stream));
+    t->outside_root = n > 0 ? h_outside : 0;
+    // the split's scratch goes back to the cache before an error leaves
+    auto release = [&]() {
+      wb_free(c, src.packed, pk_bytes);
+      wb_free(c, act, sizeof(Active) * node_cap);
+      wb_free(c, nxt, sizeof(Active) * node_cap);
+      wb_free(c, lt, sizeof(LeafTmp) * leaf_cap);
+      wb_free(c, d_out, sizeof(SplitOut));
+      wb_free(c, d_cnt, 4 * (WB_MAX_LEVEL + 2));
+      wb_free(c, d_outside, 4);
+    };
     if (h_out.overflow & 1u) {
       if (passes >= 8) {
-        wb_free(c, src.packed, pk_bytes);
+        release();
         throw std::string("octree deeper than 21 levels (more than 537 points in a level-21 cell)");
       }
       passes++;
       continue;  // the sort did not reach deep enough: redo with one more digit
     }
     if (h_out.overflow) {
-      wb_free(c, src.packed, pk_bytes);
+      release();
       throw std::string("octree node/leaf capacity exceeded");
     }
     break;
@@ -845,6 +869,7 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
   t->maxdepth = (int)h_out.maxdepth;
   wb_free(c, d_out, sizeof(SplitOut));
   wb_free(c, d_cnt, 4 * (WB_MAX_LEVEL + 2));
+  wb_free(c, d_outside, 4);
 
   // leaves in DFS order = ascending start
   const uint32_t nl = t->nleaves;

@@ -1091,6 +1091,10 @@ struct ClassifyArgs {
   const uint32_t *nmemb;
   const TileRec *rec;
   double thickness, max_slope;
+  // 1: every point lies in its leaf's cube, so the care wedges may skip a
+  // node or leaf by its cube; 0 (a point outside the root cube): only the
+  // 32-point groups' boxes, which bound the points themselves, are tested
+  int cube_wedges;
   const double *tab;     // tan[511], cos[512], sin[512] packed
   const float *edges;    // bin-edge directions (wb::fill_edge_table)
   wb::AtanConst K;
@@ -1222,7 +1226,7 @@ __device__ inline int child_rank(int k, int o, int zf) { return ((k ^ o) & 3) | 
 // visiting order.  Returns the new stack depth.
 __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int code, bool on,
                                   const wb::Shape &H, double rx, double ry, int zf,
-                                  const LaneWedges &LW)
+                                  const LaneWedges &LW, bool cube_wedges)
 {
   const double *nc = T.node_cube + 4 * (size_t)code;
   const double cx = nc[0], cy = nc[1], cz = nc[2], side = nc[3];
@@ -1240,8 +1244,9 @@ __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int c
       const double ncy = cy + (double)(2 * ((k & 2) / 2) - 1) * side / 4;
       const double ncz = cz + (double)(2 * ((k & 4) / 4) - 1) * side / 4;
       const bool h = on && wb::hyp_intersect(H, ncx, ncy, ncz, side / 2) &&
-                     lane_hit(LW, threadIdx.x, H.cx, H.cy, ncx - side / 4, ncx + side / 4,
-                              ncy - side / 4, ncy + side / 4);
+                     (!cube_wedges ||
+                      lane_hit(LW, threadIdx.x, H.cx, H.cy, ncx - side / 4, ncx + side / 4,
+                               ncy - side / 4, ncy + side / 4));
       hits |= (h ? 1u : 0u) << k;
       if (__ballot(h)) pushed |= 1u << child_rank(k, o, zf);
     }
@@ -1365,7 +1370,7 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
           if (phase == 0) cnt[CT_EXP_A]++;
           else cnt[CT_EXP_B]++;
         }
-        sp = expand_node(T, S, sp, code, on, H, rx, ry, zf, LW);
+        sp = expand_node(T, S, sp, code, on, H, rx, ry, zf, LW, A.cube_wedges != 0);
         continue;
       }
       const int leaf = -2 - code;
@@ -1374,8 +1379,8 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
       // the lane skips a leaf outside its care wedges
       const double *lc = T.leaf_cube + 4 * (size_t)leaf;
       const double lh = lc[3] / 2;
-      const bool onl = on && lane_hit(LW, threadIdx.x, x, y, lc[0] - lh, lc[0] + lh,
-                                      lc[1] - lh, lc[1] + lh);
+      const bool onl = on && (!A.cube_wedges || lane_hit(LW, threadIdx.x, x, y, lc[0] - lh,
+                                                         lc[0] + lh, lc[1] - lh, lc[1] + lh));
       if (phase == 0) {
         if (COUNT) {
           cnt[CT_WLEAF_A]++;
@@ -1590,7 +1595,8 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
           const double ncy = cy + (double)(2 * ((k & 2) / 2) - 1) * side / 4;
           const double ncz = cz + (double)(2 * ((k & 4) / 4) - 1) * side / 4;
           h = wb::hyp_intersect(H, ncx, ncy, ncz, side / 2) &&
-              wb::wedge_hit(wg, x, y, ncx - side / 4, ncx + side / 4, ncy - side / 4, ncy + side / 4);
+              (!A.cube_wedges || wb::wedge_hit(wg, x, y, ncx - side / 4, ncx + side / 4,
+                                               ncy - side / 4, ncy + side / 4));
         }
         const int t = child_rank(k, o, zf);
         uint32_t rn = h && ck >= 0 ? 1u << t : 0u, rl = h && ck < -1 ? 1u << t : 0u;
@@ -1627,7 +1633,8 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
           const double *lc = T.leaf_cube + 4 * (size_t)leaf;
           const double h2 = lc[3] / 2;
           // the wedges may have narrowed since the leaf was collected
-          const bool seen = wb::wedge_hit(wg, x, y, lc[0] - h2, lc[0] + h2, lc[1] - h2, lc[1] + h2);
+          const bool seen =
+              !A.cube_wedges || wb::wedge_hit(wg, x, y, lc[0] - h2, lc[0] + h2, lc[1] - h2, lc[1] + h2);
           if (phase != 0 && seen) {
             const bool inside = wb::hyp_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
             mask = inside ? 0x80000000u : 0u;
@@ -1936,6 +1943,7 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
     ClassifyArgs A;
     A.T = T; A.g = g; A.memb = memb; A.nmemb = nmemb; A.rec = rec;
     A.thickness = p->thickness; A.max_slope = p->max_slope;
+    A.cube_wedges = t->outside_root ? 0 : 1;
     A.tab = c->d_tan; A.K = c->kconst; A.cls = cls; A.nslow = nslow;
     A.edges = c->d_edges;
     A.atan_exact = ctr + 1;

@@ -59,6 +59,10 @@ struct wb_tree {
   uint64_t n;        // distinct locations in the tree
   uint64_t n_input;  // input points (n plus the duplicates merged away)
   uint64_t ndup;     // inputs merged into an earlier point at the same location
+  // nonzero when some point lies outside the closed root cube: such a point
+  // sits in an extreme cell, possibly far outside its leaf's cube, so no
+  // test may bound a leaf's points by the leaf's cube
+  uint32_t outside_root;
   double center[3], side;
   double scale[3], offset[3];
   int sort_passes, maxdepth;
