@@ -124,6 +124,11 @@ def test_dir_bin_matches_exact_bin(hm):
                          np.zeros(10), rng.normal(size=10), sy * d])
     xs = np.concatenate([rng.normal(size=n), rng.integers(-4000, 4000, n) * 0.001, r * np.cos(ang),
                          rng.normal(size=10), np.zeros(10), sx * d])
+    # and magnitudes a float cannot carry, or only as a subnormal
+    k = rng.integers(0, 2000, 400)
+    sc = np.array([1e-45, 1e-38, 1e-31, 1e31, 1e38, 1e45])[rng.integers(0, 6, 400)]
+    ys = np.concatenate([ys, ys[k] * sc, ys[k] * sc[::-1]])
+    xs = np.concatenate([xs, xs[k] * sc, xs[k]])
     bad = 0
     for y, x in zip(ys, xs):
         if x == 0 and y == 0:

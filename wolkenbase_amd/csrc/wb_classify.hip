@@ -1404,7 +1404,7 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
           const uint32_t ns = min((uint32_t)WAVE, ng - gb);
           for (uint32_t s = 0; s < ns; s++) {
             const double qx = rdl(sx, s), qy = rdl(sy, s), qz = rdl(sz, s);
-            if (onl && wb::hyp_in(H, qx, qy, qz) && (qx != x || qy != y)) {
+            if (onl & wb::hyp_in(H, qx, qy, qz) & ((qx != x) | (qy != y))) {
               if (COUNT) cnt[CT_DIR_A]++;
               occ |= 1ull << wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
             }
@@ -1434,7 +1434,7 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
             if (COUNT && chit) cnt[CT_POINT_B] += ce - c0;
             for (uint32_t s = 0; s < ce - c0; s++) {
               const double qx = rdl(sx, s), qy = rdl(sy, s), qz = rdl(sz, s);
-              if (chit && (all || wb::hyp_in(H, qx, qy, qz)) && (qx != x || qy != y)) {
+              if (chit & (all | wb::hyp_in(H, qx, qy, qz)) & ((qx != x) | (qy != y))) {
                 if (COUNT) cnt[CT_DIR_B]++;
                 occ |= 1ull << wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
               }

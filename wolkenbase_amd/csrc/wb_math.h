@@ -399,8 +399,8 @@ WB_HD int atan2i(double y, double x, TabPtr tab_tan, TabPtr tab_cos, TabPtr tab_
 // 3e-7 rad to the fit's 3.2e-7 -- against a 6e-6 rad margin -- and atan2i
 // itself is within one unit (3e-9 rad) of the true angle, so outside the
 // edge band both give the same bin.  Inside it (which holds every direction
-// on an axis or a diagonal), or for operands a float cannot carry, atan2i
-// decides.
+// on an axis or a diagonal), or when max(|dx|, |dy|) as a float lies outside
+// [2^-100, 2^100], atan2i decides.
 // ---------------------------------------------------------------------------
 constexpr float DIR_MARGIN = 0x1p-14f;
 
@@ -428,8 +428,7 @@ WB_HD int atan2i_ext(double y, double x, const double *tab_tan, const double *ta
 WB_HD int dir_bin(double dy, double dx, const double *tab_tan, const double *tab_cos,
                   const double *tab_sin, const AtanConst &K, int *exact_used = nullptr)
 {
-  const double m = dabs(dy) > dabs(dx) ? dabs(dy) : dabs(dx);
-  if (m > 1e-30 && m < 1e30) {
+  {
     const float fx = (float)dx, fy = (float)dy;
     const float ax = fabsf(fx), ay = fabsf(fy);
     const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
@@ -449,7 +448,10 @@ WB_HD int dir_bin(double dy, double dx, const double *tab_tan, const double *tab
     const float s = (p * t) * 10.185916357881302f;  // 32/pi
     const float fl = floorf(s);
     const float fr = s - fl;  // exact
-    if (fr > DIR_MARGIN && fr < 1.0f - DIR_MARGIN && s < 8.0f) {
+    // mx in [2^-100, 2^100]: both conversions normal or t below the margin
+    // (a NaN fails a compare and takes atan2i)
+    if (fr > DIR_MARGIN && fr < 1.0f - DIR_MARGIN && s < 8.0f && mx >= 0x1p-100f &&
+        mx <= 0x1p100f) {
       const uint32_t ux = __builtin_bit_cast(uint32_t, fx), uy = __builtin_bit_cast(uint32_t, fy);
       int b = (int)fl;
       if (ay > ax) b ^= 15;
@@ -539,23 +541,24 @@ WB_HD bool sph_in(const Shape &s, double qx, double qy, double qz)
   return sph_in_tail(dx, dy, dz, s.r);
 }
 
-// Hyperboloid::in, shape.cpp:127-135.
+// Hyperboloid::in, shape.cpp:127-135: the point is on the hyperboloid's
+// side of the vertex (below it for a positive slope) and
+// (zd^2 - slope^2 (dx^2 + dy^2)) - por^2 >= 0, decided from the estimate
+// unless it lies within 2^-44 (relative) of zero.  Written as one predicate
+// (no early returns), so the hot loops carry one exec-mask branch, the rare
+// exact tail.
 WB_HD bool hyp_in(const Shape &s, double qx, double qy, double qz)
 {
-  double zd = s.cz - qz;
-  if (s.slope > 0) {
-    if (!(zd > 0)) return false;
-  } else {
-    if (!(zd < 0)) return false;
-  }
-  double dx = s.cx - qx, dy = s.cy - qy;
-  double a = zd * zd;
-  double b = (dx * dx + dy * dy) * (s.slope * s.slope);
-  double v = a - b;
-  double m = 0x1p-44 * (a + b);
-  if (v - s.por2 > m) return true;
-  if (v - s.por2 < -m) return false;
-  return hyp_in_tail(dx, dy, a, s.slope, s.por2);
+  const double zd = s.cz - qz;
+  const bool side = zd * (s.slope > 0 ? 1.0 : -1.0) > 0;  // zd > 0, or zd < 0
+  const double dx = s.cx - qx, dy = s.cy - qy;
+  const double a = zd * zd;
+  const double b = (dx * dx + dy * dy) * (s.slope * s.slope);
+  const double d = (a - b) - s.por2;
+  const double m = 0x1p-44 * (a + b);
+  bool in = side & (d > m);
+  if (side & !(d > m) & !(d < -m)) in = hyp_in_tail(dx, dy, a, s.slope, s.por2);
+  return in;
 }
 
 WB_HD bool shape_in(const Shape &s, double qx, double qy, double qz)
