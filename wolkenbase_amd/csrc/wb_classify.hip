@@ -1224,6 +1224,13 @@ __device__ inline int child_rank(int k, int o, int zf) { return ((k ^ o) & 3) | 
 // Pushes the children of internal node `code` that intersect the
 // hyperboloid of a lane in `m` (this lane takes part iff `on`), in reverse
 // visiting order.  Returns the new stack depth.
+// Each child's test is wb::hyp_intersect on the child cube of Octree::cube
+// and the lane's care wedges on the child's xy square, with the shared parts
+// computed once: the clamped coordinate of the closest point takes one of
+// two values per axis (x from bit 0 of the child index, y from bit 1, z from
+// bit 2) and the two children of one xy square (bit 2 differs) share the
+// wedge test.  Every value is the same double operation on the same operands
+// as in the per-child form.
 __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int code, bool on,
                                   const wb::Shape &H, double rx, double ry, int zf,
                                   const LaneWedges &LW, bool cube_wedges)
@@ -1233,32 +1240,58 @@ __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int c
   const int *chp = T.node_child + 8 * (size_t)code;
   const int o = rfl((rx >= cx ? 1 : 0) | (ry >= cy ? 2 : 0));
   int ck[8];
-  uint32_t hits = 0;    // bit k: child k intersects this lane's hyperboloid
   uint32_t pushed = 0;  // bit t: the child of rank t is pushed
+  uint64_t mk[8];       // lanes for which child k is pushed
 #pragma unroll
   for (int k = 0; k < 8; k++) ck[k] = rfl(chp[k]);
+  // Octree::cube (octree.cpp:349-358) per axis bit, and the closest point of
+  // wb::hyp_intersect: clamp_axis on x and y, the bottom (top for a negative
+  // slope) face on z
+  const double cs = side / 2;
+  double ncx[2], ncy[2], qx[2], qy[2], qz[2];
+#pragma unroll
+  for (int b = 0; b < 2; b++) {
+    ncx[b] = cx + (double)(2 * b - 1) * side / 4;
+    ncy[b] = cy + (double)(2 * b - 1) * side / 4;
+    const double ncz = cz + (double)(2 * b - 1) * side / 4;
+    qx[b] = wb::clamp_axis(H.cx, ncx[b], cs);
+    qy[b] = wb::clamp_axis(H.cy, ncy[b], cs);
+    qz[b] = ncz;
+    if (H.slope > 0) qz[b] -= cs / 2;
+    if (H.slope < 0) qz[b] += cs / 2;
+  }
+  // the lane's wedges, read from LDS once; one test per xy square
+  bool wq[4] = {true, true, true, true};
+  if (cube_wedges && LW.n[threadIdx.x] >= 0) {
+    wb::Wedges w;
+    w.live = (1 << LW.n[threadIdx.x]) - 1;
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      w.ax[r] = LW.ax[r][threadIdx.x]; w.ay[r] = LW.ay[r][threadIdx.x];
+      w.bx[r] = LW.bx[r][threadIdx.x]; w.by[r] = LW.by[r][threadIdx.x];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if ((ck[q] != -1) | (ck[q + 4] != -1))
+        wq[q] = wb::wedge_hit(w, H.cx, H.cy, ncx[q & 1] - side / 4, ncx[q & 1] + side / 4,
+                              ncy[q >> 1] - side / 4, ncy[q >> 1] + side / 4);
+  }
 #pragma unroll
   for (int k = 0; k < 8; k++) {
+    mk[k] = 0;
     if (ck[k] != -1) {
-      const double ncx = cx + (double)(2 * (k & 1) - 1) * side / 4;
-      const double ncy = cy + (double)(2 * ((k & 2) / 2) - 1) * side / 4;
-      const double ncz = cz + (double)(2 * ((k & 4) / 4) - 1) * side / 4;
-      const bool h = on && wb::hyp_intersect(H, ncx, ncy, ncz, side / 2) &&
-                     (!cube_wedges ||
-                      lane_hit(LW, threadIdx.x, H.cx, H.cy, ncx - side / 4, ncx + side / 4,
-                               ncy - side / 4, ncy + side / 4));
-      hits |= (h ? 1u : 0u) << k;
-      if (__ballot(h)) pushed |= 1u << child_rank(k, o, zf);
+      const bool h = on & wq[k & 3] & wb::hyp_in(H, qx[k & 1], qy[(k >> 1) & 1], qz[k >> 2]);
+      mk[k] = __ballot(h);
+      if (mk[k]) pushed |= 1u << child_rank(k, o, zf);
     }
   }
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    const uint64_t mk = __ballot((hits >> k) & 1);
-    if (mk) {
+    if (mk[k]) {
       const int pos = sp + __popc(pushed >> (child_rank(k, o, zf) + 1));
       S.code[pos] = ck[k];
-      S.lo[pos] = (uint32_t)mk;
-      S.hi[pos] = (uint32_t)(mk >> 32);
+      S.lo[pos] = (uint32_t)mk[k];
+      S.hi[pos] = (uint32_t)(mk[k] >> 32);
     }
   }
   return sp + __popc(pushed);
@@ -1268,7 +1301,14 @@ __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int c
 enum {
   CT_LANES_A, CT_LANES_B, CT_LANES_C, CT_LEAF_A, CT_GROUP_A, CT_DIR_A, CT_LEAF_B, CT_POINT_B,
   CT_DIR_B, CT_EXP_A, CT_WLEAF_A, CT_EXP_B, CT_WLEAF_B, CT_HEAVY, CT_WAVES,
-  CT_TSUM, CT_TMAX, CT_THIST, CT_TSUMS = CT_THIST + 5, CT_N = CT_TSUMS + 5
+  CT_TSUM, CT_TMAX, CT_THIST, CT_TSUMS = CT_THIST + 5,
+  // shader cycles per wave in: walk A expansions, walk A leaves, walk B
+  // expansions, walk B leaves, wedge refreshes; wave-level sample and slot
+  // iterations of walks A and B
+  CT_CYC = CT_TSUMS + 5, CT_ITER_A = CT_CYC + 8, CT_ITER_B, CT_N,
+  // the counting kernels add into 64 copies (block index mod 64) so their
+  // own atomics do not slow the memory system they measure
+  CT_STRIDE = CT_N + 24, CT_COPIES = 64
 };
 
 // a lane whose sampled pass enters leaves with more than HEAVY_B_GROUPS
@@ -1309,6 +1349,7 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
   const double *ttan = A.tab, *tcos = A.tab + 511, *tsin = A.tab + 1023;
   const uint64_t t_start = COUNT ? wall_clock64() : 0;
   uint32_t cnt[CT_WAVES] = {};
+  uint64_t cyc[8] = {}, iter_a = 0, iter_b = 0;
   // lane state: 0 undecided, 1 not ground, 2 ground, 3 no tile; handed to
   // k_ground_heavy: 5 walk A cut short, 6 after walk A, 7 after walk B
   int state = 3;
@@ -1365,12 +1406,14 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
                           (uint32_t)rfl((int)S.lo[sp])) & alive;
       if (m == 0) continue;
       const bool on = (m >> lane) & 1;
+      const uint64_t c0t = COUNT ? clock64() : 0;
       if (code >= 0) {
         if (COUNT) {
           if (phase == 0) cnt[CT_EXP_A]++;
           else cnt[CT_EXP_B]++;
         }
         sp = expand_node(T, S, sp, code, on, H, rx, ry, zf, LW, A.cube_wedges != 0);
+        if (COUNT) cyc[phase == 0 ? 0 : 2] += clock64() - c0t;
         continue;
       }
       const int leaf = -2 - code;
@@ -1381,6 +1424,12 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
       const double lh = lc[3] / 2;
       const bool onl = on && (!A.cube_wedges || lane_hit(LW, threadIdx.x, x, y, lc[0] - lh,
                                                          lc[0] + lh, lc[1] - lh, lc[1] + lh));
+      uint64_t c2t = 0;
+      if (COUNT && phase == 0) {
+        __builtin_amdgcn_s_waitcnt(0);
+        c2t = clock64();
+        cyc[5] += c2t - c0t;
+      }
       if (phase == 0) {
         if (COUNT) {
           cnt[CT_WLEAF_A]++;
@@ -1401,13 +1450,25 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
             if (j < b || j >= e) j = g == 0 ? b : (g0 + g) * WB_CHUNK;
             sx = T.px[j]; sy = T.py[j]; sz = T.pz[j];
           }
+          if (COUNT) {
+            __builtin_amdgcn_s_waitcnt(0);
+            const uint64_t c3t = clock64();
+            cyc[6] += c3t - c2t;
+            c2t = c3t;
+          }
           const uint32_t ns = min((uint32_t)WAVE, ng - gb);
+          if (COUNT && __ballot(onl)) iter_a += ns;
           for (uint32_t s = 0; s < ns; s++) {
             const double qx = rdl(sx, s), qy = rdl(sy, s), qz = rdl(sz, s);
             if (onl & wb::hyp_in(H, qx, qy, qz) & ((qx != x) | (qy != y))) {
               if (COUNT) cnt[CT_DIR_A]++;
               occ |= 1ull << wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
             }
+          }
+          if (COUNT) {
+            const uint64_t c3t = clock64();
+            cyc[7] += c3t - c2t;
+            c2t = c3t;
           }
         }
       } else {
@@ -1431,6 +1492,7 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
             if (chit) chit = lane_hit(LW, threadIdx.x, x, y, bx[0], bx[1], bx[2], bx[3]);
           }
           if (__ballot(chit)) {
+            if (COUNT) iter_b += ce - c0;
             if (COUNT && chit) cnt[CT_POINT_B] += ce - c0;
             for (uint32_t s = 0; s < ce - c0; s++) {
               const double qx = rdl(sx, s), qy = rdl(sy, s), qz = rdl(sz, s);
@@ -1443,11 +1505,14 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
           c0 = ce;
         }
       }
+      const uint64_t c1t = COUNT ? clock64() : 0;
+      if (COUNT) cyc[phase == 0 ? 1 : 3] += c1t - c0t;
       if (state == 0 && occ != checked) {
         checked = occ;
         if (gap_verdict(occ) == 1) state = 1;
         else lane_wedges_update(occ, A.edges, LW, threadIdx.x);
       }
+      if (COUNT) cyc[4] += clock64() - c1t;
       if (phase == 0 && state == 0 && groups_a > HEAVY_A_GROUPS) state = 5;
       alive = __ballot(state == 0);
       if (alive == 0) break;
@@ -1486,15 +1551,19 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
       v[k] = t;
     }
     if (lane == 0) {
-      for (int k = 0; k < CT_WAVES; k++) atomicAdd(&A.counters[k], (unsigned long long)v[k]);
-      atomicAdd(&A.counters[CT_WAVES], 1ull);
+      unsigned long long *ctr = A.counters + (blockIdx.x % CT_COPIES) * CT_STRIDE;
+      for (int k = 0; k < CT_WAVES; k++) atomicAdd(&ctr[k], (unsigned long long)v[k]);
+      atomicAdd(&ctr[CT_WAVES], 1ull);
       const uint64_t dt = wall_clock64() - t_start;
-      atomicAdd(&A.counters[CT_TSUM], (unsigned long long)dt);
-      atomicMax(&A.counters[CT_TMAX], (unsigned long long)dt);
+      atomicAdd(&ctr[CT_TSUM], (unsigned long long)dt);
+      atomicMax(&ctr[CT_TMAX], (unsigned long long)dt);
       // wave times in ticks: <1e3, <1e4, <1e5, <1e6, >=1e6
       const int hb = dt < 1000 ? 0 : dt < 10000 ? 1 : dt < 100000 ? 2 : dt < 1000000 ? 3 : 4;
-      atomicAdd(&A.counters[CT_THIST + hb], 1ull);
-      atomicAdd(&A.counters[CT_TSUMS + hb], (unsigned long long)dt);
+      atomicAdd(&ctr[CT_THIST + hb], 1ull);
+      atomicAdd(&ctr[CT_TSUMS + hb], (unsigned long long)dt);
+      for (int k = 0; k < 8; k++) atomicAdd(&ctr[CT_CYC + k], (unsigned long long)cyc[k]);
+      atomicAdd(&ctr[CT_ITER_A], (unsigned long long)iter_a);
+      atomicAdd(&ctr[CT_ITER_B], (unsigned long long)iter_b);
     }
   }
 }
@@ -1760,7 +1829,8 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
     if (COUNT) {
       // per start walk: points, passes, leaves, slots, expansions, ticks,
       // slots tested, slots inside the shape
-      unsigned long long *h = A.counters + CT_N + 8 * hr.start;
+      unsigned long long *h =
+          A.counters + (blockIdx.x % CT_COPIES) * CT_STRIDE + CT_N + 8 * hr.start;
       atomicAdd(&h[0], 1ull);
       if (verdict == 1) atomicAdd(&h[1], 1ull);
       atomicAdd(&h[2], (unsigned long long)c_leaf);
@@ -1950,8 +2020,8 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
     WB_CHECK(hipEventRecord(ev[4], st));
     static const bool count_work = getenv("WB_CLASSIFY_COUNTERS") != nullptr;
     if (count_work) {
-      dcount = (unsigned long long *)wb_alloc(c, (CT_N + 24) * 8);
-      WB_CHECK(hipMemsetAsync(dcount, 0, (CT_N + 24) * 8, st));
+      dcount = (unsigned long long *)wb_alloc(c, CT_STRIDE * CT_COPIES * 8);
+      WB_CHECK(hipMemsetAsync(dcount, 0, CT_STRIDE * CT_COPIES * 8, st));
     }
     A.counters = dcount;
     heavy = (HeavyRec *)wb_alloc(c, sizeof(HeavyRec) * (n + 1));
@@ -2030,8 +2100,14 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
   out->fallbacks = hc[7];
   out->nslow = h_nslow;
   if (dcount) {
-    unsigned long long h[CT_N + 24];
-    WB_CHECK(hipMemcpy(h, dcount, sizeof h, hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> hc((size_t)CT_STRIDE * CT_COPIES);
+    WB_CHECK(hipMemcpy(hc.data(), dcount, hc.size() * 8, hipMemcpyDeviceToHost));
+    unsigned long long h[CT_STRIDE] = {};
+    for (int k = 0; k < CT_STRIDE; k++)
+      for (int r = 0; r < CT_COPIES; r++) {
+        const unsigned long long v = hc[(size_t)r * CT_STRIDE + k];
+        h[k] = k == CT_TMAX ? std::max(h[k], v) : h[k] + v;
+      }
     for (int s = 0; s < 3; s++)
       fprintf(stderr,
               "[wb counters] heavy from walk %c: points %llu passed %llu leaves %llu slots %llu "
@@ -2046,6 +2122,12 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
             h[CT_LANES_A], h[CT_LANES_B], h[CT_LANES_C], h[CT_HEAVY], h[CT_LEAF_A], h[CT_GROUP_A],
             h[CT_DIR_A], h[CT_LEAF_B], h[CT_POINT_B], h[CT_DIR_B], h[CT_EXP_A], h[CT_WLEAF_A],
             h[CT_EXP_B], h[CT_WLEAF_B], h[CT_WAVES]);
+    fprintf(stderr,
+            "[wb counters] cycles: expansions A %llu leaves A %llu expansions B %llu leaves B "
+            "%llu wedge refresh %llu | leaves A: to bounds %llu to samples %llu loop %llu | wave "
+            "iterations: samples A %llu slots B %llu\n",
+            h[CT_CYC], h[CT_CYC + 1], h[CT_CYC + 2], h[CT_CYC + 3], h[CT_CYC + 4], h[CT_CYC + 5],
+            h[CT_CYC + 6], h[CT_CYC + 7], h[CT_ITER_A], h[CT_ITER_B]);
     int rate = 0;
     hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, c->device);
     fprintf(stderr,
@@ -2054,7 +2136,7 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
             h[CT_TSUM], h[CT_TMAX], rate, h[CT_THIST], h[CT_THIST + 1], h[CT_THIST + 2],
             h[CT_THIST + 3], h[CT_THIST + 4], h[CT_TSUMS], h[CT_TSUMS + 1], h[CT_TSUMS + 2],
             h[CT_TSUMS + 3], h[CT_TSUMS + 4]);
-    wb_free(c, dcount, (CT_N + 24) * 8);
+    wb_free(c, dcount, CT_STRIDE * CT_COPIES * 8);
   }
   out->ms_tiles = ms1; out->ms_scan = ms2; out->ms_classify = ms3;
   wb_free(c, grid, 4 * gcells);
