@@ -505,12 +505,20 @@ __global__ void k_leaf_final(const LeafTmp *lt, const uint32_t *perm, uint32_t n
   inv[perm[i]] = i;
 }
 
-__global__ void k_patch_children(int32_t *child, uint32_t nn, const uint32_t *inv)
+// leaf children renumbered in DFS order, with the slot range of each
+__global__ void k_patch_children(int32_t *child, uint32_t nn, const uint32_t *inv,
+                                 const uint32_t *leaf_start, uint2 *crange)
 {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nn * 8u) return;
   int32_t c = child[i];
-  if (c <= -2) child[i] = -2 - (int32_t)inv[-2 - c];
+  uint2 r = make_uint2(0u, 0u);
+  if (c <= -2) {
+    const uint32_t l = inv[-2 - c];
+    child[i] = -2 - (int32_t)l;
+    r = make_uint2(leaf_start[l], leaf_start[l + 1]);
+  }
+  crange[i] = r;
 }
 
 __global__ void k_fill_pt_leaf(const uint32_t *leaf_start, uint32_t nl, uint32_t *pt_leaf)
@@ -883,8 +891,10 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
     wb_radix_sort_u32_pairs(c, lk, lv, nl);
     hipLaunchKernelGGL(k_leaf_final, dim3(wb_blocks(nl + 1, 256)), dim3(256), 0, c->stream, lt, lv,
                        nl, n, t->leaf_start, t->leaf_cube, t->leaf_parent, inv);
+    t->node_crange = wb_talloc<uint2>(t, 8 * (size_t)t->nnodes);
     hipLaunchKernelGGL(k_patch_children, dim3(wb_blocks((uint64_t)t->nnodes * 8, 256)), dim3(256),
-                       0, c->stream, t->node_child, t->nnodes, inv);
+                       0, c->stream, t->node_child, t->nnodes, inv, t->leaf_start,
+                       t->node_crange);
     wb_free(c, lk, 4 * (size_t)nl);
     wb_free(c, lv, 4 * (size_t)nl);
     wb_free(c, inv, 4 * (size_t)nl);

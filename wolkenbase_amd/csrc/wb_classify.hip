@@ -61,6 +61,7 @@ struct TreeView {
   const int32_t *node_child, *node_parent;
   const double *chunk_box;
   const uint32_t *chunk_low;
+  const uint2 *node_crange;
   const uint32_t *pt_leaf, *canon, *crank, *sidx;
   uint32_t nleaves, nnodes;
   uint64_t n;
@@ -72,7 +73,8 @@ static TreeView view(const wb_tree *t)
   v.px = t->px; v.py = t->py; v.pz = t->pz;
   v.leaf_start = t->leaf_start; v.leaf_cube = t->leaf_cube; v.leaf_parent = t->leaf_parent;
   v.node_cube = t->node_cube; v.node_child = t->node_child; v.node_parent = t->node_parent;
-  v.chunk_box = t->chunk_box; v.chunk_low = t->chunk_low; v.pt_leaf = t->pt_leaf; v.canon = t->canon; v.crank = t->crank;
+  v.chunk_box = t->chunk_box; v.chunk_low = t->chunk_low; v.node_crange = t->node_crange;
+  v.pt_leaf = t->pt_leaf; v.canon = t->canon; v.crank = t->crank;
   v.sidx = t->sidx; v.nleaves = t->nleaves; v.nnodes = t->nnodes; v.n = t->n;
   return v;
 }
@@ -1216,6 +1218,7 @@ __device__ inline uint64_t wave_or(uint64_t v)
 struct WaveStack {
   int code[WB_STK];
   uint32_t lo[WB_STK], hi[WB_STK];
+  uint32_t b[WB_STK], e[WB_STK];  // slot range of a leaf entry
 };
 
 // rank of child k in the visiting order for quadrant o and lower-first flag
@@ -1244,6 +1247,9 @@ __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int c
   uint64_t mk[8];       // lanes for which child k is pushed
 #pragma unroll
   for (int k = 0; k < 8; k++) ck[k] = rfl(chp[k]);
+  // lane k < 8 holds child k's slot range (leaf children), pushed with it
+  const uint2 cr = __lane_id() < 8 ? T.node_crange[8 * (size_t)code + __lane_id()]
+                                   : make_uint2(0u, 0u);
   // Octree::cube (octree.cpp:349-358) per axis bit, and the closest point of
   // wb::hyp_intersect: clamp_axis on x and y, the bottom (top for a negative
   // slope) face on z
@@ -1292,6 +1298,8 @@ __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int c
       S.code[pos] = ck[k];
       S.lo[pos] = (uint32_t)mk[k];
       S.hi[pos] = (uint32_t)(mk[k] >> 32);
+      S.b[pos] = (uint32_t)__builtin_amdgcn_readlane((int)cr.x, k);
+      S.e[pos] = (uint32_t)__builtin_amdgcn_readlane((int)cr.y, k);
     }
   }
   return sp + __popc(pushed);
@@ -1417,8 +1425,8 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
         continue;
       }
       const int leaf = -2 - code;
-      const uint32_t b = (uint32_t)rfl((int)T.leaf_start[leaf]);
-      const uint32_t e = (uint32_t)rfl((int)T.leaf_start[leaf + 1]);
+      const uint32_t b = (uint32_t)rfl((int)S.b[sp]);
+      const uint32_t e = (uint32_t)rfl((int)S.e[sp]);
       // the lane skips a leaf outside its care wedges
       const double *lc = T.leaf_cube + 4 * (size_t)leaf;
       const double lh = lc[3] / 2;

@@ -13,6 +13,7 @@
 //                                 equal z)
 //   leaf_start[nl+1] u32, leaf_cube[nl] f64x4 (cx,cy,cz,side), leaf_parent[nl]
 //   node_cube[nn] f64x4, node_child[8*nn] i32 (>=0 node, -2-l leaf, -1 empty),
+//   node_crange[8*nn] u32x2 (slot range of each leaf child; 0,0 otherwise),
 //   node_parent[nn] i32
 #pragma once
 #include <hip/hip_runtime.h>
@@ -80,6 +81,7 @@ struct wb_tree {
   int32_t *leaf_parent;
   double *node_cube;
   int32_t *node_child, *node_parent;
+  uint2 *node_crange;
   // with duplicates merged: (merged input index, input index of the first
   // point at its location) pairs, and per sorted slot the input index of
   // the last point at the slot's location (whose record a writer keeps)
