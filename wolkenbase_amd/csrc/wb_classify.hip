@@ -1577,15 +1577,25 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
 }
 
 // One heavy point per 64-lane block.  The shape is the same in every lane,
-// so the lanes split the work.  The walk (lanes 0-7 test the eight children
-// of a node; same visiting order as k_ground) collects the leaves it reaches
-// into a batch of up to HB_LEAVES.  A batch is served lane-parallel: per
+// so the lanes split the work.  The walk pops up to eight nodes at a time
+// (lane 8i+k tests child k of the i-th node from the top; children in the
+// visiting order of k_ground, the popped nodes' children stacked so the top
+// node's come out first) and collects the leaves it reaches into a batch of
+// up to HB_LEAVES.  A batch is served lane-parallel: per
 // leaf its bounds, whether its cube lies inside the hyperboloid and which
 // 32-point groups' boxes may meet it; then 64 of the batch's samples (walk
 // A: the lowest point of each group) or slots (walk B, and C, the exact
 // border pass for a longest empty run of 24 or 25 bins) per step, with the
 // early exit checked after every step.
 #define HB_LEAVES 256
+// The heavy walk's stack.  Its levels never decrease from bottom to top
+// (the children of the i-th popped node sit above those of the (i+1)-th, one
+// level deeper than their parents).  So a round that pops a level-(m-1) node
+// also pops every level >= m entry, and afterwards the level-m entries are
+// that round's children, at most 8 x 8 = 64; until the next such round they
+// only decrease.  Internal nodes sit at levels 0..20: at most 64 entries at
+// each of levels 1..20.
+#define HB_STK (64 * WB_MAX_LEVEL - 64)
 
 template <bool COUNT>
 #ifndef WB_HEAVY_MINW
@@ -1594,7 +1604,7 @@ template <bool COUNT>
 __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs A, const HeavyRec *heavy,
                                                      uint32_t nheavy)
 {
-  __shared__ int stk[WB_STK];
+  __shared__ int stk[HB_STK];
   __shared__ uint32_t lq[HB_LEAVES], lb[HB_LEAVES], le[HB_LEAVES], lmask[HB_LEAVES];
   __shared__ uint32_t lpre[HB_LEAVES + 1];
   __shared__ wb::Wedges wg;  // the care wedges of walks A and B (uniform)
@@ -1657,37 +1667,52 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
       // collect the leaves of the next part of the walk
       uint32_t nq = 0;
       while (sp > 0 && nq + 8 <= lim) {
-        sp--;
-        const int code = rfl(stk[sp]);
+        // P nodes from the top of the stack, with room for their leaves in
+        // the batch (the stack's own room follows from HB_STK)
+        const int P = min(min(8, sp), (int)((lim - nq) / 8));
+        const int gi = (int)(lane >> 3), k = (int)(lane & 7);
+        const int code = gi < P ? stk[sp - 1 - gi] : 0;
         __syncthreads();
-        if (COUNT) c_exp++;
-        const double *nc = T.node_cube + 4 * (size_t)code;
-        const double cx = nc[0], cy = nc[1], cz = nc[2], side = nc[3];
-        const int o = (x >= cx ? 1 : 0) | (y >= cy ? 2 : 0);
-        const int k = lane & 7;
-        const int ck = T.node_child[8 * (size_t)code + k];
+        if (COUNT) c_exp += P;
         bool h = false;
-        if (lane < 8 && ck != -1) {
-          const double ncx = cx + (double)(2 * (k & 1) - 1) * side / 4;
-          const double ncy = cy + (double)(2 * ((k & 2) / 2) - 1) * side / 4;
-          const double ncz = cz + (double)(2 * ((k & 4) / 4) - 1) * side / 4;
-          h = wb::hyp_intersect(H, ncx, ncy, ncz, side / 2) &&
-              (!A.cube_wedges || wb::wedge_hit(wg, x, y, ncx - side / 4, ncx + side / 4,
-                                               ncy - side / 4, ncy + side / 4));
+        int ck = -1, t = 0;
+        if (gi < P) {
+          const double *nc = T.node_cube + 4 * (size_t)code;
+          const double cx = nc[0], cy = nc[1], cz = nc[2], side = nc[3];
+          const int o = (x >= cx ? 1 : 0) | (y >= cy ? 2 : 0);
+          ck = T.node_child[8 * (size_t)code + k];
+          t = child_rank(k, o, zf);
+          if (ck != -1) {
+            const double ncx = cx + (double)(2 * (k & 1) - 1) * side / 4;
+            const double ncy = cy + (double)(2 * ((k & 2) / 2) - 1) * side / 4;
+            const double ncz = cz + (double)(2 * ((k & 4) / 4) - 1) * side / 4;
+            h = wb::hyp_intersect(H, ncx, ncy, ncz, side / 2) &&
+                (!A.cube_wedges || wb::wedge_hit(wg, x, y, ncx - side / 4, ncx + side / 4,
+                                                 ncy - side / 4, ncy + side / 4));
+          }
         }
-        const int t = child_rank(k, o, zf);
+        // per popped node: node children and leaf children by rank
         uint32_t rn = h && ck >= 0 ? 1u << t : 0u, rl = h && ck < -1 ? 1u << t : 0u;
-        for (int s = 1; s < 8; s <<= 1) {
-          rn |= __shfl_xor(rn, s, 64);
-          rl |= __shfl_xor(rl, s, 64);
+        for (int s2 = 1; s2 < 8; s2 <<= 1) {
+          rn |= __shfl_xor(rn, s2, 64);
+          rl |= __shfl_xor(rl, s2, 64);
         }
-        rn = (uint32_t)rfl((int)rn);
-        rl = (uint32_t)rfl((int)rl);
-        // nodes onto the stack, rank 0 on top; leaves into the batch by rank
-        if (h && ck >= 0) stk[sp + __popc(rn >> (t + 1))] = ck;
-        if (h && ck < -1) lq[nq + __popc(rl & ((1u << t) - 1))] = (uint32_t)(-2 - ck);
-        sp += __popc(rn);
-        nq += __popc(rl);
+        // node i's children go above those of nodes i+1..P-1; its leaves
+        // after those of nodes 0..i-1
+        const int base = sp - P;
+        int nabove = 0, lbefore = 0, ntot = 0, ltot = 0;
+        for (int i = 0; i < P; i++) {
+          const int cn = __popc((uint32_t)__builtin_amdgcn_readlane((int)rn, 8 * i));
+          const int cl = __popc((uint32_t)__builtin_amdgcn_readlane((int)rl, 8 * i));
+          if (i > gi) nabove += cn;
+          if (i < gi) lbefore += cl;
+          ntot += cn;
+          ltot += cl;
+        }
+        if (h && ck >= 0) stk[base + nabove + __popc(rn >> (t + 1))] = ck;
+        if (h && ck < -1) lq[nq + lbefore + __popc(rl & ((1u << t) - 1))] = (uint32_t)(-2 - ck);
+        sp = base + ntot;
+        nq += ltot;
         __syncthreads();
       }
       if (nq == 0) break;
