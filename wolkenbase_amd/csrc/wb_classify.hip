@@ -1347,8 +1347,10 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
 {
   __shared__ WaveStack stacks[256 / WAVE];
   __shared__ LaneWedges LW;
+  __shared__ float edges[3 * 128];  // the bin-edge table, read by wedge refreshes
   WaveStack &S = stacks[threadIdx.x / WAVE];
   LW.n[threadIdx.x] = -1;
+  for (int k = threadIdx.x; k < 3 * 128; k += 256) edges[k] = A.edges[k];
   __syncthreads();
   const TreeView &T = A.T;
   const uint32_t lane = __lane_id();
@@ -1518,7 +1520,7 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
       if (state == 0 && occ != checked) {
         checked = occ;
         if (gap_verdict(occ) == 1) state = 1;
-        else lane_wedges_update(occ, A.edges, LW, threadIdx.x);
+        else lane_wedges_update(occ, edges, LW, threadIdx.x);
       }
       if (COUNT) cyc[4] += clock64() - c1t;
       if (phase == 0 && state == 0 && groups_a > HEAVY_A_GROUPS) state = 5;
