@@ -391,7 +391,15 @@ __device__ inline uint32_t digit_at(uint64_t k, int level) { return (uint32_t)(k
 // The levels are launched back to back: the number of active nodes of a
 // level is the counter the previous level incremented (act_cnt), and a level
 // after a failed one does nothing.
-__global__ void k_split_level(const uint64_t *key, const Active *act, const uint32_t *act_cnt,
+// A level with at most SPLIT_WAVE_MAX active nodes (the top of the tree:
+// few nodes, long ranges) takes a wave per node; a level with more takes a
+// thread per node (k_split_level_t).  Both kernels are launched for every
+// level, since only the device knows the count; the other returns at once.
+#define SPLIT_WAVE_MAX 2048u
+
+// One thread per active node (levels with many active nodes, each with a
+// short range): seven binary searches for the boundaries.
+__global__ void k_split_level_t(const uint64_t *key, const Active *act, const uint32_t *act_cnt,
                               uint32_t act_cap, int level, int max_level, Active *next,
                               uint32_t *next_cnt, uint32_t next_cap, LeafTmp *leaves,
                               uint32_t leaf_cap, double *node_cube, int32_t *node_child,
@@ -399,6 +407,7 @@ __global__ void k_split_level(const uint64_t *key, const Active *act, const uint
 {
   if (out->overflow) return;
   const uint32_t nact = min(*act_cnt, act_cap);
+  if (nact <= SPLIT_WAVE_MAX) return;  // k_split_level takes this level
   uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= nact) return;
   Active A = act[a];
@@ -474,6 +483,147 @@ __global__ void k_split_level(const uint64_t *key, const Active *act, const uint
       }
     }
     node_child[8 * node + c] = code;
+  }
+}
+
+// One wave per active node.  The eight child ranges follow from the seven
+// boundaries "first slot whose level-(l+1) digit is >= c" (c = 1..7) of the
+// node's sorted keys: 64 evenly spaced samples place every boundary between
+// two neighbouring samples, then lanes 8(c-1)..8(c-1)+7 narrow boundary c by
+// a 9-way search (8 probes per round).  Lanes 0..7 then make child c = lane.
+__global__ void __launch_bounds__(256) k_split_level(
+    const uint64_t *key, const Active *act, const uint32_t *act_cnt, uint32_t act_cap, int level,
+    int max_level, Active *next, uint32_t *next_cnt, uint32_t next_cap, LeafTmp *leaves,
+    uint32_t leaf_cap, double *node_cube, int32_t *node_child, int32_t *node_parent,
+    uint32_t node_cap, SplitOut *out)
+{
+  if (out->overflow) return;
+  const uint32_t nact = min(*act_cnt, act_cap);
+  if (nact > SPLIT_WAVE_MAX) return;  // k_split_level_t takes this level
+  const uint32_t lane = threadIdx.x & 63;
+  const int lv = level + 1;
+  for (uint32_t a = blockIdx.x * 4 + threadIdx.x / 64; a < nact; a += gridDim.x * 4) {
+    const Active A = act[a];
+    const int node = A.node;
+    const uint32_t len = A.end - A.start;  // > 537: an active node
+    // samples l = 0..63 (sorted, so their digits are too)
+    const uint32_t ps = A.start + (uint32_t)(((uint64_t)lane * len) >> 6);
+    const uint32_t ds = digit_at(key[ps], lv);
+    // boundary c of this lane's group: it lies after the last sample with a
+    // digit below c and at or before the first one with a digit of c or more
+    const uint32_t c = (lane >> 3) + 1, q = lane & 7;
+    uint32_t lo = A.start, hi = A.end;
+#pragma unroll
+    for (uint32_t cc = 1; cc < 8; cc++) {
+      const uint32_t j = (uint32_t)__popcll(__ballot(ds < cc));
+      const uint32_t plo = (uint32_t)__shfl((int)ps, (int)(j == 0 ? 0 : j - 1), 64);
+      const uint32_t phi = (uint32_t)__shfl((int)ps, (int)(j == 64 ? 63 : j), 64);
+      if (c == cc) {
+        lo = j == 0 ? A.start : plo + 1;
+        hi = j == 64 ? A.end : phi;
+      }
+    }
+    // the first slot of [lo, hi) with digit >= c, or hi; 9-way per round
+    for (;;) {
+      const bool open = c < 8 && lo < hi;
+      if (!__ballot(open)) break;
+      if (open) {
+        // the 8 lanes of a group share lo and hi, so they open and close
+        // together and the ballot and shuffles stay inside the group
+        const uint32_t p = lo + (uint32_t)(((uint64_t)(hi - lo) * (q + 1)) / 9);
+        const bool below = digit_at(key[p], lv) < c;
+        // probes of the group, ascending in q: count those below c
+        const uint64_t bm = __ballot(below);
+        const uint32_t k = (uint32_t)__popcll((bm >> (lane & ~7u)) & 0xffull);
+        const uint32_t pk1 = (uint32_t)__shfl((int)p, (int)((lane & ~7u) + (k == 0 ? 0 : k - 1)), 64);
+        const uint32_t pk = (uint32_t)__shfl((int)p, (int)((lane & ~7u) + (k == 8 ? 7 : k)), 64);
+        const uint32_t nlo = k == 0 ? lo : pk1 + 1;
+        const uint32_t nhi = k == 8 ? hi : pk;
+        lo = nlo;
+        hi = nhi;
+      }
+    }
+    // child c = lane (0..7): slots [bound[c], bound[c+1]); every lane takes
+    // part in both shuffles (a lane left out of one supplies no value)
+    const uint32_t s_lo = (uint32_t)__shfl((int)lo, (int)(8 * ((lane - 1) & 7)), 64);
+    const uint32_t s_hi = (uint32_t)__shfl((int)lo, (int)(8 * (lane & 7)), 64);
+    const uint32_t b_lo = lane == 0 ? A.start : s_lo;
+    const uint32_t b_hi = lane == 7 ? A.end : s_hi;
+    const uint32_t cnt = b_hi - b_lo;
+    const bool valid = lane < 8 && cnt > 0;
+    const bool big = valid && cnt > WB_RECORDS;
+    const bool is_node = big && lv < max_level, is_leaf = valid && !big;
+    if (big && lv >= max_level) atomicOr(&out->overflow, 1u);
+    // one counter update per wave and kind; children numbered in child order
+    const uint64_t mnode = __ballot(is_node), mleaf = __ballot(is_leaf);
+    uint32_t nbase = 0, sbase = 0, lbase = 0;
+    if (lane == 0) {
+      if (mnode) {
+        nbase = atomicAdd(&out->n_node, (uint32_t)__popcll(mnode));
+        sbase = atomicAdd(next_cnt, (uint32_t)__popcll(mnode));
+      }
+      if (mleaf) {
+        lbase = atomicAdd(&out->n_leaf, (uint32_t)__popcll(mleaf));
+        atomicMax(&out->maxdepth, (uint32_t)lv);
+      }
+    }
+    nbase = (uint32_t)__shfl((int)nbase, 0, 64);
+    sbase = (uint32_t)__shfl((int)sbase, 0, 64);
+    lbase = (uint32_t)__shfl((int)lbase, 0, 64);
+    const uint64_t below = (1ull << lane) - 1;
+    if (lane < 8) {
+      const int cch = (int)lane;
+      int32_t code = -1;
+      if (valid) {
+        const double cx = node_cube[4 * node], cy = node_cube[4 * node + 1];
+        const double cz = node_cube[4 * node + 2], side = node_cube[4 * node + 3];
+        // Octree::cube(c), octree.cpp:349-358
+        const int xb = cch & 1, yb = (cch & 2) / 2, zb = (cch & 4) / 4;
+        const double ncx = cx + (2 * xb - 1) * side / 4;
+        const double ncy = cy + (2 * yb - 1) * side / 4;
+        const double ncz = cz + (2 * zb - 1) * side / 4;
+        const double ns = side / 2;
+        if (is_node) {
+          const uint32_t nid = nbase + (uint32_t)__popcll(mnode & below);
+          const uint32_t slot = sbase + (uint32_t)__popcll(mnode & below);
+          if (nid < node_cap && slot < next_cap) {
+            node_cube[4 * nid] = ncx;
+            node_cube[4 * nid + 1] = ncy;
+            node_cube[4 * nid + 2] = ncz;
+            node_cube[4 * nid + 3] = ns;
+            node_parent[nid] = node;
+            for (int k = 0; k < 8; k++) node_child[8 * nid + k] = -1;
+            Active na;
+            na.start = b_lo;
+            na.end = b_hi;
+            na.node = (int32_t)nid;
+            na.pad = 0;
+            next[slot] = na;
+            code = (int32_t)nid;
+          } else {
+            atomicOr(&out->overflow, 2u);
+          }
+        } else if (is_leaf) {
+          const uint32_t ls = lbase + (uint32_t)__popcll(mleaf & below);
+          if (ls < leaf_cap) {
+            LeafTmp L;
+            L.start = b_lo;
+            L.count = cnt;
+            L.parent = node;
+            L.depth = lv;
+            L.c[0] = ncx;
+            L.c[1] = ncy;
+            L.c[2] = ncz;
+            L.c[3] = ns;
+            leaves[ls] = L;
+            code = -2 - (int32_t)ls;
+          } else {
+            atomicOr(&out->overflow, 4u);
+          }
+        }
+      }
+      node_child[8 * node + cch] = code;
+    }
   }
 }
 
@@ -842,10 +992,16 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
     }
     WB_CHECK(hipMemcpyAsync(d_cnt, hcnt, sizeof hcnt, hipMemcpyHostToDevice, c->stream));
     for (int level = 0; n > 0 && level < max_level; level++) {
-      hipLaunchKernelGGL(k_split_level, dim3(wb_blocks(act_ub, 128)), dim3(128), 0, c->stream,
+      hipLaunchKernelGGL(k_split_level, dim3(std::min<uint32_t>(wb_blocks(act_ub, 4), 512)),
+                         dim3(256), 0, c->stream,
                          t->key, act, d_cnt + level, node_cap, level, max_level, nxt,
                          d_cnt + level + 1, node_cap, lt, leaf_cap, t->node_cube, t->node_child,
                          t->node_parent, node_cap, d_out);
+      if (act_ub > SPLIT_WAVE_MAX)
+        hipLaunchKernelGGL(k_split_level_t, dim3(wb_blocks(act_ub, 128)), dim3(128), 0, c->stream,
+                           t->key, act, d_cnt + level, node_cap, level, max_level, nxt,
+                           d_cnt + level + 1, node_cap, lt, leaf_cap, t->node_cube,
+                           t->node_child, t->node_parent, node_cap, d_out);
       std::swap(act, nxt);
     }
     WB_CHECK(hipMemcpyAsync(&h_out, d_out, sizeof h_out, hipMemcpyDeviceToHost, c->stream));
