@@ -250,6 +250,41 @@ def test_wedge_pruning_is_conservative(hm):
                 assert hm.wbh_dir_bin(qy, qx) not in care, (hex(occ), ang, u, v)
 
 
+def test_scan_sector_fast_form(hm):
+    # the comparison form of the scan's sector (sqrt(3)|vy| against |vx|, the
+    # signs, and the squared radius against (r/sqrt 7)^2) equals the atan2 /
+    # lrint / hypot form on random points and on points placed just either
+    # side of every sector boundary, of the axis lines and of the inner radius
+    hm.wbh_scan_sector.restype = C.c_int
+    hm.wbh_scan_sector.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+    rng = np.random.default_rng(29)
+    pts = []
+    for _ in range(20000):
+        r = float(rng.uniform(0.3, 20))
+        q = rng.normal(size=2) * r
+        pts.append((float(q[0]), float(q[1]), r))
+    for k in range(12):  # boundaries every 30 degrees (half of them sector edges)
+        for off in (0.0, 1e-15, -1e-15, 1e-12, -1e-12, 1e-10, -1e-10, 1e-8, -1e-8, 1e-4):
+            for rad in (0.05, 1.0, 7.0):
+                t = k * np.pi / 6 + off
+                pts.append((rad * np.cos(t), rad * np.sin(t), 2.0))
+    for vy in (0.0, -0.0, 1e-300, -1e-300):
+        for vx in (1.0, -1.0, 0.0, -0.0):
+            pts.append((vx, vy, 1.0))
+    for off in (0.0, 1e-16, -1e-16, 1e-13, -1e-13, 1e-11, -1e-11):  # the inner radius
+        r = 3.0
+        rr = r / 2.6457513110645905905016 * (1 + off)
+        for t in rng.uniform(-np.pi, np.pi, 20):
+            pts.append((rr * np.cos(t), rr * np.sin(t), r))
+    fast = exact = 0
+    for vx, vy, r in pts:
+        a = hm.wbh_scan_sector(0, vx, vy, r)
+        b = hm.wbh_scan_sector(1, vx, vy, r)
+        assert a == b, (vx, vy, r, a, b)
+        fast += 1
+    assert fast == len(pts)
+
+
 def test_x87_emulation_ops(hm):
     rng = np.random.default_rng(14)
     for _ in range(40000):

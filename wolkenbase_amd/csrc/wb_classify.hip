@@ -738,7 +738,7 @@ __device__ inline bool plane_tilt_f32(uint32_t m, const double (&vx)[K], const d
 // e/64 of lane e%64): the same operations and summation order as the
 // general path of k_scan, with the canonical sort done by a bitonic network
 // over lanes and slots and every point read once.
-template <int K>
+template <int K, bool F32>
 __device__ __attribute__((always_inline)) inline void scan_reg(const TreeView &T, const TileGeo &g,
                                                                 int ex, int ey, uint32_t off,
                          uint32_t m, const uint32_t *lists, double min_hs, int prec,
@@ -794,9 +794,9 @@ __device__ __attribute__((always_inline)) inline void scan_reg(const TreeView &T
   }
   double sx = 0, sy = 0;
   // FP32 fits are an option for the precision study; FP64 is the reference
-  bool fit64 = prec == WB_FIT_FP64;
+  bool fit64 = !F32 || prec == WB_FIT_FP64;
   int fell_back = 0;
-  if (!fit64) {
+  if (F32 && !fit64) {
     const bool well = plane_tilt_f32<K>(m, vx, vy, vz, mine, prec == WB_FIT_FP32_COMP, &sx, &sy);
     if (!well && prec == WB_FIT_FP32_COMP) fit64 = true, fell_back = 1;
   }
@@ -849,14 +849,9 @@ __device__ __attribute__((always_inline)) inline void scan_reg(const TreeView &T
     int sec = -1;
     bool edge = false;
     if (mine[s] && u[s] < bottom2 + 2 * r) {
-      const double th = atan2(vy[s], vx[s]) * 3 / M_PI;
-      long sc = lrint(th);
-      const double fr = th - floor(th);
-      edge = wb::dabs(fr - 0.5) < 1e-12;
-      if (sc < 0) sc += 6;
-      sc = (sc % 6) + 1;
-      if (wb::hypot(vx[s], vy[s]) < r / kSqrt7) sc = 0;
-      sec = (int)sc;
+      int e = 0;
+      sec = wb::scan_sector(vx[s], vy[s], r, &e);
+      edge = e != 0;
     }
     nbottom += (uint32_t)__popcll(__ballot(sec >= 0));
     nedge += (uint32_t)__popcll(__ballot(edge));
@@ -891,9 +886,11 @@ __device__ __attribute__((always_inline)) inline void scan_reg(const TreeView &T
 // 129-256 points (K = 1, 2, 4 points per lane; the K = 1 kernel also writes
 // the empty tiles).  Each class has its own register budget.
 #ifndef WB_SCAN_MINW
-#define WB_SCAN_MINW 4
+#define WB_SCAN_MINW 5
 #endif
-template <int K>
+// F32: the kernel of the FP32 fit options (the precision study); the FP64
+// kernel does not carry their code or registers.
+template <int K, bool F32>
 __global__ void __launch_bounds__(64, WB_SCAN_MINW) k_scan_reg(TreeView T, TileGeo g, const int32_t *tex,
                                                  const int32_t *tey, const uint32_t *tile_off,
                                                  const uint32_t *tile_cnt, const uint32_t *lists,
@@ -911,7 +908,7 @@ __global__ void __launch_bounds__(64, WB_SCAN_MINW) k_scan_reg(TreeView T, TileG
     }
     const uint32_t below = K == 1 ? 0u : (uint32_t)(WAVE * K / 2);  // the next smaller class
     if (m <= below || m > (uint32_t)(WAVE * K)) continue;
-    scan_reg<K>(T, g, tex[t], tey[t], tile_off[t], m, lists, min_hs, prec, rec + t, ctr);
+    scan_reg<K, F32>(T, g, tex[t], tey[t], tile_off[t], m, lists, min_hs, prec, rec + t, ctr);
   }
 }
 
@@ -986,13 +983,9 @@ __device__ void scan_big(const TreeView &T, const TileGeo &g, int ex, int ey, ui
     if (!(U(j) < bottom2 + 2 * r)) continue;
     nbottom++;
     const double vx = VX(j), vy = VY(j);
-    double th = atan2(vy, vx) * 3 / M_PI;
-    long sec = lrint(th);
-    double fr = th - floor(th);
-    if (wb::dabs(fr - 0.5) < 1e-12) atomicAdd(&ctr[0], 1ull);
-    if (sec < 0) sec += 6;
-    sec = (sec % 6) + 1;
-    if (wb::hypot(vx, vy) < r / kSqrt7) sec = 0;
+    int edge = 0;
+    const int sec = wb::scan_sector(vx, vy, r, &edge);
+    if (edge) atomicAdd(&ctr[0], 1ull);
     atomicAdd(&histo[sec], 1);
   }
   for (int s = 1; s < WAVE; s <<= 1) nbottom += __shfl_xor((int)nbottom, s, WAVE);
@@ -2020,12 +2013,21 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
     const unsigned nb = (unsigned)std::min<uint64_t>(g.ntiles, 1u << 16);
     const int prec = p->fit_precision;
     if (nb) {
-      hipLaunchKernelGGL(k_scan_reg<1>, dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,
-                         tile_cnt, lists, p->min_hs, prec, rec, ctr);
-      hipLaunchKernelGGL(k_scan_reg<2>, dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,
-                         tile_cnt, lists, p->min_hs, prec, rec, ctr);
-      hipLaunchKernelGGL(k_scan_reg<4>, dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,
-                         tile_cnt, lists, p->min_hs, prec, rec, ctr);
+      if (prec == WB_FIT_FP64) {
+        hipLaunchKernelGGL((k_scan_reg<1, false>), dim3(nb), dim3(64), 0, st, T, g, tex, tey,
+                           tile_off, tile_cnt, lists, p->min_hs, prec, rec, ctr);
+        hipLaunchKernelGGL((k_scan_reg<2, false>), dim3(nb), dim3(64), 0, st, T, g, tex, tey,
+                           tile_off, tile_cnt, lists, p->min_hs, prec, rec, ctr);
+        hipLaunchKernelGGL((k_scan_reg<4, false>), dim3(nb), dim3(64), 0, st, T, g, tex, tey,
+                           tile_off, tile_cnt, lists, p->min_hs, prec, rec, ctr);
+      } else {
+        hipLaunchKernelGGL((k_scan_reg<1, true>), dim3(nb), dim3(64), 0, st, T, g, tex, tey,
+                           tile_off, tile_cnt, lists, p->min_hs, prec, rec, ctr);
+        hipLaunchKernelGGL((k_scan_reg<2, true>), dim3(nb), dim3(64), 0, st, T, g, tex, tey,
+                           tile_off, tile_cnt, lists, p->min_hs, prec, rec, ctr);
+        hipLaunchKernelGGL((k_scan_reg<4, true>), dim3(nb), dim3(64), 0, st, T, g, tex, tey,
+                           tile_off, tile_cnt, lists, p->min_hs, prec, rec, ctr);
+      }
       hipLaunchKernelGGL(k_scan, dim3(std::min(nb, 4096u)), dim3(64), 0, st, T, g, tex, tey,
                          tile_off, tile_cnt, lists, p->min_hs, prec, rec, err, ctr);
     }

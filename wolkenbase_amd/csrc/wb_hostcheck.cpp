@@ -189,4 +189,13 @@ void wbh_snake_setup(const double *bbmin, const double *bbmax, double ts, double
   out3[0] = s.lo; out3[1] = s.hi; out3[2] = s.index;
 }
 void wbh_size_fit(const double *pts, int n, double *c, double *side) { wb::size_fit(pts, n, c, side); }
+// the scan's histogram sector: the fast form (which = 0) or the exact one;
+// returns sector * 2 + edge flag
+int wbh_scan_sector(int which, double vx, double vy, double r)
+{
+  int edge = 0;
+  const int sec = which == 0 ? wb::scan_sector(vx, vy, r, &edge)
+                             : wb::scan_sector_exact(vx, vy, r, &edge);
+  return sec * 2 + edge;
+}
 }

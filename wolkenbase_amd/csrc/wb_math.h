@@ -464,6 +464,49 @@ WB_HD int dir_bin(double dy, double dx, const double *tab_tan, const double *tab
 }
 
 // ---------------------------------------------------------------------------
+constexpr double kScanSqrt7 = 2.6457513110645905905016;  // scan.h:23 (M_SQRT7)
+constexpr double kScanSqrt3 = 1.7320508075688772935274463;
+
+// The histogram sector of a bottom-layer point (vx, vy) from the cylinder
+// axis (scan.cpp:93-101): lrint(atan2(vy, vx)*3/pi) folded to 1..6, or 0
+// inside radius r/sqrt(7); *edge when atan2*3/pi lies within 1e-12 of a
+// half-integer.  Out of line: the exact form for the rare points near a
+// boundary.
+WB_HD_COLD int scan_sector_exact(double vx, double vy, double r, int *edge)
+{
+  const double th = atan2(vy, vx) * 3 / M_PI;
+  long sc = lrint(th);
+  const double fr = th - floor(th);
+  *edge = dabs(fr - 0.5) < 1e-12 ? 1 : 0;
+  if (sc < 0) sc += 6;
+  sc = (sc % 6) + 1;
+  if (hypot(vx, vy) < r / kScanSqrt7) sc = 0;
+  return (int)sc;
+}
+
+// The same from comparisons: the sector boundaries are the directions at
+// 30, 90 and 150 degrees either side of +x, i.e. sqrt(3)|vy| = |vx| and
+// vx = 0.  When sqrt(3)|vy| and |vx| differ by more than 1e-9 of their sum and
+// |vx| exceeds it, the direction is at least 8e-10 rad (in atan2*3/pi, 8e-10)
+// from every boundary, far beyond the roundings of atan2, the product and
+// the compares, and the edge test cannot fire; when vx^2 + vy^2 differs from
+// (r/sqrt(7))^2 by more than 1e-12 of it, hypot's rounding cannot change the
+// radius test.  Otherwise the exact form decides.
+WB_HD int scan_sector(double vx, double vy, double r, int *edge)
+{
+  const double rr = r / kScanSqrt7;
+  const double a = kScanSqrt3 * dabs(vy), b = dabs(vx), m = 1e-9 * (a + b);
+  const double q2 = vx * vx + vy * vy, t2 = rr * rr;
+  if ((dabs(a - b) > m) & (b > m) & (dabs(q2 - t2) > 1e-12 * t2)) {
+    *edge = 0;
+    if (q2 < t2) return 0;
+    if (a < b) return vx > 0 ? 1 : 4;
+    return vy > 0 ? (vx > 0 ? 2 : 3) : (vx > 0 ? 6 : 5);
+  }
+  return scan_sector_exact(vx, vy, r, edge);
+}
+
+// ---------------------------------------------------------------------------
 // Shapes (shape.cpp).  Kinds follow the C ABI: 0 cylinder, 1 sphere,
 // 2 hyperboloid.
 // ---------------------------------------------------------------------------
