@@ -185,6 +185,28 @@ def main_las(args):
     for nm in full["names"]:
         os.remove(nm)
     st = stats[-1]
+    store = None
+    if args.store:
+        # the store's blocks of the classified tree (OctBuffer::write of every
+        # leaf) into pinned host memory: the spill tier's write rate
+        cl = WL.Cloud(ctx, [las], zeros="read")
+        tree = cl.build()
+        cl.classify(tree, want_tiles=False, out=cls)
+        nl = tree.info()["nleaves"]
+        host = torch.empty(nl * W.STORE_BLOCK, dtype=torch.uint8, pin_memory=True)
+        tree.store_blocks(cloud=cl, cls=cls, out=host)  # warm
+        reps = 3
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            tree.store_blocks(cloud=cl, cls=cls, out=host)
+        ms = (time.perf_counter() - t1) * 1e3 / reps
+        store = {"blocks": nl, "bytes": nl * W.STORE_BLOCK, "points": tree.n,
+                 "ms_pack_and_copy_to_pinned_host": round(ms, 2),
+                 "gbs_to_host": round(nl * W.STORE_BLOCK / (ms * 1e-3) / 1e9, 1)}
+        log(f"[bench] store blocks: {nl} blocks in {ms:.2f} ms")
+        del host
+        tree.free()
+        cl.close()
     kern = {"k_ground": avg("ms_classify_kernel"), "k_ground_heavy": avg("ms_heavy_kernel"),
             "radix_sort": avg("ms_sort"), "k_las_decode": avg("ms_decode"),
             "k_las_encode": avg("ms_encode")}
@@ -237,6 +259,7 @@ def main_las(args):
                                 "bytes_in": len(img), "bytes_out": full["bytes_out"]},
         "duplicates": st["ndup"],
         "files": st["nfiles"],
+        "store_outside_timed": store,
         "classes": {str(k): v for k, v in st["classes"].items()},
     }
     if not args.no_cpu_baseline:
@@ -256,6 +279,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="tile20m", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--store", action="store_true",
+                    help="las20m: also time the store's blocks into pinned host memory")
     ap.add_argument("--cpu-window", type=float, default=60.0)
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="run one 256 MiB device copy first, so a --pmc profile of this "

@@ -317,6 +317,40 @@ int wb_cloud_write(wb_cloud *cloud, wb_tree *tree, const uint8_t *cls, int flags
                    const wb_las_write_params *params, wb_las_write_stats *stats, char *names,
                    uint64_t names_cap);
 
+/* --- the store's block format (octree.h:35-43, octree.cpp:427-559): the
+ *     spill tier of the HBM-resident tree.  Block l holds leaf l (DFS order):
+ *     its stored points in canonical order as LasPoint::write records
+ *     (las.cpp:177-207, 61 bytes), then noPoint records (NaN location, zero
+ *     fields) up to 537, then zero bytes up to 32768 (OctBuffer::write).
+ *     With a cloud (the tree built from it by wb_cloud_build) a record holds
+ *     the fields of the location's last input as readPoint decodes them
+ *     (las.cpp:808-904) with the READ phase's return fix; without one,
+ *     LasPoint's defaults (las.cpp:122-132) at the point's location.  cls
+ *     (may be NULL): class bytes in input order as wb_classify returns them
+ *     (a device pointer with WB_DEVICE_INPUT); a nonzero byte replaces the
+ *     classification, as CLASSIFY's put does (classify.cpp:158-163). */
+#define WB_STORE_RECORD 61
+#define WB_STORE_RECORDS 537
+#define WB_STORE_BLOCK 32768
+/* the blocks of every leaf into out (host memory, pinned or not, of cap
+ * bytes; NULL: only *nblocks, the number of leaves, is returned) */
+int wb_store_blocks(wb_tree *tree, wb_cloud *cloud, const uint8_t *cls, int flags, void *out,
+                    uint64_t cap, uint64_t *nblocks);
+/* the blocks as the store's files (OctStore::open and OctBuffer::write,
+ * octree.cpp:812-830, 427-460): block b in file name<b % nfiles> at offset
+ * 32768 * (b / nfiles); nfiles below 1 is 1 */
+int wb_store_write_files(wb_tree *tree, wb_cloud *cloud, const uint8_t *cls, int flags,
+                         const char *name, int nfiles);
+/* OctBuffer::read (octree.cpp:508-559) of nblocks blocks in host memory:
+ * the records with a location (LasPoint::isEmpty, point.cpp:252-255),
+ * block by block in record order, into x, y, z and classification (each may
+ * be NULL; room for the total); block_count[b] (may be NULL) the points of
+ * block b, 0 for a block whose first two locations are equal (a block never
+ * written reads as zeros); *npoints the total */
+int wb_store_read_blocks(wb_ctx *ctx, const void *blocks, uint64_t nblocks, double *x, double *y,
+                         double *z, uint16_t *classification, uint32_t *block_count,
+                         uint64_t *npoints);
+
 #ifdef __cplusplus
 }
 #endif

@@ -643,3 +643,92 @@ void wbo_ldecimal(double x, double toler, char *out, int cap)
     snprintf(out, cap, "%s", buffer);
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* The store's block format (octree.h:35-43).                          */
+/* ------------------------------------------------------------------ */
+#define WBO_STORE_RECORD 61
+#define WBO_STORE_RECORDS 537
+#define WBO_STORE_BLOCK 32768
+
+/* LasPoint::write (las.cpp:177-207), built without WAVEFORM: the location,
+ * then the fields in declaration order, little-endian; the two flags share
+ * one byte as edgeLine*2+scanDirection */
+void wbo_store_record(const wbo_las_point *p, uint8_t *o)
+{
+  wrd(o, 0, p->x);
+  wrd(o, 8, p->y);
+  wrd(o, 16, p->z);
+  wr(o, 24, p->intensity, 2);
+  wr(o, 26, p->ret, 2);
+  wr(o, 28, p->nret, 2);
+  o[30] = (uint8_t)(p->edge * 2 + p->scan_dir);
+  wr(o, 31, p->cls, 2);
+  wr(o, 33, p->cls_flags, 2);
+  wr(o, 35, p->channel, 2);
+  wr(o, 37, p->user, 2);
+  wr(o, 39, p->source, 2);
+  wr(o, 41, (uint32_t)p->scan_angle, 4);
+  wrd(o, 45, p->gps);
+  wr(o, 53, p->nir, 2);
+  wr(o, 55, p->red, 2);
+  wr(o, 57, p->green, 2);
+  wr(o, 59, p->blue, 2);
+}
+
+/* OctBuffer::write (octree.cpp:427-460): n <= 537 points, then noPoint
+ * (LasPoint(), las.cpp:122-132: NaN location, zero fields) up to RECORDS,
+ * then zero bytes up to BLOCKSIZE.  Returns 0, or -1 for n > 537. */
+int wbo_store_block(const wbo_las_point *pts, int n, uint8_t *out)
+{
+  wbo_las_point none;
+  if (n < 0 || n > WBO_STORE_RECORDS) return -1;
+  memset(&none, 0, sizeof none);
+  none.x = none.y = none.z = NAN;
+  for (int r = 0; r < WBO_STORE_RECORDS; r++)
+    wbo_store_record(r < n ? &pts[r] : &none, out + (size_t)r * WBO_STORE_RECORD);
+  memset(out + WBO_STORE_RECORDS * WBO_STORE_RECORD, 0,
+         WBO_STORE_BLOCK - WBO_STORE_RECORDS * WBO_STORE_RECORD);
+  return 0;
+}
+
+/* LasPoint::read (las.cpp:139-176) of one record */
+void wbo_store_read_record(const uint8_t *d, wbo_las_point *p)
+{
+  const size_t L = WBO_STORE_RECORD;
+  p->x = rdd(d, L, 0);
+  p->y = rdd(d, L, 8);
+  p->z = rdd(d, L, 16);
+  p->intensity = (uint16_t)rd(d, L, 24, 2);
+  p->ret = (uint16_t)rd(d, L, 26, 2);
+  p->nret = (uint16_t)rd(d, L, 28, 2);
+  p->scan_dir = d[30] & 1;
+  p->edge = (d[30] >> 1) & 1;
+  p->cls = (uint16_t)rd(d, L, 31, 2);
+  p->cls_flags = (uint16_t)rd(d, L, 33, 2);
+  p->channel = (uint16_t)rd(d, L, 35, 2);
+  p->user = (uint16_t)rd(d, L, 37, 2);
+  p->source = (uint16_t)rd(d, L, 39, 2);
+  p->scan_angle = (int32_t)(uint32_t)rd(d, L, 41, 4);
+  p->gps = rdd(d, L, 45);
+  p->nir = (uint16_t)rd(d, L, 53, 2);
+  p->red = (uint16_t)rd(d, L, 55, 2);
+  p->green = (uint16_t)rd(d, L, 57, 2);
+  p->blue = (uint16_t)rd(d, L, 59, 2);
+}
+
+/* OctBuffer::read (octree.cpp:508-559): the records whose location has no
+ * NaN (LasPoint::isEmpty, point.cpp:252-255) in record order; none when
+ * there are two or more and the first two locations are equal (a block
+ * never written reads as zeros).  Returns the number kept. */
+int wbo_store_read_block(const uint8_t *blk, wbo_las_point *out)
+{
+  int n = 0;
+  for (int r = 0; r < WBO_STORE_RECORDS; r++) {
+    wbo_las_point p;
+    wbo_store_read_record(blk + (size_t)r * WBO_STORE_RECORD, &p);
+    if (!(isnan(p.x) || isnan(p.y) || isnan(p.z))) out[n++] = p;
+  }
+  if (n > 1 && out[0].x == out[1].x && out[0].y == out[1].y && out[0].z == out[1].z) n = 0;
+  return n;
+}

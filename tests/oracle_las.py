@@ -87,6 +87,12 @@ def lib():
                                   C.POINTER(O.Stats)]
         L.wbo_run_xyz.restype = C.c_long
         L.wbo_sizefit.argtypes = [_dp, C.c_int, _dp, _dp]
+        L.wbo_store_record.argtypes = [C.POINTER(Point), C.c_char_p]
+        L.wbo_store_block.argtypes = [C.POINTER(Point), C.c_int, C.c_char_p]
+        L.wbo_store_block.restype = C.c_int
+        L.wbo_store_read_record.argtypes = [C.c_char_p, C.POINTER(Point)]
+        L.wbo_store_read_block.argtypes = [C.c_char_p, C.POINTER(Point)]
+        L.wbo_store_read_block.restype = C.c_int
         _ready = True
     return L
 
@@ -278,7 +284,40 @@ def pipeline(images, params=None, unit=1.0, zeros="read", name="out", separate=T
             files[fname] = hd.raw[:375] + bytes(body)
     return dict(x=x, y=y, z=z, cls=cls, ndup=int(nd), canon=canon, totals=totals, files=files,
                 leaves=lv, tree=tree, center=cen, side=float(side[0]), corners=corners,
-                resolution=res, join_format=fo)
+                resolution=res, join_format=fo, stored=recs_out)
+
+
+STORE_BLOCK = 32768
+
+
+def store_record(p):
+    """LasPoint::write (las.cpp:177-207) of one Point: 61 bytes."""
+    out = C.create_string_buffer(61)
+    lib().wbo_store_record(C.byref(p), out)
+    return out.raw[:61]
+
+
+def store_blocks(stored, counts):
+    """OctBuffer::write (octree.cpp:427-460) of consecutive leaves: the
+    stored Points in canonical order split by the leaves' counts."""
+    out = bytearray()
+    k = 0
+    for c in counts:
+        arr = (Point * max(int(c), 1))()
+        for j in range(int(c)):
+            arr[j] = stored[k + j]
+        blk = C.create_string_buffer(STORE_BLOCK)
+        assert lib().wbo_store_block(arr, int(c), blk) == 0
+        out += blk.raw[:STORE_BLOCK]
+        k += int(c)
+    return bytes(out)
+
+
+def store_read_block(blk):
+    """OctBuffer::read (octree.cpp:508-559) of one block: list of Points."""
+    arr = (Point * 537)()
+    n = lib().wbo_store_read_block(bytes(blk), arr)
+    return [arr[i] for i in range(n)]
 
 
 def dump_text(tree, x, y, z):

@@ -25,6 +25,19 @@ WB_BUILD_DEDUPE = 2
 WB_WRITE_DRY = 4
 WB_ZEROS_READ, WB_ZEROS_KEEP = 0, 1
 WB_FIT_FP64, WB_FIT_FP32, WB_FIT_FP32_COMP = 0, 1, 2
+STORE_RECORD, STORE_RECORDS, STORE_BLOCK = 61, 537, 32768  # octree.h:35-43
+
+
+def _class_arg(cls, n):
+    """A host class-byte array of n entries as a ctypes pointer (None
+    passes through); the array must outlive the call."""
+    if cls is None:
+        return None
+    a = np.ascontiguousarray(cls, dtype=np.uint8)
+    if a.size != n:
+        raise ValueError(f"class bytes: {a.size} for {n} input points")
+    _class_arg.keep = a
+    return a.ctypes.data_as(_vp)
 
 
 class WbError(RuntimeError):
@@ -156,6 +169,10 @@ SIGNATURES = {
     "wb_cloud_classify": (C.c_int, [_vp, _vp, C.POINTER(ClassifyParams), _vp, C.c_int,
                                     C.POINTER(TileRecord), C.c_uint64, C.POINTER(Stats)]),
     "wb_cloud_free": (C.c_int, [_vp]),
+    "wb_store_blocks": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_uint64, _u64p]),
+    "wb_store_write_files": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_char_p, C.c_int]),
+    "wb_store_read_blocks": (C.c_int, [_vp, _vp, C.c_uint64, _dp, _dp, _dp,
+                                       C.POINTER(C.c_uint16), _u32p, _u64p]),
     "wb_cloud_write": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(LasWriteParams),
                                  C.POINTER(LasWriteStats), C.c_char_p, C.c_uint64]),
 }
@@ -204,6 +221,34 @@ class Context:
 
     def sync(self):
         self.check(lib().wb_ctx_sync(self.h), "wb_ctx_sync")
+
+    def store_read_blocks(self, blocks):
+        """OctBuffer::read (octree.cpp:508-559) of store blocks (bytes or a
+        uint8 array of 32768-byte blocks): (x, y, z, classification,
+        block_count) of the records with a location, block by block."""
+        buf = np.ascontiguousarray(np.frombuffer(blocks, dtype=np.uint8)
+                                   if not isinstance(blocks, np.ndarray) else blocks,
+                                   dtype=np.uint8)
+        if buf.size % STORE_BLOCK:
+            raise ValueError("not a whole number of 32768-byte blocks")
+        nb = buf.size // STORE_BLOCK
+        L = lib()
+        counts = np.zeros(max(nb, 1), dtype=np.uint32)
+        n = C.c_uint64()
+        self.check(L.wb_store_read_blocks(self.h, buf.ctypes.data_as(_vp), nb, None, None, None,
+                                          None, counts.ctypes.data_as(_u32p), C.byref(n)),
+                   "wb_store_read_blocks")
+        m = max(n.value, 1)
+        x, y, z = np.zeros(m), np.zeros(m), np.zeros(m)
+        cls = np.zeros(m, dtype=np.uint16)
+        self.check(L.wb_store_read_blocks(self.h, buf.ctypes.data_as(_vp), nb,
+                                          x.ctypes.data_as(_dp), y.ctypes.data_as(_dp),
+                                          z.ctypes.data_as(_dp),
+                                          cls.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                          counts.ctypes.data_as(_u32p), C.byref(n)),
+                   "wb_store_read_blocks")
+        k = n.value
+        return x[:k], y[:k], z[:k], cls[:k], counts[:nb]
 
     def close(self):
         """Frees the clouds, LAS files and trees still alive on this context,
@@ -350,6 +395,45 @@ class Octree:
         self.ctx.check(L.wb_tree_dump(self.h, path.encode() if path else None, buf, n.value + 1,
                                       C.byref(n)), "wb_tree_dump")
         return buf.raw[:n.value].decode("utf-8")
+
+    def store_blocks(self, cloud=None, cls=None, out=None):
+        """The store's blocks (OctBuffer::write, octree.cpp:427-460) of every
+        leaf in DFS order: nleaves * 32768 bytes into out (a numpy uint8
+        array or a host tensor, e.g. pinned; allocated when None), returned.
+        cloud: the Cloud the tree was built from (its records' fields);
+        cls: class bytes in input order, host array or device tensor (a
+        nonzero byte replaces the classification)."""
+        L = lib()
+        nb = C.c_uint64()
+        ch = cloud.h if cloud is not None else None
+        flags = 0
+        if cls is not None and hasattr(cls, "data_ptr") and getattr(cls, "is_cuda", False):
+            if cls.numel() != self.n_input:
+                raise ValueError(f"class bytes: {cls.numel()} for {self.n_input} input points")
+            cp, flags = _vp(cls.data_ptr()), WB_DEVICE_INPUT
+        else:
+            cp = _class_arg(cls, self.n_input)
+        self.ctx.check(L.wb_store_blocks(self.h, ch, cp, flags, None, 0, C.byref(nb)),
+                       "wb_store_blocks")
+        need = nb.value * STORE_BLOCK
+        if out is None:
+            out = np.zeros(need, dtype=np.uint8)
+        if hasattr(out, "data_ptr"):
+            ptr, cap = out.data_ptr(), out.numel() * out.element_size()
+        else:
+            ptr, cap = out.ctypes.data, out.nbytes
+        self.ctx.check(L.wb_store_blocks(self.h, ch, cp, flags, _vp(ptr), cap, C.byref(nb)),
+                       "wb_store_blocks")
+        return out
+
+    def store_write_files(self, name, nfiles=1, cloud=None, cls=None):
+        """The blocks as the store's files name0 .. name<nfiles-1>
+        (OctStore::open, octree.cpp:812-830): block b in file b % nfiles at
+        offset 32768 * (b // nfiles)."""
+        ch = cloud.h if cloud is not None else None
+        cp = _class_arg(cls, self.n_input)
+        self.ctx.check(lib().wb_store_write_files(self.h, ch, cp, 0, name.encode(), int(nfiles)),
+                       "wb_store_write_files")
 
     def duplicates(self):
         """(merged input index, input index of the first point at its

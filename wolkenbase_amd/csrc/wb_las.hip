@@ -570,6 +570,164 @@ __global__ void __launch_bounds__(64) k_las_encode(
 }
 
 // ---------------------------------------------------------------------------
+// device: the store's block format (octree.h:35-43)
+// ---------------------------------------------------------------------------
+// LasPoint::write (las.cpp:177-207) of one point: 61 little-endian bytes
+__device__ inline void put_bytes(uint8_t *d, uint64_t v, int n)
+{
+  for (int k = 0; k < n; k++) d[k] = (uint8_t)(v >> (8 * k));
+}
+__device__ inline void store_rec61(uint8_t *d, const LasPt &p)
+{
+  put_bytes(d, __double_as_longlong(p.x), 8);
+  put_bytes(d + 8, __double_as_longlong(p.y), 8);
+  put_bytes(d + 16, __double_as_longlong(p.z), 8);
+  put_bytes(d + 24, p.intensity, 2);
+  put_bytes(d + 26, p.ret, 2);
+  put_bytes(d + 28, p.nret, 2);
+  d[30] = (uint8_t)(p.edge * 2 + p.dir);
+  put_bytes(d + 31, p.cls, 2);
+  put_bytes(d + 33, p.flags, 2);
+  put_bytes(d + 35, p.channel, 2);
+  put_bytes(d + 37, p.user, 2);
+  put_bytes(d + 39, p.source, 2);
+  put_bytes(d + 41, (uint32_t)p.angle, 4);
+  put_bytes(d + 45, __double_as_longlong(p.gps), 8);
+  put_bytes(d + 53, p.nir, 2);
+  put_bytes(d + 55, p.red, 2);
+  put_bytes(d + 57, p.green, 2);
+  put_bytes(d + 59, p.blue, 2);
+}
+
+// OctBuffer::write (octree.cpp:427-460) of leaf l as block l: the leaf's
+// stored points in canonical order (with a cloud, the record of the
+// location's last input as readPoint decodes it, with the READ phase's
+// return fix; without, LasPoint's defaults at the point's location; a
+// nonzero class byte replaces the classification, as CLASSIFY's put does),
+// noPoint records (NaN location, zero fields) to 537, zero bytes to 32768.
+// The block is assembled in LDS and copied out in 16-byte words.
+__global__ void __launch_bounds__(64) k_store_blocks(
+    const uint32_t *leaf_start, uint32_t nl, const uint32_t *canon, const uint32_t *sidx,
+    const uint32_t *last_of, const uint8_t *cls, const LasSrc *files, int nfiles,
+    const uint32_t *recidx, double unit, const double *px, const double *py, const double *pz,
+    uint8_t *out)
+{
+  __shared__ uint4 blk[WB_STORE_BLOCK / 16];
+  const uint32_t l = blockIdx.x;
+  if (l >= nl) return;
+  uint8_t *bb = (uint8_t *)blk;
+  const uint32_t b = leaf_start[l], m = leaf_start[l + 1] - b;
+  for (uint32_t r = threadIdx.x; r < WB_STORE_RECORDS; r += blockDim.x) {
+    LasPt p;
+    p.x = p.y = p.z = __builtin_nan("");
+    p.angle = 0;
+    p.gps = 0;
+    p.intensity = p.ret = p.nret = p.cls = p.flags = p.channel = p.user = p.source = 0;
+    p.nir = p.red = p.green = p.blue = 0;
+    p.dir = p.edge = 0;
+    if (r < m) {
+      const uint32_t sl = canon[b + r];
+      const uint32_t first = sidx[sl], last = last_of ? last_of[sl] : first;
+      const uint8_t c = cls ? cls[first] : 0;
+      if (files) {
+        stored_point(files, nfiles, recidx, unit, last, c, p);
+      } else {
+        p.x = px[sl]; p.y = py[sl]; p.z = pz[sl];
+        p.cls = c;
+      }
+    }
+    store_rec61(bb + (size_t)r * WB_STORE_RECORD, p);
+  }
+  for (uint32_t k = WB_STORE_RECORDS * WB_STORE_RECORD + threadIdx.x; k < WB_STORE_BLOCK;
+       k += blockDim.x)
+    bb[k] = 0;
+  __syncthreads();
+  uint4 *dst = (uint4 *)(out + (size_t)l * WB_STORE_BLOCK);
+  for (uint32_t k = threadIdx.x; k < WB_STORE_BLOCK / 16; k += blockDim.x) dst[k] = blk[k];
+}
+
+__device__ inline uint64_t get_bytes(const uint8_t *s, int n)
+{
+  uint64_t v = 0;
+  for (int k = 0; k < n; k++) v |= (uint64_t)s[k] << (8 * k);
+  return v;
+}
+
+// OctBuffer::read (octree.cpp:508-559) of block b, first half: the records
+// with a location (LasPoint::isEmpty: a NaN coordinate), counted; a block
+// whose first two kept locations are equal (a block never written reads as
+// all zeros) keeps none.  One wave per block.
+__global__ void __launch_bounds__(64) k_store_count(const uint8_t *blocks, uint64_t nblocks,
+                                                    uint32_t *count)
+{
+  const uint64_t bk = blockIdx.x;
+  if (bk >= nblocks) return;
+  const uint8_t *base = blocks + bk * WB_STORE_BLOCK;
+  uint32_t n = 0, found = 0;
+  double f[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  for (uint32_t r0 = 0; r0 < WB_STORE_RECORDS; r0 += 64) {
+    const uint32_t r = r0 + threadIdx.x;
+    bool keep = false;
+    double x = 0, y = 0, z = 0;
+    if (r < WB_STORE_RECORDS) {
+      const uint8_t *d = base + (size_t)r * WB_STORE_RECORD;
+      x = __longlong_as_double((long long)get_bytes(d, 8));
+      y = __longlong_as_double((long long)get_bytes(d + 8, 8));
+      z = __longlong_as_double((long long)get_bytes(d + 16, 8));
+      keep = !(isnan(x) || isnan(y) || isnan(z));
+    }
+    // the block's first two kept locations
+    uint64_t km = __ballot(keep);
+    n += (uint32_t)__popcll(km);
+    while (found < 2 && km) {
+      const int ln = __ffsll((long long)km) - 1;
+      f[found][0] = __shfl(x, ln, 64);
+      f[found][1] = __shfl(y, ln, 64);
+      f[found][2] = __shfl(z, ln, 64);
+      found++;
+      km &= km - 1;
+    }
+  }
+  if (n > 1 && f[0][0] == f[1][0] && f[0][1] == f[1][1] && f[0][2] == f[1][2]) n = 0;
+  if (threadIdx.x == 0) count[bk] = n;
+}
+
+// second half: the kept records of block b at pos[b], in record order
+__global__ void __launch_bounds__(64) k_store_points(const uint8_t *blocks, uint64_t nblocks,
+                                                     const uint32_t *count, const uint32_t *pos,
+                                                     double *x, double *y, double *z,
+                                                     uint16_t *cls)
+{
+  const uint64_t bk = blockIdx.x;
+  if (bk >= nblocks || count[bk] == 0) return;
+  const uint8_t *base = blocks + bk * WB_STORE_BLOCK;
+  uint32_t at = pos[bk];
+  for (uint32_t r0 = 0; r0 < WB_STORE_RECORDS; r0 += 64) {
+    const uint32_t r = r0 + threadIdx.x;
+    bool keep = false;
+    double vx = 0, vy = 0, vz = 0;
+    uint16_t c = 0;
+    if (r < WB_STORE_RECORDS) {
+      const uint8_t *d = base + (size_t)r * WB_STORE_RECORD;
+      vx = __longlong_as_double((long long)get_bytes(d, 8));
+      vy = __longlong_as_double((long long)get_bytes(d + 8, 8));
+      vz = __longlong_as_double((long long)get_bytes(d + 16, 8));
+      c = (uint16_t)get_bytes(d + 31, 2);
+      keep = !(isnan(vx) || isnan(vy) || isnan(vz));
+    }
+    const uint64_t km = __ballot(keep);
+    if (keep) {
+      const uint32_t o = at + (uint32_t)__popcll(km & ((1ull << threadIdx.x) - 1));
+      x[o] = vx;
+      y[o] = vy;
+      z[o] = vz;
+      if (cls) cls[o] = c;
+    }
+    at += (uint32_t)__popcll(km);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host: the reference's small helpers
 // ---------------------------------------------------------------------------
 // joinPointFormat (las.cpp:104-121): the least format with every feature
@@ -1326,3 +1484,139 @@ void wb_release_las(wb_ctx *c)
     wb_cloud_free(cl);
   for (wb_las *L : std::vector<wb_las *>(c->las.begin(), c->las.end())) wb_las_close(L);
 }
+
+// the store blocks of tree t into device memory dst (nl blocks)
+static void store_blocks_device(wb_tree *t, wb_cloud *cl, const uint8_t *cls, int flags,
+                                uint8_t *dst)
+{
+  wb_ctx *c = t->ctx;
+  const uint8_t *dcls = nullptr;
+  uint8_t *dcls_own = nullptr;
+  const uint64_t nin = t->n_input;
+  if (cls && !(flags & WB_DEVICE_INPUT)) {
+    dcls_own = (uint8_t *)wb_alloc(c, nin + 16);
+    WB_CHECK(hipMemcpyAsync(dcls_own, cls, nin, hipMemcpyHostToDevice, c->stream));
+    dcls = dcls_own;
+  } else {
+    dcls = cls;
+  }
+  if (t->nleaves)
+    hipLaunchKernelGGL(k_store_blocks, dim3(t->nleaves), dim3(64), 0, c->stream, t->leaf_start,
+                       t->nleaves, t->canon, t->sidx, t->last_of, dcls, cl ? cl->d_src : nullptr,
+                       cl ? (int)cl->files.size() : 0, cl ? cl->rec : nullptr,
+                       cl ? cl->unit : 1.0, t->px, t->py, t->pz, dst);
+  WB_CHECK(hipGetLastError());
+  WB_CHECK(hipStreamSynchronize(c->stream));
+  if (dcls_own) wb_free(c, dcls_own, nin + 16);
+}
+
+static int store_check(wb_tree *t, wb_cloud *cl)
+{
+  if (!t) return WB_EINVAL;
+  if (cl && (cl->ctx != t->ctx || cl->n != t->n_input))
+    return fail(t->ctx, "tree not built from this cloud", WB_EINVAL);
+  return WB_OK;
+}
+
+extern "C" {
+
+int wb_store_blocks(wb_tree *t, wb_cloud *cl, const uint8_t *cls, int flags, void *out,
+                    uint64_t cap, uint64_t *nblocks)
+{
+  const int rc = store_check(t, cl);
+  if (rc) return rc;
+  wb_ctx *c = t->ctx;
+  const uint64_t nl = t->nleaves, bytes = nl * WB_STORE_BLOCK;
+  if (nblocks) *nblocks = nl;
+  if (!out) return WB_OK;
+  if (cap < bytes) return fail(c, "output smaller than the blocks", WB_ERANGE);
+  API_TRY(c)
+  WB_CHECK(hipSetDevice(c->device));
+  if (nl) {
+    uint8_t *d = (uint8_t *)wb_alloc(c, bytes);
+    store_blocks_device(t, cl, cls, flags, d);
+    WB_CHECK(hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, c->stream));
+    WB_CHECK(hipStreamSynchronize(c->stream));
+    wb_free(c, d, bytes);
+  }
+  return WB_OK;
+  API_CATCH(c)
+}
+
+int wb_store_write_files(wb_tree *t, wb_cloud *cl, const uint8_t *cls, int flags,
+                         const char *name, int nfiles)
+{
+  const int rc = store_check(t, cl);
+  if (rc) return rc;
+  wb_ctx *c = t->ctx;
+  if (!name) return WB_EINVAL;
+  if (nfiles < 1) nfiles = 1;  // OctStore::open
+  API_TRY(c)
+  WB_CHECK(hipSetDevice(c->device));
+  const uint64_t nl = t->nleaves, bytes = nl * WB_STORE_BLOCK;
+  std::vector<uint8_t> h(bytes ? bytes : 1);
+  if (nl) {
+    uint8_t *d = (uint8_t *)wb_alloc(c, bytes);
+    store_blocks_device(t, cl, cls, flags, d);
+    WB_CHECK(hipMemcpyAsync(h.data(), d, bytes, hipMemcpyDeviceToHost, c->stream));
+    WB_CHECK(hipStreamSynchronize(c->stream));
+    wb_free(c, d, bytes);
+  }
+  // block b in file name<b % nfiles> at BLOCKSIZE * (b / nfiles)
+  for (int f = 0; f < nfiles; f++) {
+    const std::string fn = std::string(name) + std::to_string(f);
+    FILE *fp = fopen(fn.c_str(), "wb");
+    if (!fp) return fail(c, "cannot create " + fn, WB_EINVAL);
+    bool ok = true;
+    for (uint64_t bk = f; bk < nl && ok; bk += nfiles)
+      ok = fwrite(h.data() + bk * WB_STORE_BLOCK, 1, WB_STORE_BLOCK, fp) == WB_STORE_BLOCK;
+    if (fclose(fp) != 0 || !ok) return fail(c, "write failed: " + fn, WB_EINVAL);
+  }
+  return WB_OK;
+  API_CATCH(c)
+}
+
+int wb_store_read_blocks(wb_ctx *c, const void *blocks, uint64_t nblocks, double *x, double *y,
+                         double *z, uint16_t *classification, uint32_t *block_count,
+                         uint64_t *npoints)
+{
+  if (!c || (nblocks && !blocks) || !npoints) return WB_EINVAL;
+  if (nblocks > 0xffffffffull / WB_STORE_RECORDS) return fail(c, "too many blocks", WB_ERANGE);
+  API_TRY(c)
+  WB_CHECK(hipSetDevice(c->device));
+  *npoints = 0;
+  if (!nblocks) return WB_OK;
+  const uint64_t bytes = nblocks * WB_STORE_BLOCK;
+  uint8_t *d = (uint8_t *)wb_alloc(c, bytes);
+  uint32_t *cnt = (uint32_t *)wb_alloc(c, 4 * nblocks), *pos = (uint32_t *)wb_alloc(c, 4 * nblocks);
+  WB_CHECK(hipMemcpyAsync(d, blocks, bytes, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_store_count, dim3((unsigned)nblocks), dim3(64), 0, c->stream, d, nblocks,
+                     cnt);
+  WB_CHECK(hipMemcpyAsync(pos, cnt, 4 * nblocks, hipMemcpyDeviceToDevice, c->stream));
+  const uint64_t n = wb_exclusive_scan_u32(c, pos, nblocks);
+  *npoints = n;
+  if (block_count)
+    WB_CHECK(hipMemcpyAsync(block_count, cnt, 4 * nblocks, hipMemcpyDeviceToHost, c->stream));
+  if (n && (x || y || z || classification)) {
+    double *dx = (double *)wb_alloc(c, 24 * n);
+    uint16_t *dc = (uint16_t *)wb_alloc(c, 2 * n + 16);
+    hipLaunchKernelGGL(k_store_points, dim3((unsigned)nblocks), dim3(64), 0, c->stream, d, nblocks,
+                       cnt, pos, dx, dx + n, dx + 2 * n, dc);
+    if (x) WB_CHECK(hipMemcpyAsync(x, dx, 8 * n, hipMemcpyDeviceToHost, c->stream));
+    if (y) WB_CHECK(hipMemcpyAsync(y, dx + n, 8 * n, hipMemcpyDeviceToHost, c->stream));
+    if (z) WB_CHECK(hipMemcpyAsync(z, dx + 2 * n, 8 * n, hipMemcpyDeviceToHost, c->stream));
+    if (classification)
+      WB_CHECK(hipMemcpyAsync(classification, dc, 2 * n, hipMemcpyDeviceToHost, c->stream));
+    WB_CHECK(hipStreamSynchronize(c->stream));
+    wb_free(c, dx, 24 * n);
+    wb_free(c, dc, 2 * n + 16);
+  }
+  WB_CHECK(hipStreamSynchronize(c->stream));
+  wb_free(c, d, bytes);
+  wb_free(c, cnt, 4 * nblocks);
+  wb_free(c, pos, 4 * nblocks);
+  return WB_OK;
+  API_CATCH(c)
+}
+
+}  // extern "C"
