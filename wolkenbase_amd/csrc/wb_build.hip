@@ -853,17 +853,19 @@ __global__ void __launch_bounds__(256) k_leaf_canon(const uint32_t *leaf_start, 
 // host driver
 // ---------------------------------------------------------------------------
 // Sort digits for a first attempt (a build that needs more retries with one
-// more digit, so this choice changes only the time): the fewer of
-//   the levels down to cells of ~1 m, plus one, and
-//   the levels a surface of n points needs for leaves of 537 points, each
-//   level quartering a leaf's points, plus two for relief and clustering.
-static int default_passes(double side, uint64_t n)
+// more digit, so this choice changes only the time): the levels down to
+// cells of ~1 m, plus one, or, without the duplicate merge, the fewer of
+// that and the levels a surface of n points needs for leaves of 537 points
+// (each level quartering a leaf's points) plus two for relief and
+// clustering.  The merge searches runs of equal sort prefix for equal
+// locations, so it keeps the finer prefix and its short runs.
+static int default_passes(double side, uint64_t n, bool dedupe)
 {
   int lv = 1;
   while (lv < WB_MAX_LEVEL && side / (double)(1ll << lv) > 1.0) lv++;
   int ls = 2;
   while (ls < WB_MAX_LEVEL && ((double)WB_RECORDS * (double)(1ull << (2 * (ls - 2)))) < (double)n) ls++;
-  const int levels = std::min(lv + 1, ls);
+  const int levels = dedupe ? lv + 1 : std::min(lv + 1, ls);
   const int p = (3 * levels + 7) / 8;
   return std::min(8, std::max(2, p));
 }
@@ -894,7 +896,7 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
   t->sidx = wb_talloc<uint32_t>(t, n);
   hipEvent_t evb[4];
   for (int k = 0; k < 4; k++) WB_CHECK(hipEventCreate(&evb[k]));
-  int passes = t->sort_passes > 0 ? t->sort_passes : default_passes(t->side, n_input);
+  int passes = t->sort_passes > 0 ? t->sort_passes : default_passes(t->side, n_input, dedupe);
   const uint32_t node_cap = (uint32_t)std::min<uint64_t>(n / 64 + 1024, 0x7fffffff);
   t->node_cube = wb_talloc<double>(t, 4 * (size_t)node_cap);
   t->node_child = wb_talloc<int32_t>(t, 8 * (size_t)node_cap);
