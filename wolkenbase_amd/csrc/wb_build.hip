@@ -810,42 +810,69 @@ __global__ void k_chunk_box(const double *px, const double *py, const double *pz
   }
 }
 
-// per-leaf canonical order: input order inside each leaf (bitonic sort of up
-// to 1024 slots in LDS, one block per leaf)
-__global__ void __launch_bounds__(256) k_leaf_canon(const uint32_t *leaf_start, uint32_t nl,
-                                                     const uint32_t *sidx, uint32_t *canon,
-                                                     uint32_t *crank)
+// Per-leaf canonical order, input order inside each leaf: a bitonic sort of
+// the leaf's input indices by a wave per leaf, in registers: leaves of
+// 64(K/2)+1 .. 64K points (K = 1: up to 64) hold K keys per lane, element
+// e = 64s + lane in slot s; the bitonic network's exchanges across lanes
+// are shuffles, those within a lane (partner distance 64 or more) register
+// swaps.  No barrier, no LDS.  Keys (input indices) are distinct.
+template <int K>
+__global__ void __launch_bounds__(256) k_leaf_canon_reg(const uint32_t *leaf_start, uint32_t nl,
+                                                        const uint32_t *sidx, uint32_t *canon,
+                                                        uint32_t *crank)
 {
-  __shared__ uint32_t sk[1024], sv[1024];
-  uint32_t l = blockIdx.x;
-  if (l >= nl) return;
-  uint32_t s = leaf_start[l], e = leaf_start[l + 1], m = e - s;
-  uint32_t P = 1;
-  while (P < m) P <<= 1;
-  for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
-    sk[i] = (i < m) ? sidx[s + i] : 0xffffffffu;
-    sv[i] = s + i;
-  }
-  __syncthreads();
-  for (uint32_t k = 2; k <= P; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
-        uint32_t ixj = i ^ j;
-        if (ixj > i) {
-          bool up = (i & k) == 0;
-          uint32_t a = sk[i], b = sk[ixj];
-          if ((a > b) == up) {
-            sk[i] = b; sk[ixj] = a;
-            uint32_t t = sv[i]; sv[i] = sv[ixj]; sv[ixj] = t;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t l = blockIdx.x * 4 + threadIdx.x / 64; l < nl; l += gridDim.x * 4) {
+    const uint32_t b = leaf_start[l], m = leaf_start[l + 1] - b;
+    if (m <= (K == 1 ? 0u : 32u * K) || m > 64u * K) continue;
+    uint32_t key[K], val[K];
+#pragma unroll
+    for (int s = 0; s < K; s++) {
+      const uint32_t e = 64u * s + lane;
+      key[s] = e < m ? sidx[b + e] : 0xffffffffu;
+      val[s] = b + e;
+    }
+#pragma unroll
+    for (int k = 2; k <= 64 * K; k <<= 1) {
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        if (j < 64) {
+#pragma unroll
+          for (int s = 0; s < K; s++) {
+            const uint32_t e = 64u * s + lane;
+            const uint32_t ok = (uint32_t)__shfl_xor((int)key[s], j, 64);
+            const uint32_t ov = (uint32_t)__shfl_xor((int)val[s], j, 64);
+            const bool up = (e & (uint32_t)k) == 0, low = (e & (uint32_t)j) == 0;
+            // the lower element of a pair keeps the smaller key when sorting up
+            const bool take = (low == up) ? ok < key[s] : ok > key[s];
+            key[s] = take ? ok : key[s];
+            val[s] = take ? ov : val[s];
+          }
+        } else {
+#pragma unroll
+          for (int s = 0; s < K; s++) {
+            const int s2 = s ^ (j / 64);
+            if (s2 > s) {
+              const uint32_t e = 64u * s + lane;
+              const bool up = (e & (uint32_t)k) == 0;
+              if ((key[s] > key[s2]) == up) {
+                const uint32_t tk = key[s], tv = val[s];
+                key[s] = key[s2]; val[s] = val[s2];
+                key[s2] = tk; val[s2] = tv;
+              }
+            }
           }
         }
       }
-      __syncthreads();
     }
-  }
-  for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
-    canon[s + r] = sv[r];
-    crank[sv[r]] = s + r;
+#pragma unroll
+    for (int s = 0; s < K; s++) {
+      const uint32_t e = 64u * s + lane;
+      if (e < m) {
+        canon[b + e] = val[s];
+        crank[val[s]] = b + e;
+      }
+    }
   }
 }
 
@@ -1091,8 +1118,20 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
     src.packed = nullptr;
     hipLaunchKernelGGL(k_chunk_box, dim3(wb_blocks(nchunks * WB_CHUNK, 256)), dim3(256), 0,
                        c->stream, t->px, t->py, t->pz, n, t->chunk_box, t->chunk_low);
-    hipLaunchKernelGGL(k_leaf_canon, dim3(nl), dim3(256), 0, c->stream, t->leaf_start, nl,
-                       t->sidx, t->canon, t->crank);
+    {
+      // a wave per leaf, by size class (every leaf holds at most 537 points)
+      const unsigned nb = std::min<unsigned>(wb_blocks(nl, 4), 4096);
+      hipLaunchKernelGGL(k_leaf_canon_reg<1>, dim3(nb), dim3(256), 0, c->stream, t->leaf_start,
+                         nl, t->sidx, t->canon, t->crank);
+      hipLaunchKernelGGL(k_leaf_canon_reg<2>, dim3(nb), dim3(256), 0, c->stream, t->leaf_start,
+                         nl, t->sidx, t->canon, t->crank);
+      hipLaunchKernelGGL(k_leaf_canon_reg<4>, dim3(nb), dim3(256), 0, c->stream, t->leaf_start,
+                         nl, t->sidx, t->canon, t->crank);
+      hipLaunchKernelGGL(k_leaf_canon_reg<8>, dim3(nb), dim3(256), 0, c->stream, t->leaf_start,
+                         nl, t->sidx, t->canon, t->crank);
+      hipLaunchKernelGGL(k_leaf_canon_reg<16>, dim3(nb), dim3(256), 0, c->stream, t->leaf_start,
+                         nl, t->sidx, t->canon, t->crank);
+    }
     WB_CHECK(hipEventRecord(evb[3], c->stream));
     WB_CHECK(hipEventSynchronize(evb[3]));
     float a = 0, b = 0;
