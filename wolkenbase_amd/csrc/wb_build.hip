@@ -145,11 +145,13 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K *kin, const u
 {
   __shared__ uint32_t wcnt[4][256];
   __shared__ uint32_t dstart[256];
+  __shared__ uint32_t dbase[256];  // the block's output start of each digit
   __shared__ K skey[RS_TILE];
   __shared__ uint32_t sval[RS_TILE];
   const int w = threadIdx.x / WAVE;
   const unsigned lane = lane_id();
   for (int d = threadIdx.x; d < 4 * 256; d += RS_THREADS) (&wcnt[0][0])[d] = 0;
+  for (int d = threadIdx.x; d < 256; d += RS_THREADS) dbase[d] = offs[(uint64_t)d * nblocks + blockIdx.x];
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * RS_TILE + (uint64_t)w * (RS_TILE / 4);
   K key[RS_ITEMS];
@@ -215,7 +217,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K *kin, const u
   for (uint32_t j = threadIdx.x; j < cnt; j += RS_THREADS) {
     const K k = skey[j];
     const uint32_t d = (unsigned)(k >> shift) & 255u;
-    const uint64_t pos = (uint64_t)offs[(uint64_t)d * nblocks + blockIdx.x] + (j - dstart[d]);
+    const uint64_t pos = (uint64_t)dbase[d] + (j - dstart[d]);
     kout[pos] = k;
     vout[pos] = sval[j];
   }
