@@ -686,15 +686,45 @@ __global__ void k_fill_pt_leaf(const uint32_t *leaf_start, uint32_t nl, uint32_t
 // ---------------------------------------------------------------------------
 template <bool F64>
 __global__ void k_permute(Src src, const uint32_t *sidx, uint64_t n, Xform f, double *px,
-                          double *py, double *pz)
+                          double *py, double *pz, double *box, uint32_t *low)
 {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double x, y, z;
-  load_packed<F64>(src, f, sidx[i], x, y, z);
-  px[i] = x;
-  py[i] = y;
-  pz[i] = z;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = i < n;
+  double x = 0, y = 0, z = 0;
+  if (ok) {
+    load_packed<F64>(src, f, sidx[i], x, y, z);
+    px[i] = x;
+    py[i] = y;
+    pz[i] = z;
+  }
+  // the box and the lowest point (smallest z, then smallest slot) of each
+  // 32-slot group, by half-wave butterflies over the values just gathered
+  double x0 = ok ? x : __builtin_inf(), x1 = ok ? x : -__builtin_inf();
+  double y0 = ok ? y : __builtin_inf(), y1 = ok ? y : -__builtin_inf();
+  double z0 = ok ? z : __builtin_inf(), z1 = ok ? z : -__builtin_inf();
+  double lz = z0;
+  uint32_t li = (uint32_t)i;
+#pragma unroll
+  for (int d = 1; d < WB_CHUNK; d <<= 1) {
+    const double oz = __shfl_xor(lz, d, WAVE);
+    const uint32_t oi = __shfl_xor(li, d, WAVE);
+    if (oz < lz || (oz == lz && oi < li)) {
+      lz = oz;
+      li = oi;
+    }
+    x0 = fmin(x0, __shfl_xor(x0, d, WAVE));
+    x1 = fmax(x1, __shfl_xor(x1, d, WAVE));
+    y0 = fmin(y0, __shfl_xor(y0, d, WAVE));
+    y1 = fmax(y1, __shfl_xor(y1, d, WAVE));
+    z0 = fmin(z0, __shfl_xor(z0, d, WAVE));
+    z1 = fmax(z1, __shfl_xor(z1, d, WAVE));
+  }
+  const uint64_t chunk = i / WB_CHUNK;
+  if ((i % WB_CHUNK) == 0 && chunk < (n + WB_CHUNK - 1) / WB_CHUNK) {
+    double *b = box + 6 * chunk;
+    b[0] = x0; b[1] = x1; b[2] = y0; b[3] = y1; b[4] = z0; b[5] = z1;
+    low[chunk] = li;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -772,43 +802,6 @@ __global__ void k_dup_compact(const uint64_t *key, const uint32_t *sidx, const u
     const uint64_t d = s - pos[s];
     pairs[2 * d] = sidx[s];
     pairs[2 * d + 1] = sidx[g];
-  }
-}
-
-__global__ void k_chunk_box(const double *px, const double *py, const double *pz, uint64_t n,
-                            double *box, uint32_t *low)
-{
-  // one half-wave (32 lanes) per chunk
-  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t chunk = g / WB_CHUNK;
-  uint64_t i = g;
-  bool ok = i < n;
-  double x0 = ok ? px[i] : __builtin_inf(), x1 = ok ? px[i] : -__builtin_inf();
-  double y0 = ok ? py[i] : __builtin_inf(), y1 = ok ? py[i] : -__builtin_inf();
-  double z0 = ok ? pz[i] : __builtin_inf(), z1 = ok ? pz[i] : -__builtin_inf();
-  // lowest point: smallest z, then smallest slot
-  double lz = z0;
-  uint32_t li = (uint32_t)i;
-#pragma unroll
-  for (int d = 1; d < WB_CHUNK; d <<= 1) {
-    const double oz = __shfl_xor(lz, d, WAVE);
-    const uint32_t oi = __shfl_xor(li, d, WAVE);
-    if (oz < lz || (oz == lz && oi < li)) {
-      lz = oz;
-      li = oi;
-    }
-    x0 = fmin(x0, __shfl_xor(x0, d, WAVE));
-    x1 = fmax(x1, __shfl_xor(x1, d, WAVE));
-    y0 = fmin(y0, __shfl_xor(y0, d, WAVE));
-    y1 = fmax(y1, __shfl_xor(y1, d, WAVE));
-    z0 = fmin(z0, __shfl_xor(z0, d, WAVE));
-    z1 = fmax(z1, __shfl_xor(z1, d, WAVE));
-  }
-  uint64_t nchunks = (n + WB_CHUNK - 1) / WB_CHUNK;
-  if ((g % WB_CHUNK) == 0 && chunk < nchunks) {
-    double *b = box + 6 * chunk;
-    b[0] = x0; b[1] = x1; b[2] = y0; b[3] = y1; b[4] = z0; b[5] = z1;
-    low[chunk] = li;
   }
 }
 
@@ -1111,15 +1104,15 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
     hipLaunchKernelGGL(k_fill_pt_leaf, dim3(nl), dim3(256), 0, c->stream, t->leaf_start, nl,
                        t->pt_leaf);
     if (f64)
-      hipLaunchKernelGGL(k_permute<true>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, src,
-                         t->sidx, n, f, t->px, t->py, t->pz);
+      hipLaunchKernelGGL(k_permute<true>, dim3(wb_blocks(nchunks * WB_CHUNK, 256)), dim3(256), 0,
+                         c->stream, src, t->sidx, n, f, t->px, t->py, t->pz, t->chunk_box,
+                         t->chunk_low);
     else
-      hipLaunchKernelGGL(k_permute<false>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, src,
-                         t->sidx, n, f, t->px, t->py, t->pz);
+      hipLaunchKernelGGL(k_permute<false>, dim3(wb_blocks(nchunks * WB_CHUNK, 256)), dim3(256), 0,
+                         c->stream, src, t->sidx, n, f, t->px, t->py, t->pz, t->chunk_box,
+                         t->chunk_low);
     wb_free(c, src.packed, pk_bytes);
     src.packed = nullptr;
-    hipLaunchKernelGGL(k_chunk_box, dim3(wb_blocks(nchunks * WB_CHUNK, 256)), dim3(256), 0,
-                       c->stream, t->px, t->py, t->pz, n, t->chunk_box, t->chunk_low);
     {
       // a wave per leaf, by size class (every leaf holds at most 537 points)
       const unsigned nb = std::min<unsigned>(wb_blocks(nl, 4), 4096);
