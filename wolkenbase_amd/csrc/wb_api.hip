@@ -423,6 +423,7 @@ static int run_pass(wb_tree *t, const double *corners, int ncorners,
                     wb_tile *tiles_out, uint64_t tiles_cap, wb_stats *stats, bool scan_only)
 {
   wb_ctx *c = t->ctx;
+  if (!scan_only && !class_out) return fail(c, "no class output", WB_EINVAL);
   if (!(p->tile_size > 0) || !(p->max_slope > 0) || !(p->min_hs >= 0) || !isfinite(p->thickness))
     return fail(c, "bad classification parameters", WB_EINVAL);
   if (p->fit_precision < WB_FIT_FP64 || p->fit_precision > WB_FIT_FP32_COMP)

@@ -1094,6 +1094,7 @@ struct ClassifyArgs {
   const float *edges;    // bin-edge directions (wb::fill_edge_table)
   wb::AtanConst K;
   uint8_t *cls;          // per sorted slot
+  uint8_t *cls_in;       // per input index (the caller's class bytes)
   uint32_t *nslow;       // points decided by the exact gap pass
   HeavyRec *heavy;       // points handed to the wave-per-point kernel
   uint32_t *nheavy;
@@ -1541,7 +1542,11 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
     }
   }
   if (nexact) atomicAdd(A.atan_exact, (unsigned long long)nexact);
-  if (inrange && state < 5) A.cls[i] = state == 1 ? 1 : (state == 2 ? 2 : 0);
+  if (inrange && state < 5) {
+    const uint8_t v = state == 1 ? 1 : (state == 2 ? 2 : 0);
+    A.cls[i] = v;
+    A.cls_in[T.sidx[i]] = v;
+  }
   if (COUNT) {
     if (state == 7) cnt[CT_LANES_C]++;
     if (state >= 5) cnt[CT_HEAVY]++;
@@ -1853,6 +1858,7 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
     }
   if (lane == 0) {
     A.cls[i] = (uint8_t)verdict;
+    A.cls_in[T.sidx[i]] = (uint8_t)verdict;
     if (nexact) atomicAdd(A.atan_exact, (unsigned long long)nexact);
     if (COUNT) {
       // per start walk: points, passes, leaves, slots, expansions, ticks,
@@ -1869,13 +1875,6 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
       atomicAdd(&h[7], (unsigned long long)c_in);
     }
   }
-}
-
-__global__ void k_scatter_class(uint64_t n, const uint8_t *cls, const uint32_t *sidx, uint8_t *out)
-{
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  out[sidx[i]] = cls[i];
 }
 
 // an input merged into an earlier point at its location gets that point's class
@@ -2051,7 +2050,7 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
     A.T = T; A.g = g; A.memb = memb; A.nmemb = nmemb; A.rec = rec;
     A.thickness = p->thickness; A.max_slope = p->max_slope;
     A.cube_wedges = t->outside_root ? 0 : 1;
-    A.tab = c->d_tan; A.K = c->kconst; A.cls = cls; A.nslow = nslow;
+    A.tab = c->d_tan; A.K = c->kconst; A.cls = cls; A.cls_in = d_class_out; A.nslow = nslow;
     A.edges = c->d_edges;
     A.atan_exact = ctr + 1;
     WB_CHECK(hipEventRecord(ev[4], st));
@@ -2094,8 +2093,6 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
                          nground, partial, want_tiles ? 1 : 0);
       hipLaunchKernelGGL(k_sum_hist, dim3(1), dim3(256), 0, st, partial, nbk, ctr + 2);
       wb_free(c, partial, 12ull * nbk);
-      hipLaunchKernelGGL(k_scatter_class, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, cls,
-                         t->sidx, d_class_out);
       if (t->ndup)
         hipLaunchKernelGGL(k_dup_class, dim3(wb_blocks(t->ndup, 256)), dim3(256), 0, st, t->ndup,
                            t->dup_pairs, d_class_out);
