@@ -622,12 +622,63 @@ __device__ inline V slot_sel(const V (&v)[K], uint32_t s)
   return r;
 }
 
+// v moved across the lanes of each row of 16 by the DPP control CTRL (every
+// lane active; the controls used here read inside the row)
+template <int CTRL>
+__device__ inline double dpp_d(double v)
+{
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xf, 0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf,
+                                                            false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Pairwise sum of lanes base .. base+sz-1 (sz a wave-uniform power of two up
+// to 64, base a multiple of sz): lanes 2i and 2i+1 are added, then pairs of
+// those sums, and so on.  Each level inside a row pairs every lane with a
+// lane of the sibling group (quad_perm 1,0,3,2 and 2,3,0,1, then the
+// half-row and row mirrors), whose lanes all hold that group's sum, so
+// every addition has the operands of the xor butterfly; the row sums are
+// then read and added in the same tree.
+__device__ inline double lanes_pairwise(double b, uint32_t sz, uint32_t base)
+{
+  if (sz >= 2) b = b + dpp_d<0xb1>(b);
+  if (sz >= 4) b = b + dpp_d<0x4e>(b);
+  if (sz >= 8) b = b + dpp_d<0x141>(b);
+  if (sz >= 16) b = b + dpp_d<0x140>(b);
+  if (sz == 64) return (rdl(b, 0) + rdl(b, 16)) + (rdl(b, 32) + rdl(b, 48));
+  if (sz == 32) return rdl(b, base) + rdl(b, base + 16);
+  return rdl(b, base);
+}
+
+// minimum and maximum over the wave: the row steps by DPP, the four row
+// results read
+__device__ inline double wave_dmin(double v)
+{
+  v = fmin(v, dpp_d<0xb1>(v));
+  v = fmin(v, dpp_d<0x4e>(v));
+  v = fmin(v, dpp_d<0x141>(v));
+  v = fmin(v, dpp_d<0x140>(v));
+  return fmin(fmin(rdl(v, 0), rdl(v, 16)), fmin(rdl(v, 32), rdl(v, 48)));
+}
+__device__ inline double wave_dmax(double v)
+{
+  v = fmax(v, dpp_d<0xb1>(v));
+  v = fmax(v, dpp_d<0x4e>(v));
+  v = fmax(v, dpp_d<0x141>(v));
+  v = fmax(v, dpp_d<0x140>(v));
+  return fmax(fmax(rdl(v, 0), rdl(v, 16)), fmax(rdl(v, 32), rdl(v, 48)));
+}
+
 // wave_pairwise for m <= 64*K values held K per lane (value e in slot e/64
-// of lane e%64): the same block trees and block order.
+// of lane e%64): the same block trees and block order.  A block of up to 64
+// values lies in one slot, aligned to its size; a larger one is the
+// balanced tree over its whole slots.
 template <int K>
 __device__ inline double wave_pairwise_reg(uint32_t m, const double (&v)[K])
 {
-  const int lane = __lane_id();
+  static_assert(K <= 4, "blocks of up to 256 values");
   double acc = 0;
   for (int k = 0; k < 9; k++) {
     if (!((m >> k) & 1u)) continue;
@@ -635,23 +686,18 @@ __device__ inline double wave_pairwise_reg(uint32_t m, const double (&v)[K])
     const uint32_t sz = 1u << k;
     double blk;
     if (sz <= WAVE) {
-      // the block lies inside slot off/64
-      const double vs = slot_sel<K>(v, off / WAVE);
-      double b = __shfl(vs, (int)((off + lane) & (WAVE - 1)), WAVE);
-      b = (uint32_t)lane < sz ? b : 0.0;
-      for (uint32_t s = 1; s < sz; s <<= 1) b = b + __shfl_xor(b, (int)s, WAVE);
-      blk = __shfl(b, 0, WAVE);
+      blk = lanes_pairwise(slot_sel<K>(v, off / WAVE), sz, off & (WAVE - 1));
     } else {
-      // chunk sums of 64 (chunk ch = slot off/64 + ch), then a balanced tree
-      const uint32_t nch = sz / WAVE;
-      double mine = 0;
-      for (uint32_t ch = 0; ch < nch; ch++) {
-        double b = slot_sel<K>(v, off / WAVE + ch);
-        for (int s = 1; s < WAVE; s <<= 1) b = b + __shfl_xor(b, s, WAVE);
-        if ((uint32_t)lane == ch) mine = b;
+      const uint32_t s0 = off / WAVE;
+      const double c0 = lanes_pairwise(slot_sel<K>(v, s0), WAVE, 0);
+      const double c1 = lanes_pairwise(slot_sel<K>(v, s0 + 1), WAVE, 0);
+      if (sz == 2 * WAVE) {
+        blk = c0 + c1;
+      } else {
+        const double c2 = lanes_pairwise(slot_sel<K>(v, s0 + 2), WAVE, 0);
+        const double c3 = lanes_pairwise(slot_sel<K>(v, s0 + 3), WAVE, 0);
+        blk = (c0 + c1) + (c2 + c3);
       }
-      for (uint32_t s = 1; s < nch; s <<= 1) mine = mine + __shfl_xor(mine, (int)s, WAVE);
-      blk = __shfl(mine, 0, WAVE);
     }
     acc += blk;
   }
@@ -826,10 +872,8 @@ __device__ __attribute__((always_inline)) inline void scan_reg(const TreeView &T
       top = fmax(top, u[s]);
     }
   }
-  for (int s = 1; s < WAVE; s <<= 1) {
-    bottom = fmin(bottom, __shfl_xor(bottom, s, WAVE));
-    top = fmax(top, __shfl_xor(top, s, WAVE));
-  }
+  bottom = wave_dmin(bottom);
+  top = wave_dmax(top);
   uint32_t first = 0xffffffffu;
 #pragma unroll
   for (int s = K - 1; s >= 0; s--) {
@@ -840,7 +884,7 @@ __device__ __attribute__((always_inline)) inline void scan_reg(const TreeView &T
 #pragma unroll
   for (int s = 0; s < K; s++)
     if (mine[s] && s * WAVE + lane < first) bottom2 = fmin(bottom2, u[s]);
-  for (int s = 1; s < WAVE; s <<= 1) bottom2 = fmin(bottom2, __shfl_xor(bottom2, s, WAVE));
+  bottom2 = wave_dmin(bottom2);
   if (wb::disinf(bottom2)) bottom2 = bottom;
   // 7-sector histogram of the bottom layer
   uint32_t nbottom = 0, nedge = 0, hist[7] = {0, 0, 0, 0, 0, 0, 0};
