@@ -1736,11 +1736,14 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
           }
         }
         // per popped node: node children and leaf children by rank
+        // (OR over each group of 8 lanes: quad swaps, then the half-row mirror)
         uint32_t rn = h && ck >= 0 ? 1u << t : 0u, rl = h && ck < -1 ? 1u << t : 0u;
-        for (int s2 = 1; s2 < 8; s2 <<= 1) {
-          rn |= __shfl_xor(rn, s2, 64);
-          rl |= __shfl_xor(rl, s2, 64);
-        }
+        rn = dpp_or<0xb1, 0xf>(rn);
+        rl = dpp_or<0xb1, 0xf>(rl);
+        rn = dpp_or<0x4e, 0xf>(rn);
+        rl = dpp_or<0x4e, 0xf>(rl);
+        rn = dpp_or<0x141, 0xf>(rn);
+        rl = dpp_or<0x141, 0xf>(rl);
         // node i's children go above those of nodes i+1..P-1; its leaves
         // after those of nodes 0..i-1
         const int base = sp - P;
