@@ -297,6 +297,28 @@ __device__ inline uint32_t wave_count_add(uint32_t *cnt, bool has, int key)
   return slot;
 }
 
+// The same claim with the atomics of all distinct keys issued together:
+// the peer masks first, then one atomic per key from its first lane, and
+// each lane's base read from that lane.  Lanes with the same key get
+// consecutive slots in lane order.
+__device__ inline uint32_t wave_claim(uint32_t *cnt, bool has, int key)
+{
+  const uint32_t lane = __lane_id();
+  uint64_t todo = __ballot(has), peers = 0;
+  while (todo) {
+    const int kl = __builtin_amdgcn_readlane(key, __ffsll((long long)todo) - 1);
+    const bool same = has && key == kl;
+    const uint64_t m = __ballot(same);
+    if (same) peers = m;
+    todo &= ~m;
+  }
+  const int first = has ? __ffsll((long long)peers) - 1 : (int)lane;
+  uint32_t base = 0;
+  if (has && first == (int)lane) base = atomicAdd(&cnt[key], (uint32_t)__popcll(peers));
+  base = (uint32_t)__shfl((int)base, first, 64);
+  return base + (uint32_t)__popcll(peers & ((1ull << lane) - 1));
+}
+
 // The (leaf, tile) part of the membership test, once per leaf for the
 // lattice cells near it: a window of cells (ex, ey), ex in [x0, x0+w), ey in
 // [y0, y0+h), w*h <= LEAF_WIN, covering every cell whose centre can lie
@@ -426,7 +448,7 @@ __global__ void k_tile_lists(uint64_t n, const int32_t *memb, const uint32_t *nm
   for (uint32_t k = 0; k < MAX_MEMB; k++) {
     const bool has = k < nm;
     const int t = has ? memb[MAX_MEMB * i + k] - lo : 0;
-    const uint32_t slot = wave_count_add(tile_fill, has, t);
+    const uint32_t slot = wave_claim(tile_fill, has, t);
     if (has) lists[tile_off[t] + slot] = crank[i];
   }
 }
