@@ -118,7 +118,7 @@ uint64_t wb_exclusive_scan_u32(wb_ctx *c, uint32_t *data, uint64_t n)
 // store instruction writes whole runs rather than 64 scattered buckets.
 // ---------------------------------------------------------------------------
 #define RS_THREADS 256
-#define RS_ITEMS 16
+#define RS_ITEMS 8
 #define RS_TILE (RS_THREADS * RS_ITEMS)
 
 template <typename K>
