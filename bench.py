@@ -26,6 +26,9 @@ WORKLOADS = {
     "tile20m": ("tile20m: 900 m x 900 m, 25 pulses/m^2 (~20.3M points), seed 2000, "
                 "6-sinusoid ground + 5 embankments (20-45 deg) + 2% outliers to +-50 m, "
                 "box buildings and cone trees (1-3 returns), int32 at 0.001 m"),
+    "tile20m_high": ("tile20m_high: the tile20m pulses, terrain, buildings and trees with its "
+                     "2% outliers all 1-50 m above the surface instead of half of them below "
+                     "(about 55% ground instead of 0.3%), int32 at 0.001 m"),
     "tile1m": ("tile1m: 200 m x 200 m, 25 pulses/m^2 (~1.2M points), seed 1000, "
                "6-sinusoid ground, 40 box buildings, 300 cone trees, int32 at 0.001 m"),
     "las20m": ("las20m: the tile20m points as one LAS 1.4 point-format-1 file (28-byte "
@@ -272,6 +275,85 @@ def main_las(args):
     print(json.dumps(out), flush=True)
 
 
+def run_tile(name, steps, warmup, ctx, pmc_calibrate=False):
+    """Generates workload `name`, puts its int32 x/y/z in HBM and times
+    `steps` build + classify steps after `warmup` untimed ones."""
+    import torch
+
+    import wolkenbase_amd as W
+    from wolkenbase_amd import synth
+
+    t0 = time.perf_counter()
+    cloud = synth.preset(name)
+    n = len(cloud["xi"])
+    log(f"[bench] generated {name}: {n} points in {time.perf_counter() - t0:.1f} s")
+    dev = torch.device("cuda:0")
+    xi = torch.from_numpy(cloud["xi"]).to(dev)
+    yi = torch.from_numpy(cloud["yi"]).to(dev)
+    zi = torch.from_numpy(cloud["zi"]).to(dev)
+    cls = torch.empty(n, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    if pmc_calibrate:
+        a = torch.empty(1 << 28, dtype=torch.uint8, device=dev).fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    center, side = W.Octree.size_fit(cloud["bbmin"], cloud["bbmax"])
+    params = W.default_params()
+
+    def step():
+        tree = W.Octree.build(ctx, xi, yi, zi, cloud["scale"], cloud["offset"], center, side)
+        _, _, st = W.classify(tree, cloud["bbmin"], cloud["bbmax"], params, want_tiles=False,
+                              out=cls)
+        tree.free()
+        return st
+
+    for w in range(warmup):
+        st = step()
+        log(f"[bench] {name} warmup {w + 1}/{warmup}: build {st['ms_build']:.2f} ms, "
+            f"classify kernel {st['ms_classify_kernel']:.2f} ms")
+    torch.cuda.synchronize()
+    ctx.sync()
+    lat, stats = [], []
+    t_start = time.perf_counter()
+    for k in range(steps):
+        ts = time.perf_counter()
+        st = step()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+        stats.append(st)
+        log(f"[bench] {name} step {k + 1}/{steps}: {lat[-1]:.2f} ms")
+    ctx.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    del xi, yi, zi, cls
+    return dict(cloud=cloud, n=n, elapsed=elapsed, lat=lat, stats=stats,
+                value=n * steps / elapsed)
+
+
+def kernel_times(stats):
+    def avg(f):
+        return sum(s[f] for s in stats) / len(stats)
+
+    return {"k_ground": avg("ms_classify_kernel"), "k_ground_heavy": avg("ms_heavy_kernel"),
+            "radix_sort": avg("ms_sort")}
+
+
+def breakdown(stats):
+    return {k: round(sum(s[k] for s in stats) / len(stats), 3) for k in
+            ("ms_build", "ms_sort", "ms_tiles", "ms_scan", "ms_classify",
+             "ms_classify_kernel", "ms_heavy_kernel")}
+
+
+def classes(st):
+    n = int(st["n_class1"] + st["n_class2"] + st["n_unclassified"])
+    return {"ground": int(st["n_class2"]), "not_ground": int(st["n_class1"]),
+            "unclassified": int(st["n_unclassified"]),
+            "ground_share": round(int(st["n_class2"]) / max(1, n), 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,6 +361,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="tile20m", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mix", action="store_true",
+                    help="tile20m: skip the second measurement on tile20m_high")
     ap.add_argument("--store", action="store_true",
                     help="las20m: also time the store's blocks into pinned host memory")
     ap.add_argument("--cpu-window", type=float, default=60.0)
@@ -294,70 +378,20 @@ def main():
     if args.workload == "las20m":
         return main_las(args)
 
-    import numpy as np
     import torch
 
     import wolkenbase_amd as W
-    from wolkenbase_amd import synth
 
-    t0 = time.perf_counter()
-    cloud = synth.preset(args.workload)
-    n = len(cloud["xi"])
-    log(f"[bench] generated {args.workload}: {n} points in {time.perf_counter() - t0:.1f} s")
-
-    dev = torch.device("cuda:0")
-    xi = torch.from_numpy(cloud["xi"]).to(dev)
-    yi = torch.from_numpy(cloud["yi"]).to(dev)
-    zi = torch.from_numpy(cloud["zi"]).to(dev)
-    cls = torch.empty(n, dtype=torch.uint8, device=dev)
+    # torch ships its own HIP runtime: it has to open the device before the
+    # library's runtime does, or it finds no GPU
+    torch.zeros(1, device="cuda:0")
     torch.cuda.synchronize()
-    if args.pmc_calibrate:
-        a = torch.empty(1 << 28, dtype=torch.uint8, device=dev).fill_(1)
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
     ctx = W.Context(0)
-    center, side = W.Octree.size_fit(cloud["bbmin"], cloud["bbmax"])
-    params = W.default_params()
-
-    def step():
-        tree = W.Octree.build(ctx, xi, yi, zi, cloud["scale"], cloud["offset"], center, side)
-        _, _, st = W.classify(tree, cloud["bbmin"], cloud["bbmax"], params, want_tiles=False,
-                              out=cls)
-        tree.free()
-        return st
-
-    for w in range(args.warmup):
-        st = step()
-        log(f"[bench] warmup {w + 1}/{args.warmup}: build {st['ms_build']:.2f} ms, "
-            f"classify kernel {st['ms_classify_kernel']:.2f} ms")
-    torch.cuda.synchronize()
-    ctx.sync()
-    lat, stats = [], []
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        ts = time.perf_counter()
-        st = step()
-        torch.cuda.synchronize()
-        lat.append((time.perf_counter() - ts) * 1e3)
-        stats.append(st)
-        log(f"[bench] step {k + 1}/{args.steps}: {lat[-1]:.2f} ms")
-    ctx.sync()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    value = n * args.steps / elapsed
-
-    def avg(f):
-        return sum(s[f] for s in stats) / len(stats)
-
+    r = run_tile(args.workload, args.steps, args.warmup, ctx, args.pmc_calibrate)
+    cloud, n, stats, lat = r["cloud"], r["n"], r["stats"], r["lat"]
+    value = r["value"]
     st = stats[-1]
-    kern = {
-        "k_ground": avg("ms_classify_kernel"),
-        "k_ground_heavy": avg("ms_heavy_kernel"),
-        "radix_sort": avg("ms_sort"),
-    }
+    kern = kernel_times(stats)
     dominant = max(kern, key=kern.get)
     passes = int(st["sort_passes"])
     # algorithmic bytes per launch (SURVEY 8(d) decomposition): the ground
@@ -389,7 +423,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
+        "ms_per_step": round(r["elapsed"] * 1e3 / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -403,16 +437,33 @@ def main():
         "roofline": roof,
         "pipeline_roofline": {"bytes_per_point": B, "maxdepth": D, "passes_formula": P,
                               "frac_of_8TBs": round(pipeline_frac, 6)},
-        "breakdown_ms": {k: round(avg(k), 3) for k in
-                         ("ms_build", "ms_sort", "ms_tiles", "ms_scan", "ms_classify",
-                          "ms_classify_kernel", "ms_heavy_kernel")},
-        "classes": {"ground": int(st["n_class2"]), "not_ground": int(st["n_class1"]),
-                    "unclassified": int(st["n_unclassified"])},
+        "breakdown_ms": breakdown(stats),
+        "classes": classes(st),
         "exact_paths": {"gap_points": int(st["exact_gap_points"]),
                         "heavy_points": int(st["heavy_points"]),
                         "atan_x87_steps": int(st["atan_exact_calls"]),
                         "sector_borderline": int(st["sector_borderline"])},
     }
+    if args.workload == "tile20m" and not args.no_mix:
+        # the same tile with every outlier above the surface: a realistic
+        # ground share, where most points need the exhaustive search
+        m = run_tile("tile20m_high", max(3, args.steps // 2), max(1, args.warmup // 2), ctx)
+        mk = kernel_times(m["stats"])
+        mst = m["stats"][-1]
+        out["ground_mix"] = {
+            "workload": WORKLOADS["tile20m_high"], "n_points": m["n"],
+            "value": round(m["value"], 1), "unit": "points/s",
+            "ms_per_step": round(m["elapsed"] * 1e3 / len(m["stats"]), 3),
+            "steps": len(m["stats"]),
+            "p50_ms": round(percentile(m["lat"], 50), 3),
+            "p99_ms": round(percentile(m["lat"], 99), 3),
+            "k_ground_ms": round(mk["k_ground"], 3),
+            "k_ground_gbs": round(13.0 * m["n"] / (mk["k_ground"] * 1e-3) / 1e9, 2),
+            "breakdown_ms": breakdown(m["stats"]),
+            "classes": classes(mst),
+            "exact_paths": {"gap_points": int(mst["exact_gap_points"]),
+                            "heavy_points": int(mst["heavy_points"])},
+        }
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline on a {args.cpu_window:g} m window ...")
         out["cpu_baseline"] = cpu_baseline(cloud, args.cpu_window)

@@ -1004,6 +1004,41 @@ int wb_las_close(wb_las *L)
 
 int wb_cloud_free(wb_cloud *cl);
 
+// What wb_cloud_create holds while it works: the two timing events, the
+// per-file position buffers of the zero rule and, until it is handed out,
+// the cloud itself.  The destructor releases whatever is still held, on the
+// refusal and exception paths as on success.
+struct CloudCreateHold {
+  wb_ctx *c;
+  wb_cloud *cl = nullptr;
+  std::vector<uint32_t *> pos;
+  std::vector<uint64_t> pos_n;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  explicit CloudCreateHold(wb_ctx *ctx, int nfiles) : c(ctx), pos(nfiles, nullptr), pos_n(nfiles, 0) {}
+  void drop_pos(int f)
+  {
+    wb_free(c, pos[f], 4 * pos_n[f]);
+    pos[f] = nullptr;
+  }
+  ~CloudCreateHold()
+  {
+    // kernels queued on the stream may still read these buffers
+    hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 2; k++)
+      if (ev[k]) hipEventDestroy(ev[k]);
+    for (size_t f = 0; f < pos.size(); f++)
+      if (pos[f]) drop_pos((int)f);
+    if (cl) {
+      wb_free(c, cl->x, 8 * cl->n + 16);
+      wb_free(c, cl->y, 8 * cl->n + 16);
+      wb_free(c, cl->z, 8 * cl->n + 16);
+      wb_free(c, cl->rec, 4 * cl->n + 16);
+      wb_free(c, cl->d_src, sizeof(LasSrc) * cl->files.size());
+      delete cl;
+    }
+  }
+};
+
 int wb_cloud_create(wb_ctx *c, wb_las *const *files, int nfiles, double unit, int zeros,
                     wb_cloud **out)
 {
@@ -1015,7 +1050,8 @@ int wb_cloud_create(wb_ctx *c, wb_las *const *files, int nfiles, double unit, in
     if (!files[f] || files[f]->ctx != c) return fail(c, "file of another context", WB_EINVAL);
   API_TRY(c)
   WB_CHECK(hipSetDevice(c->device));
-  wb_cloud *cl = new wb_cloud();
+  CloudCreateHold hold(c, nfiles);
+  wb_cloud *cl = hold.cl = new wb_cloud();
   cl->ctx = c;
   cl->unit = unit;
   cl->zeros = zeros;
@@ -1023,10 +1059,10 @@ int wb_cloud_create(wb_ctx *c, wb_las *const *files, int nfiles, double unit, in
   // per file: records kept, and the READ phase's zero rule (threads.cpp:
   // 479-545): if record 0 has a return number, records with none are
   // dropped; otherwise they are read as first returns
-  std::vector<uint32_t *> pos(nfiles, nullptr);
+  std::vector<uint32_t *> &pos = hold.pos;
   uint64_t total = 0;
   std::vector<int> formats;
-  hipEvent_t ev[2];
+  hipEvent_t *ev = hold.ev;
   for (int k = 0; k < 2; k++) WB_CHECK(hipEventCreate(&ev[k]));
   WB_CHECK(hipEventRecord(ev[0], c->stream));
   for (int f = 0; f < nfiles; f++) {
@@ -1046,13 +1082,11 @@ int wb_cloud_create(wb_ctx *c, wb_las *const *files, int nfiles, double unit, in
     uint64_t kept = L->n;
     if (drop && L->n > 0) {
       pos[f] = (uint32_t *)wb_alloc(c, 4 * L->n);
+      hold.pos_n[f] = L->n;
       hipLaunchKernelGGL(k_las_keep, dim3(wb_blocks(L->n, 256)), dim3(256), 0, c->stream, L->rec,
                          L->n, s.len, s.format, pos[f]);
       kept = wb_exclusive_scan_u32(c, pos[f], L->n);
-      if (kept == L->n) {
-        wb_free(c, pos[f], 4 * L->n);
-        pos[f] = nullptr;
-      }
+      if (kept == L->n) hold.drop_pos(f);
     }
     cl->src.push_back(s);
     total += kept;
@@ -1062,12 +1096,7 @@ int wb_cloud_create(wb_ctx *c, wb_las *const *files, int nfiles, double unit, in
     cl->corners.insert(cl->corners.end(), mn, mn + 3);
     cl->corners.insert(cl->corners.end(), mx, mx + 3);
   }
-  if (total > 0xffffffffull) {
-    for (int f = 0; f < nfiles; f++)
-      if (pos[f]) wb_free(c, pos[f], 4 * files[f]->n);
-    delete cl;
-    return fail(c, "more than 2^32-1 points", WB_ERANGE);
-  }
+  if (total > 0xffffffffull) return fail(c, "more than 2^32-1 points", WB_ERANGE);
   cl->n = total;
   cl->x = (double *)wb_alloc(c, 8 * total + 16);
   cl->y = (double *)wb_alloc(c, 8 * total + 16);
@@ -1087,9 +1116,6 @@ int wb_cloud_create(wb_ctx *c, wb_las *const *files, int nfiles, double unit, in
   float ems = 0;
   hipEventElapsedTime(&ems, ev[0], ev[1]);
   cl->ms_decode = ems;
-  for (int k = 0; k < 2; k++) hipEventDestroy(ev[k]);
-  for (int f = 0; f < nfiles; f++)
-    if (pos[f]) wb_free(c, pos[f], 4 * files[f]->n);
   // the lattice spacing of the decoded coordinates: combineScales of the
   // headers' scale and offset times the unit (LasHeader::getScale/getOffset)
   for (int a = 0; a < 3; a++) {
@@ -1102,6 +1128,7 @@ int wb_cloud_create(wb_ctx *c, wb_las *const *files, int nfiles, double unit, in
   }
   cl->join_format = join_point_format(formats);
   c->clouds.insert(cl);
+  hold.cl = nullptr;  // handed out: freed by wb_cloud_free or with its context
   *out = cl;
   return WB_OK;
   API_CATCH(c)

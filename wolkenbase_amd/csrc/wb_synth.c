@@ -4,7 +4,8 @@
  *
  * Scene: ground = sum of 6 sinusoids (amplitude 0.5-5 m, wavelength
  * 20-150 m) + N(0, 0.03 m); optional embankments (ramps of 20-45 deg) and
- * isolated low/high outliers; box buildings (10-30 m sides, 4-15 m tall,
+ * isolated outliers 1-50 m off the surface (a given share of them below it,
+ * the rest above); box buildings (10-30 m sides, 4-15 m tall,
  * points on roofs and walls); cone trees (3-20 m, 1-3 returns per pulse,
  * last return often on the ground); optional thin linear features (0.2 m
  * wide curbs and wires).  Feature counts are given per 200 m x 200 m and
@@ -27,6 +28,7 @@ typedef struct {
   double outlier_frac;
   double linear_frac;
   double origin_x, origin_y;
+  double outlier_low;     /* share of the outliers placed below the surface */
 } wb_synth_config;
 
 typedef struct {
@@ -311,7 +313,7 @@ int wb_synth_generate(const wb_synth_config *cfg, wb_synth_cloud *out)
     }
     if (cfg->outlier_frac > 0 && unif(&r) < cfg->outlier_frac) {
       double dz = unifab(&r, 1, 50);
-      push(&pb, x, y, (unif(&r) < 0.5) ? g - dz : g + dz, inten, 0x11, 7);
+      push(&pb, x, y, (unif(&r) < cfg->outlier_low) ? g - dz : g + dz, inten, 0x11, 7);
       continue;
     }
     push(&pb, x, y, g + gauss(&r) * 0.03, inten, 0x11, 2);

@@ -62,16 +62,46 @@ class Server:
         self.q = collections.deque()
         self.cv = threading.Condition()
         self.stopping = False
-        self.workers = [threading.Thread(target=self._run, daemon=True) for _ in range(slots)]
+        self.failed = None
+        # the contexts are opened here, so a device that cannot be opened
+        # fails the constructor instead of a worker thread
+        ctxs = []
+        try:
+            for _ in range(slots):
+                ctxs.append(W.Context(device))
+        except BaseException:
+            for c in ctxs:
+                c.close()
+            raise
+        self.workers = [threading.Thread(target=self._run, args=(c,), daemon=True) for c in ctxs]
         for w in self.workers:
             w.start()
 
     def submit(self, req):
         req.t_arrive = time.perf_counter()
         with self.cv:
-            self.q.append(req)
-            self.cv.notify_all()
+            if self.failed is None:
+                self.q.append(req)
+                self.cv.notify_all()
+                return req
+            err = self.failed
+        req.error = RuntimeError(f"serving loop failed: {err!r}")
+        req.done.set()
         return req
+
+    def _fail(self, err):
+        """A worker died: no request may wait for it.  Queued requests and
+        every later submit fail with the error."""
+        with self.cv:
+            if self.failed is None:
+                self.failed = err
+            self.stopping = True
+            pending = list(self.q)
+            self.q.clear()
+            self.cv.notify_all()
+        for r in pending:
+            r.error = RuntimeError(f"serving loop failed: {err!r}")
+            r.done.set()
 
     def close(self):
         with self.cv:
@@ -98,8 +128,8 @@ class Server:
                     return None
                 self.cv.wait()
 
-    def _run(self):
-        ctx = W.Context(self.device)
+    def _run(self, ctx):
+        req = None
         try:
             while True:
                 req = self._next()
@@ -115,6 +145,12 @@ class Server:
                     self.used -= req.need
                     self.cv.notify_all()
                 req.done.set()
+                req = None
+        except BaseException as e:
+            if req is not None and not req.done.is_set():
+                req.error = e
+                req.done.set()
+            self._fail(e)
         finally:
             ctx.close()
 
