@@ -682,6 +682,43 @@ __global__ void k_fill_pt_leaf(const uint32_t *leaf_start, uint32_t nl, uint32_t
 }
 
 // ---------------------------------------------------------------------------
+// The lowest point under every child of every internal node (the ground
+// test raises a child cube's bottom face to it, wb::hyp_intersect_low): a
+// wave per leaf for the leaves' lowest z, then one launch per depth, deepest
+// first, in which each internal node of that depth takes the lowest of its
+// children's.  Node ids grow with depth (a level's nodes are made while the
+// level above splits), so a node's depth follows from its side.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_leaf_zlow(const uint32_t *leaf_start, uint32_t nl,
+                                                   const double *pz, double *leaf_zlow)
+{
+  const uint32_t l = blockIdx.x * 4 + threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  if (l >= nl) return;
+  const uint32_t b = leaf_start[l], e = leaf_start[l + 1];
+  double m = __builtin_inf();
+  for (uint32_t j = b + lane; j < e; j += WAVE) m = fmin(m, pz[j]);
+  for (int d = 1; d < WAVE; d <<= 1) m = fmin(m, __shfl_xor(m, d, WAVE));
+  if (lane == 0) leaf_zlow[l] = m;
+}
+
+__global__ void k_node_zlow(const int32_t *node_child, const double *node_cube, uint32_t nn,
+                            int depth, const double *leaf_zlow, double *node_zlow,
+                            double *child_zlow)
+{
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  if (ilogb(node_cube[3]) - ilogb(node_cube[4 * (size_t)v + 3]) != depth) return;
+  double m = __builtin_inf();
+  for (int k = 0; k < 8; k++) {
+    const int32_t c = node_child[8 * (size_t)v + k];
+    const double zl = c <= -2 ? leaf_zlow[-2 - c] : c >= 0 ? node_zlow[c] : __builtin_inf();
+    child_zlow[8 * (size_t)v + k] = zl;
+    m = fmin(m, zl);
+  }
+  node_zlow[v] = m;
+}
+
+// ---------------------------------------------------------------------------
 // K4: SoA permute (decode in sorted order) and 32-slot bounding boxes
 // ---------------------------------------------------------------------------
 template <bool F64>
@@ -1138,6 +1175,19 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
                          nl, t->sidx, t->canon, t->crank);
       hipLaunchKernelGGL(k_leaf_canon_reg<16>, dim3(nb), dim3(256), 0, c->stream, t->leaf_start,
                          nl, t->sidx, t->canon, t->crank);
+    }
+    {
+      double *leaf_zlow = (double *)wb_alloc(c, 8 * (size_t)nl);
+      double *node_zlow = (double *)wb_alloc(c, 8 * (size_t)t->nnodes);
+      t->child_zlow = wb_talloc<double>(t, 8 * (size_t)t->nnodes);
+      hipLaunchKernelGGL(k_leaf_zlow, dim3(wb_blocks(nl, 4)), dim3(256), 0, c->stream,
+                         t->leaf_start, nl, t->pz, leaf_zlow);
+      for (int d = t->maxdepth - 1; d >= 0; d--)
+        hipLaunchKernelGGL(k_node_zlow, dim3(wb_blocks(t->nnodes, 256)), dim3(256), 0, c->stream,
+                           t->node_child, t->node_cube, t->nnodes, d, leaf_zlow, node_zlow,
+                           t->child_zlow);
+      wb_free(c, leaf_zlow, 8 * (size_t)nl);
+      wb_free(c, node_zlow, 8 * (size_t)t->nnodes);
     }
     WB_CHECK(hipEventRecord(evb[3], c->stream));
     WB_CHECK(hipEventSynchronize(evb[3]));

@@ -62,6 +62,7 @@ struct TreeView {
   const double *chunk_box;
   const uint32_t *chunk_low;
   const uint2 *node_crange;
+  const double *child_zlow;
   const uint32_t *pt_leaf, *canon, *crank, *sidx;
   uint32_t nleaves, nnodes;
   uint64_t n;
@@ -74,6 +75,7 @@ static TreeView view(const wb_tree *t)
   v.leaf_start = t->leaf_start; v.leaf_cube = t->leaf_cube; v.leaf_parent = t->leaf_parent;
   v.node_cube = t->node_cube; v.node_child = t->node_child; v.node_parent = t->node_parent;
   v.chunk_box = t->chunk_box; v.chunk_low = t->chunk_low; v.node_crange = t->node_crange;
+  v.child_zlow = t->child_zlow;
   v.pt_leaf = t->pt_leaf; v.canon = t->canon; v.crank = t->crank;
   v.sidx = t->sidx; v.nleaves = t->nleaves; v.nnodes = t->nnodes; v.n = t->n;
   return v;
@@ -1153,8 +1155,10 @@ struct ClassifyArgs {
   const TileRec *rec;
   double thickness, max_slope;
   // 1: every point lies in its leaf's cube, so the care wedges may skip a
-  // node or leaf by its cube; 0 (a point outside the root cube): only the
-  // 32-point groups' boxes, which bound the points themselves, are tested
+  // node or leaf by its cube and a child cube's bottom face may be raised to
+  // the lowest point stored under it; 0 (a point outside the root cube):
+  // only the 32-point groups' boxes, which bound the points themselves, are
+  // tested
   int cube_wedges;
   const double *tab;     // tan[511], cos[512], sin[512] packed
   const float *edges;    // bin-edge directions (wb::fill_edge_table)
@@ -1288,12 +1292,13 @@ __device__ inline int child_rank(int k, int o, int zf) { return ((k ^ o) & 3) | 
 // hyperboloid of a lane in `m` (this lane takes part iff `on`), in reverse
 // visiting order.  Returns the new stack depth.
 // Each child's test is wb::hyp_intersect on the child cube of Octree::cube
-// and the lane's care wedges on the child's xy square, with the shared parts
-// computed once: the clamped coordinate of the closest point takes one of
-// two values per axis (x from bit 0 of the child index, y from bit 1, z from
-// bit 2) and the two children of one xy square (bit 2 differs) share the
-// wedge test.  Every value is the same double operation on the same operands
-// as in the per-child form.
+// (wb::hyp_intersect_low with the lowest point stored under the child when
+// every point lies in its leaf's cube) and the lane's care wedges on the
+// child's xy square, with the shared parts computed once: the clamped
+// coordinate of the closest point takes one of two values per axis (x from
+// bit 0 of the child index, y from bit 1, z from bit 2) and the two children
+// of one xy square (bit 2 differs) share the wedge test.  Every value is the
+// same double operation on the same operands as in the per-child form.
 __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int code, bool on,
                                   const wb::Shape &H, double rx, double ry, int zf,
                                   const LaneWedges &LW, bool cube_wedges)
@@ -1310,6 +1315,9 @@ __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int c
   // lane k < 8 holds child k's slot range (leaf children), pushed with it
   const uint2 cr = __lane_id() < 8 ? T.node_crange[8 * (size_t)code + __lane_id()]
                                    : make_uint2(0u, 0u);
+  // lane k < 8 holds the lowest z under child k
+  const double zl = cube_wedges && __lane_id() < 8 ? T.child_zlow[8 * (size_t)code + __lane_id()]
+                                                   : -__builtin_inf();
   // Octree::cube (octree.cpp:349-358) per axis bit, and the closest point of
   // wb::hyp_intersect: clamp_axis on x and y, the bottom (top for a negative
   // slope) face on z
@@ -1346,7 +1354,9 @@ __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int c
   for (int k = 0; k < 8; k++) {
     mk[k] = 0;
     if (ck[k] != -1) {
-      const bool h = on & wq[k & 3] & wb::hyp_in(H, qx[k & 1], qy[(k >> 1) & 1], qz[k >> 2]);
+      const double zlk = rdl(zl, k);
+      const double qzk = H.slope > 0 && zlk > qz[k >> 2] ? zlk : qz[k >> 2];
+      const bool h = on & wq[k & 3] & wb::hyp_in(H, qx[k & 1], qy[(k >> 1) & 1], qzk);
       mk[k] = __ballot(h);
       if (mk[k]) pushed |= 1u << child_rank(k, o, zf);
     }
@@ -1752,7 +1762,8 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
             const double ncx = cx + (double)(2 * (k & 1) - 1) * side / 4;
             const double ncy = cy + (double)(2 * ((k & 2) / 2) - 1) * side / 4;
             const double ncz = cz + (double)(2 * ((k & 4) / 4) - 1) * side / 4;
-            h = wb::hyp_intersect(H, ncx, ncy, ncz, side / 2) &&
+            const double zl = A.cube_wedges ? T.child_zlow[8 * (size_t)code + k] : -__builtin_inf();
+            h = wb::hyp_intersect_low(H, ncx, ncy, ncz, side / 2, zl) &&
                 (!A.cube_wedges || wb::wedge_hit(wg, x, y, ncx - side / 4, ncx + side / 4,
                                                  ncy - side / 4, ncy + side / 4));
           }

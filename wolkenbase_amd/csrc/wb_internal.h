@@ -82,6 +82,9 @@ struct wb_tree {
   double *node_cube;
   int32_t *node_child, *node_parent;
   uint2 *node_crange;
+  // per child slot of every internal node (8 per node): the lowest z of the
+  // points stored under the child (+inf for an empty child)
+  double *child_zlow;
   // with duplicates merged: (merged input index, input index of the first
   // point at its location) pairs, and per sorted slot the input index of
   // the last point at the slot's location (whose record a writer keeps)

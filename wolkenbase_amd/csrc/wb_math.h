@@ -683,6 +683,28 @@ WB_HD bool hyp_intersect(const Shape &s, double cx, double cy, double cz, double
   return hyp_in(s, qx, qy, qz);
 }
 
+// hyp_intersect for a positive slope with the cube's bottom face raised to
+// zlow, the lowest point stored under the cube (-inf: no such bound).  When
+// every stored point lies in its leaf's cube, no point under the cube is
+// lower than zlow, and (clamped x, clamped y, max(bottom, zlow)) has per-axis
+// differences no larger than any of them, so a false here means that no
+// point under the cube lies in the hyperboloid.  Nor can the cube hold a leaf
+// whose eight corners are all in (Shape::in(Cube&), which takes a leaf's
+// points without testing them): such a leaf's top corners would be in, and
+// the clamped point with zlow (at most the top, a stored point being below
+// it) differs from a top corner by no more on any axis.
+WB_HD bool hyp_intersect_low(const Shape &s, double cx, double cy, double cz, double side,
+                             double zlow)
+{
+  double qx = clamp_axis(s.cx, cx, side), qy = clamp_axis(s.cy, cy, side), qz = cz;
+  if (s.slope > 0) {
+    qz -= side / 2;
+    if (zlow > qz) qz = zlow;
+  }
+  if (s.slope < 0) qz += side / 2;
+  return hyp_in(s, qx, qy, qz);
+}
+
 WB_HD bool hyp_may_hit_box(const Shape &s, double x0, double x1, double y0, double y1,
                            double z0, double z1)
 {
