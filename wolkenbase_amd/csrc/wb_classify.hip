@@ -1201,20 +1201,16 @@ __device__ inline void lane_wedges_update(uint64_t occ, const float *edges, Lane
   }
 }
 
-// wb::wedge_hit against thread t's wedges
+// wb::wedge_hit against thread t's wedges (at most two)
 __device__ inline bool lane_hit(const LaneWedges &W, int t, double px, double py, double x0,
                                 double x1, double y0, double y1)
 {
   const int n = W.n[t];
   if (n < 0) return true;
-  wb::Wedges w;
-  w.live = (1 << n) - 1;
-#pragma unroll
-  for (int k = 0; k < 2; k++) {
-    w.ax[k] = W.ax[k][t]; w.ay[k] = W.ay[k][t];
-    w.bx[k] = W.bx[k][t]; w.by[k] = W.by[k][t];
-  }
-  return wb::wedge_hit(w, px, py, x0, x1, y0, y1);
+  const float ax[2] = {W.ax[0][t], W.ax[1][t]}, ay[2] = {W.ay[0][t], W.ay[1][t]};
+  const float bx[2] = {W.bx[0][t], W.bx[1][t]}, by[2] = {W.by[0][t], W.by[1][t]};
+  return wb::wedge_hit2(n, ax, ay, bx, by, (float)(x0 - px), (float)(x1 - px), (float)(y0 - py),
+                        (float)(y1 - py));
 }
 
 // classification of the empty-bin mask: 1 pass, 2 fail, 0 undecided
@@ -1256,6 +1252,22 @@ __device__ inline uint32_t wave_or32(uint32_t v)
 __device__ inline uint64_t wave_or(uint64_t v)
 {
   return ((uint64_t)wave_or32((uint32_t)(v >> 32)) << 32) | wave_or32((uint32_t)v);
+}
+// OR over lanes 0-31 (h0) and over lanes 32-63 (h1): the row steps of
+// wave_or32, then row 0 folded into row 1 and row 2 into row 3, so lanes 31
+// and 63 hold the two halves' ORs.  Every lane active.
+__device__ inline void half_or(uint64_t v, uint64_t &h0, uint64_t &h1)
+{
+  uint32_t a = (uint32_t)v, b = (uint32_t)(v >> 32);
+  a = dpp_or<0xb1, 0xf>(a);   b = dpp_or<0xb1, 0xf>(b);   // quad_perm 1,0,3,2
+  a = dpp_or<0x4e, 0xf>(a);   b = dpp_or<0x4e, 0xf>(b);   // quad_perm 2,3,0,1
+  a = dpp_or<0x124, 0xf>(a);  b = dpp_or<0x124, 0xf>(b);  // row_ror 4
+  a = dpp_or<0x128, 0xf>(a);  b = dpp_or<0x128, 0xf>(b);  // row_ror 8
+  a = dpp_or<0x142, 0xa>(a);  b = dpp_or<0x142, 0xa>(b);  // row_bcast 15 into rows 1, 3
+  h0 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)b, 31) << 32) |
+       (uint32_t)__builtin_amdgcn_readlane((int)a, 31);
+  h1 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)b, 63) << 32) |
+       (uint32_t)__builtin_amdgcn_readlane((int)a, 63);
 }
 
 
@@ -1334,21 +1346,25 @@ __device__ inline int expand_node(const TreeView &T, WaveStack &S, int sp, int c
     if (H.slope > 0) qz[b] -= cs / 2;
     if (H.slope < 0) qz[b] += cs / 2;
   }
-  // the lane's wedges, read from LDS once; one test per xy square
+  // the lane's wedges (at most two), read from LDS once; one test per xy
+  // square, on the three x and three y edges of the four squares as offsets
+  // from the point (the middle edge ncx[0] + side/4 = ncx[1] - side/4: cube
+  // centres and sides are exact dyadics)
   bool wq[4] = {true, true, true, true};
-  if (cube_wedges && LW.n[threadIdx.x] >= 0) {
-    wb::Wedges w;
-    w.live = (1 << LW.n[threadIdx.x]) - 1;
-#pragma unroll
-    for (int r = 0; r < 2; r++) {
-      w.ax[r] = LW.ax[r][threadIdx.x]; w.ay[r] = LW.ay[r][threadIdx.x];
-      w.bx[r] = LW.bx[r][threadIdx.x]; w.by[r] = LW.by[r][threadIdx.x];
-    }
+  const int nw = cube_wedges ? LW.n[threadIdx.x] : -1;
+  if (nw >= 0) {
+    const int t = threadIdx.x;
+    const float ax[2] = {LW.ax[0][t], LW.ax[1][t]}, ay[2] = {LW.ay[0][t], LW.ay[1][t]};
+    const float bx[2] = {LW.bx[0][t], LW.bx[1][t]}, by[2] = {LW.by[0][t], LW.by[1][t]};
+    const float ux[3] = {(float)((ncx[0] - side / 4) - H.cx), (float)((ncx[0] + side / 4) - H.cx),
+                         (float)((ncx[1] + side / 4) - H.cx)};
+    const float vy[3] = {(float)((ncy[0] - side / 4) - H.cy), (float)((ncy[0] + side / 4) - H.cy),
+                         (float)((ncy[1] + side / 4) - H.cy)};
 #pragma unroll
     for (int q = 0; q < 4; q++)
       if ((ck[q] != -1) | (ck[q + 4] != -1))
-        wq[q] = wb::wedge_hit(w, H.cx, H.cy, ncx[q & 1] - side / 4, ncx[q & 1] + side / 4,
-                              ncy[q >> 1] - side / 4, ncy[q >> 1] + side / 4);
+        wq[q] = wb::wedge_hit2(nw, ax, ay, bx, by, ux[q & 1], ux[(q & 1) + 1], vy[q >> 1],
+                               vy[(q >> 1) + 1]);
   }
 #pragma unroll
   for (int k = 0; k < 8; k++) {
@@ -1557,13 +1573,15 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
           if (onl) cnt[CT_LEAF_B]++;
         }
         const bool all = onl && wb::hyp_in_cube(H, lc[0], lc[1], lc[2], lc[3]);
+        const uint64_t allm = __ballot(all);
+        const uint32_t js = lane & (WB_CHUNK - 1);
         for (uint32_t c0 = b; c0 < e;) {
           const uint32_t chk = c0 / WB_CHUNK;
           const uint32_t ce = min(e, (chk + 1) * WB_CHUNK);
-          // lane k fetches slot c0 + k while the group's box is tested
+          // lanes j and j + 32 fetch slot c0 + j while the group's box is tested
           double sx = 0, sy = 0, sz = 0;
-          if (lane < ce - c0) {
-            sx = T.px[c0 + lane]; sy = T.py[c0 + lane]; sz = T.pz[c0 + lane];
+          if (js < ce - c0) {
+            sx = T.px[c0 + js]; sy = T.py[c0 + js]; sz = T.pz[c0 + js];
           }
           bool chit = onl;
           if (chit) {
@@ -1571,16 +1589,39 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
             if (!all) chit = wb::hyp_may_hit_box(H, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
             if (chit) chit = lane_hit(LW, threadIdx.x, x, y, bx[0], bx[1], bx[2], bx[3]);
           }
-          if (__ballot(chit)) {
-            if (COUNT) iter_b += ce - c0;
-            if (COUNT && chit) cnt[CT_POINT_B] += ce - c0;
-            for (uint32_t s = 0; s < ce - c0; s++) {
-              const double qx = rdl(sx, s), qy = rdl(sy, s), qz = rdl(sz, s);
-              if (chit & (all | wb::hyp_in(H, qx, qy, qz)) & ((qx != x) | (qy != y))) {
-                if (COUNT) cnt[CT_DIR_B]++;
-                occ |= 1ull << wb::dir_bin(qy - y, qx - x, ttan, tcos, tsin, A.K, &nexact);
-              }
+          // The lanes that need the group ("owners"), two per step: lanes
+          // 0-31 test its slots against the first owner's hyperboloid, lanes
+          // 32-63 against the second's, and each half's direction bits are
+          // ORed into its owner's occupancy.
+          uint64_t own = __ballot(chit);
+          if (COUNT && chit) cnt[CT_POINT_B] += ce - c0;
+          while (own) {
+            const int o1 = __ffsll((long long)own) - 1;
+            own &= own - 1;
+            const int o2 = own ? __ffsll((long long)own) - 1 : o1;
+            const bool two = own != 0;
+            own &= own - 1;
+            if (COUNT) iter_b++;
+            const bool hi_half = lane >= WAVE / 2;
+            const int o = hi_half ? o2 : o1;
+            // the owner's hyperboloid, fetched across lanes by the LDS unit
+            wb::Shape Ho = H;
+            Ho.cx = __shfl(x, o, WAVE);
+            Ho.cy = __shfl(y, o, WAVE);
+            Ho.cz = __shfl(H.cz, o, WAVE);
+            Ho.por2 = __shfl(H.por2, o, WAVE);
+            const bool valid = js < ce - c0 && (!hi_half || two);
+            const bool in = valid & ((((allm >> o) & 1) != 0) | wb::hyp_in(Ho, sx, sy, sz)) &
+                            ((sx != Ho.cx) | (sy != Ho.cy));
+            uint64_t bit = 0;
+            if (in) {
+              if (COUNT) cnt[CT_DIR_B]++;
+              bit = 1ull << wb::dir_bin(sy - Ho.cy, sx - Ho.cx, ttan, tcos, tsin, A.K, &nexact);
             }
+            uint64_t r1, r2;
+            half_or(bit, r1, r2);
+            if ((int)lane == o1) occ |= r1;
+            if ((int)lane == o2 && two) occ |= r2;
           }
           c0 = ce;
         }

@@ -863,5 +863,23 @@ WB_HD bool wedge_hit(const Wedges &w, double px, double py, double x0, double x1
   return hit;
 }
 
+// wedge_hit for at most two wedges (n of them, n <= 2; n < 0 is no pruning)
+// on a box given by its corners' offsets from the point, u0 <= u1 in x and
+// v0 <= v1 in y, already rounded to float: the same test per wedge.
+WB_HD bool wedge_hit2(int n, const float ax[2], const float ay[2], const float bx[2],
+                      const float by[2], float u0, float u1, float v0, float v1)
+{
+  if (n < 0) return true;
+  bool hit = false;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const float ra = ax[k] * (ax[k] > 0 ? v1 : v0) - ay[k] * (ay[k] > 0 ? u0 : u1);
+    const float lb = (by[k] > 0 ? u1 : u0) * by[k] - (bx[k] > 0 ? v0 : v1) * bx[k];
+    const bool in_k = !(ra < 0) & !(lb < 0);
+    hit |= k < n ? in_k : false;
+  }
+  return hit;
+}
+
 
 }  // namespace wb
