@@ -181,6 +181,20 @@ int wb_count_points_in(wb_tree *tree, int kind, const double *params, uint64_t n
 int wb_points_in(wb_tree *tree, int kind, const double *params, uint64_t nshapes,
                  uint64_t *offsets, uint32_t *idx, uint64_t cap);
 
+/* --- wb_count_points_in with no device allocation.  No counterpart in the
+ *     reference: for callers that cannot allocate device memory (code under
+ *     HIP graph capture, tight serving loops, harnesses that forbid
+ *     hipMalloc).  The shape parameters and counts are staged in a fixed
+ *     buffer the library holds on each GPU from the first wb_ctx_create on
+ *     that GPU until the process ends, sized for WB_NOALLOC_MAX_SHAPES shapes
+ *     (3 MiB); more shapes are refused with WB_ERANGE.  Every call on a GPU
+ *     uses that GPU's one buffer: calls that overlap on one GPU (contexts of
+ *     several host threads) race, and the caller must serialise them.
+ *     Counts equal wb_count_points_in's. */
+#define WB_NOALLOC_MAX_SHAPES 65536
+int wb_count_points_in_noalloc(wb_tree *tree, int kind, const double *params, uint64_t nshapes,
+                               uint64_t *counts);
+
 /* --- the classification pass: SCAN (scanCylinder, scan.cpp:31-140),
  *     POSTSCAN (postscanCylinder, scan.cpp:142-179) and CLASSIFY
  *     (classifyCylinder, classify.cpp:96-173) over every flowsnake tile, with

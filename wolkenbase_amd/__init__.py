@@ -20,6 +20,7 @@ GPU_SO = os.path.join(HERE, "libwb_gpu.so")
 
 WB_SHAPE_CYLINDER, WB_SHAPE_SPHERE, WB_SHAPE_HYPERBOLOID = 0, 1, 2
 WB_SHAPE_NPARAMS = 5
+WB_NOALLOC_MAX_SHAPES = 65536
 WB_DEVICE_INPUT = 1
 WB_BUILD_DEDUPE = 2
 WB_WRITE_DRY = 4
@@ -142,6 +143,7 @@ SIGNATURES = {
     "wb_tree_leaves": (C.c_int, [_vp, C.POINTER(Leaf), C.c_uint64, _u64p]),
     "wb_tree_order": (C.c_int, [_vp, _u32p]),
     "wb_count_points_in": (C.c_int, [_vp, C.c_int, _dp, C.c_uint64, _u64p]),
+    "wb_count_points_in_noalloc": (C.c_int, [_vp, C.c_int, _dp, C.c_uint64, _u64p]),
     "wb_points_in": (C.c_int, [_vp, C.c_int, _dp, C.c_uint64, _u64p, _u32p, C.c_uint64]),
     "wb_classify": (C.c_int, [_vp, _dp, _dp, C.POINTER(ClassifyParams), _vp, C.c_int,
                               C.POINTER(TileRecord), C.c_uint64, C.POINTER(Stats)]),
@@ -486,6 +488,17 @@ class Octree:
         out = np.zeros(len(p), dtype=np.uint64)
         self.ctx.check(lib().wb_count_points_in(self.h, kind, _dptr(p), len(p),
                                                 out.ctypes.data_as(_u64p)), "wb_count_points_in")
+        return out
+
+    def count_points_in_noalloc(self, shapes):
+        """count_points_in without device allocation: the shapes and counts
+        are staged in the library's fixed per-GPU buffer (at most
+        WB_NOALLOC_MAX_SHAPES shapes; calls on one GPU must not overlap)."""
+        kind, p = _shape_batch(shapes)
+        out = np.zeros(len(p), dtype=np.uint64)
+        self.ctx.check(lib().wb_count_points_in_noalloc(self.h, kind, _dptr(p), len(p),
+                                                        out.ctypes.data_as(_u64p)),
+                       "wb_count_points_in_noalloc")
         return out
 
     def points_in(self, shapes):

@@ -7,6 +7,41 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
+
+// ---------------------------------------------------------------------------
+// Static scratch of wb_count_points_in_noalloc: the staged shape parameters
+// (WB_NOALLOC_MAX_SHAPES x WB_SHAPE_NPARAMS doubles) and counts (as many
+// u64) of one call.  Each GPU holds its own copy for the life of the process.
+// In host code `scratch` names a host-side stand-in, not device memory: its
+// device address on each GPU comes from hipGetSymbolAddress, taken once per
+// device with that device current (wb_ctx_create), and only that address is
+// passed to copies and kernels.
+// ---------------------------------------------------------------------------
+#define WB_NOALLOC_PARAM_BYTES ((size_t)WB_NOALLOC_MAX_SHAPES * WB_SHAPE_NPARAMS * sizeof(double))
+#define WB_NOALLOC_BYTES (WB_NOALLOC_PARAM_BYTES + (size_t)WB_NOALLOC_MAX_SHAPES * sizeof(uint64_t))
+static __device__ __align__(256) unsigned char scratch[WB_NOALLOC_BYTES];
+
+#define WB_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[WB_MAX_DEVICES];
+
+// the device address of `scratch` on `device` (current); nullptr if the
+// runtime refuses it
+static void *scratch_address(int device)
+{
+  if (device < 0 || device >= WB_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
 
 
 // ---------------------------------------------------------------------------
@@ -90,6 +125,7 @@ int wb_ctx_create(int device, wb_ctx **out)
     WB_CHECK(hipMalloc(&c->d_edges, sizeof edges));
     WB_CHECK(hipMemcpy(c->d_edges, edges, sizeof edges, hipMemcpyHostToDevice));
     c->kconst = wb::atan_const();
+    c->scratch = scratch_address(device);
   } catch (const std::string &m) {
     delete c;
     return WB_EDEVICE;
@@ -357,6 +393,34 @@ int wb_count_points_in(wb_tree *t, int kind, const double *params, uint64_t ns, 
   WB_CHECK(hipStreamSynchronize(c->stream));
   wb_free(c, dp, pb);
   wb_free(c, dc, 8 * ns);
+  return WB_OK;
+  API_CATCH(c)
+}
+
+int wb_count_points_in_noalloc(wb_tree *t, int kind, const double *params, uint64_t ns,
+                               uint64_t *counts)
+{
+  if (!t || (!params && ns) || (!counts && ns)) return WB_EINVAL;
+  wb_ctx *c = t->ctx;
+  if (!check_kind(kind)) return fail(c, "unknown shape kind", WB_EINVAL);
+  if (ns > WB_NOALLOC_MAX_SHAPES)
+    return fail(c, "wb_count_points_in_noalloc takes at most WB_NOALLOC_MAX_SHAPES (" +
+                       std::to_string(WB_NOALLOC_MAX_SHAPES) + ") shapes, got " +
+                       std::to_string(ns),
+                WB_ERANGE);
+  if (ns == 0) return WB_OK;
+  if (!c->scratch) return fail(c, "the static scratch has no device address on this GPU", WB_EDEVICE);
+  API_TRY(c)
+  WB_CHECK(hipSetDevice(c->device));
+  // every count is written by the kernel (one per shape): nothing in the
+  // shared buffer has to start at zero
+  double *dp = (double *)c->scratch;
+  uint64_t *dc = (uint64_t *)((unsigned char *)c->scratch + WB_NOALLOC_PARAM_BYTES);
+  WB_CHECK(hipMemcpyAsync(dp, params, sizeof(double) * WB_SHAPE_NPARAMS * ns,
+                          hipMemcpyHostToDevice, c->stream));
+  wb_run_count(t, kind, dp, ns, dc);
+  WB_CHECK(hipMemcpyAsync(counts, dc, 8 * ns, hipMemcpyDeviceToHost, c->stream));
+  WB_CHECK(hipStreamSynchronize(c->stream));
   return WB_OK;
   API_CATCH(c)
 }

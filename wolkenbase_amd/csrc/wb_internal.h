@@ -41,6 +41,9 @@ struct wb_ctx {
   // atan2i tables and constants (device)
   double *d_tan, *d_cos, *d_sin;
   float *d_edges;  // bin-edge directions of the ground test (wb::fill_edge_table)
+  // device address of this GPU's copy of the library's static scratch
+  // (wb_count_points_in_noalloc)
+  void *scratch;
   wb::AtanConst kconst;
   // size-keyed cache of device buffers
   std::multimap<size_t, void *> cache;
