@@ -128,6 +128,12 @@ int wbo_atan2i(double y, double x)
   return a;
 }
 
+/* wbo_atan2i over arrays (test helper for brute-force checks) */
+void wbo_atan2i_batch(const double *y, const double *x, long n, int *out)
+{
+  for (long i = 0; i < n; i++) out[i] = wbo_atan2i(y[i], x[i]);
+}
+
 /* ------------------------------------------------------------------ */
 /* pairwisesum: manysum.cpp:120-154                                    */
 /* The reference sums aligned power-of-two blocks, one block per set   */

@@ -113,6 +113,7 @@ def lib():
         L.wbo_i_to_flowsnake.restype = C.c_int
         L.wbo_i_to_flowsnake.argtypes = [C.c_int]
         L.wbo_set_exhaustive.argtypes = [C.c_int]
+        L.wbo_atan2i_batch.argtypes = [_dp, _dp, C.c_long, _ip]
         L.wbo_fill_tables()
         _lib = L
     return _lib
@@ -234,6 +235,22 @@ def snake_setup(bbmin, bbmax, tile_size=1.0):
     sn = Snake()
     lib().wbo_snake_setup(_ptr(a, _dp), _ptr(b, _dp), float(tile_size), C.byref(sn))
     return sn
+
+
+def atan2i_batch(y, x):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(len(y), dtype=np.int32)
+    lib().wbo_atan2i_batch(_ptr(y, _dp), _ptr(x, _dp), len(y), out.ctypes.data_as(_ip))
+    return out
+
+
+def snake_cyl(sn, ex, ey):
+    """Flowsnake::cyl (flowsnake.cpp:263-271): centre x, y and radius of the
+    tile at Eisenstein address (ex, ey)."""
+    cx, cy, r = C.c_double(), C.c_double(), C.c_double()
+    lib().wbo_snake_cyl(C.byref(sn), int(ex), int(ey), C.byref(cx), C.byref(cy), C.byref(r))
+    return cx.value, cy.value, r.value
 
 
 def surround(dirs):

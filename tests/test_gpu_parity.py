@@ -146,7 +146,7 @@ def test_classify_matches_oracle(ctx, cfg):
     cls, tiles, st = W.classify(tree, cloud["bbmin"], cloud["bbmax"])
     rcls, rtiles, rst = O.run(cloud)
     print("stats", st, "oracle", rst.n_class1, rst.n_class2, rst.n_unclassified)
-    for f in ("npoints", "tree_flags"):
+    for f in ("npoints", "nground", "tree_flags"):
         assert np.array_equal(tiles[f], rtiles[f]), f
     for f in ("density", "hs_scan", "hs", "height", "tilt_x", "tilt_y"):
         assert np.array_equal(tiles[f], rtiles[f]), f
@@ -155,6 +155,35 @@ def test_classify_matches_oracle(ctx, cfg):
     if cfg["seed"] == 26:
         m = tiles["npoints"]
         assert ((m > 64) & (m <= 128)).any() and ((m > 128) & (m <= 256)).any() and (m > 256).any()
+
+
+# classification parameters other than the defaults (the hyperboloid's
+# vertex drop and slope, the tile spacing, the floor of the hyperboloid size)
+PARAMS = [dict(thickness=0.05), dict(max_slope=0.7), dict(max_slope=1.5), dict(tile_size=2.0),
+          dict(min_hs=0.3),
+          dict(tile_size=0.7, max_slope=1.2, thickness=0.02, min_hs=0.2)]
+
+
+@pytest.mark.parametrize("prm", PARAMS, ids=lambda p: ",".join(f"{k}={v}" for k, v in p.items()))
+def test_classify_params_match_oracle(ctx, prm):
+    cloud = synth.generate(seed=23, extent=30.0, density=20.0, embankments=2, outliers=0.02,
+                           outlier_low=0.2)
+    tree, _, _ = build_both(ctx, cloud)
+    gp = W.default_params(**prm)
+    op = O.default_params()
+    for k, v in prm.items():
+        setattr(op, k, v)
+    cls, tiles, st = W.classify(tree, cloud["bbmin"], cloud["bbmax"], gp)
+    rcls, rtiles, rst = O.run(cloud, op)
+    print(prm, "stats", st["n_class1"], st["n_class2"], "oracle", rst.n_class1, rst.n_class2)
+    assert len(tiles) == len(rtiles)
+    for f in ("npoints", "tree_flags"):
+        assert np.array_equal(tiles[f], rtiles[f]), f
+    for f in ("density", "hs_scan", "hs", "height", "tilt_x", "tilt_y"):
+        assert np.array_equal(tiles[f], rtiles[f]), f
+    assert int((cls != rcls).sum()) == 0
+    assert 0 < st["n_class2"] < len(cls)
+    assert np.array_equal(tiles["nground"], rtiles["nground"])
 
 
 def header_box(cloud, lo_frac, hi_frac):

@@ -1170,7 +1170,42 @@ struct ClassifyArgs {
   uint32_t *nheavy;
   unsigned long long *atan_exact;
   unsigned long long *counters;  // 16 work counters (only with the counting kernel)
+  // work items in place of the points (nullptr: work index k is sorted slot
+  // k with the hyperboloid size of its deciding tile): item k is (slot,
+  // tile record index), and its verdict goes to item_cls[k] only
+  const uint2 *items;
+  uint64_t n_items;
+  uint8_t *item_cls;
 };
+
+// the slot and hyperboloid size of work index k; false for a point in no tile
+__device__ inline bool work_point(const ClassifyArgs &A, uint64_t k, uint32_t &slot, double &hs)
+{
+  if (A.items) {
+    const uint2 it = A.items[k];
+    slot = it.x;
+    hs = A.rec[it.y].hs;
+    return true;
+  }
+  slot = (uint32_t)k;
+  if (A.nmemb[k] == 0) return false;
+  int w = INT32_MIN;
+  for (uint32_t j = 0; j < A.nmemb[k]; j++) w = max(w, A.memb[MAX_MEMB * k + j]);
+  hs = A.rec[w - A.g.lo].hs;
+  return true;
+}
+
+// the verdict of work index k: the class byte of the point (per sorted slot
+// and at its input index), or of the work item
+__device__ inline void work_verdict(const ClassifyArgs &A, uint64_t k, uint32_t slot, uint8_t v)
+{
+  if (A.items) {
+    A.item_cls[k] = v;
+    return;
+  }
+  A.cls[k] = v;
+  A.cls_in[A.T.sidx[slot]] = v;
+}
 
 __device__ inline uint64_t rotr64(uint64_t x, int a) { return (x >> a) | (x << (64 - a)); }
 
@@ -1440,8 +1475,9 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
   __syncthreads();
   const TreeView &T = A.T;
   const uint32_t lane = __lane_id();
+  // work index: a sorted slot, or a work item
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool inrange = i < T.n;
+  const bool inrange = i < (A.items ? A.n_items : T.n);
   const double *ttan = A.tab, *tcos = A.tab + 511, *tsin = A.tab + 1023;
   const uint64_t t_start = COUNT ? wall_clock64() : 0;
   uint32_t cnt[CT_WAVES] = {};
@@ -1450,11 +1486,9 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
   // k_ground_heavy: 5 walk A cut short, 6 after walk A, 7 after walk B
   int state = 3;
   double x = 0, y = 0, z = 0, hs = 0;
-  if (inrange && A.nmemb[i] > 0) {
-    int w = INT32_MIN;
-    for (uint32_t k = 0; k < A.nmemb[i]; k++) w = max(w, A.memb[MAX_MEMB * i + k]);
-    x = T.px[i]; y = T.py[i]; z = T.pz[i];
-    hs = A.rec[w - A.g.lo].hs;
+  uint32_t slot = 0;
+  if (inrange && work_point(A, i, slot, hs)) {
+    x = T.px[slot]; y = T.py[slot]; z = T.pz[slot];
     state = 0;
   }
   wb::Shape H;
@@ -1659,11 +1693,7 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
     }
   }
   if (nexact) atomicAdd(A.atan_exact, (unsigned long long)nexact);
-  if (inrange && state < 5) {
-    const uint8_t v = state == 1 ? 1 : (state == 2 ? 2 : 0);
-    A.cls[i] = v;
-    A.cls_in[T.sidx[i]] = v;
-  }
+  if (inrange && state < 5) work_verdict(A, i, slot, state == 1 ? 1 : (state == 2 ? 2 : 0));
   if (COUNT) {
     if (state == 7) cnt[CT_LANES_C]++;
     if (state >= 5) cnt[CT_HEAVY]++;
@@ -1730,14 +1760,14 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
   uint64_t c_leaf = 0, c_pts = 0, c_exp = 0, c_test = 0, c_in = 0;
   const TreeView &T = A.T;
   const HeavyRec hr = heavy[blockIdx.x];
-  const uint32_t i = hr.i;
+  const uint32_t i = hr.i;  // work index (k_ground hands over only lanes with a tile)
   const uint32_t lane = threadIdx.x;
   const double *ttan = A.tab, *tcos = A.tab + 511, *tsin = A.tab + 1023;
-  int w = INT32_MIN;
-  for (uint32_t k = 0; k < A.nmemb[i]; k++) w = max(w, A.memb[MAX_MEMB * i + k]);
-  const double x = T.px[i], y = T.py[i], z = T.pz[i];
-  const wb::Shape H = wb::make_hyperboloid(x - 0, y - 0, z - A.thickness, A.rec[w - A.g.lo].hs,
-                                           A.max_slope);
+  uint32_t slot = 0;
+  double hs = 0;
+  work_point(A, i, slot, hs);
+  const double x = T.px[slot], y = T.py[slot], z = T.pz[slot];
+  const wb::Shape H = wb::make_hyperboloid(x - 0, y - 0, z - A.thickness, hs, A.max_slope);
   const int zf = A.max_slope > 0 ? 0 : 4;
   int nexact = 0;
   // the directions k_ground found are a subset of D; the walks it completed
@@ -1978,8 +2008,7 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
       c_in += __shfl_xor(c_in, s, 64);
     }
   if (lane == 0) {
-    A.cls[i] = (uint8_t)verdict;
-    A.cls_in[T.sidx[i]] = (uint8_t)verdict;
+    work_verdict(A, i, slot, (uint8_t)verdict);
     if (nexact) atomicAdd(A.atan_exact, (unsigned long long)nexact);
     if (COUNT) {
       // per start walk: points, passes, leaves, slots, expansions, ticks,
@@ -2009,9 +2038,17 @@ __global__ void k_dup_class(uint64_t ndup, const uint32_t *pairs, uint8_t *out)
 // class histogram (per-block partials, summed by k_sum_hist) and, when tile
 // records are wanted, the number of ground points each deciding tile
 // classified
+// Class totals, and Tile::nGround (classify.cpp:151-155): CLASSIFY tests
+// every point of a tile with that tile's hyperboloid size and counts the
+// ground verdicts, so a point in several tiles counts once in each tile
+// that finds it ground.  The point's final class is its deciding tile's
+// verdict, which every tile of the same size shares; the other tiles'
+// verdicts come from k_ground over work items (k_overlap_items,
+// k_item_ground).
 __global__ void __launch_bounds__(256) k_count_ground(uint64_t n, const uint8_t *cls,
                                                        const int32_t *memb, const uint32_t *nmemb,
-                                                       int lo, int32_t *nground, uint32_t *partial,
+                                                       const TileRec *rec, int lo,
+                                                       int32_t *nground, uint32_t *partial,
                                                        int per_tile)
 {
   __shared__ uint32_t h[3];
@@ -2026,10 +2063,44 @@ __global__ void __launch_bounds__(256) k_count_ground(uint64_t n, const uint8_t 
   if (per_tile && c == 2) {
     int w = INT32_MIN;
     for (uint32_t k = 0; k < nmemb[i]; k++) w = max(w, memb[MAX_MEMB * i + k]);
-    atomicAdd(&nground[w - lo], 1);
+    const double hw = rec[w - lo].hs;
+    for (uint32_t k = 0; k < nmemb[i]; k++) {
+      const int t = memb[MAX_MEMB * i + k];
+      if (t == w || rec[t - lo].hs == hw) atomicAdd(&nground[t - lo], 1);
+    }
   }
   __syncthreads();
   if (threadIdx.x < 3) partial[3 * (uint64_t)blockIdx.x + threadIdx.x] = h[threadIdx.x];
+}
+
+// A point's tiles whose hyperboloid size differs from its deciding tile's:
+// with items == nullptr, off[i] receives their number; otherwise the items
+// (slot, tile record index) are written from off[i] on.
+__global__ void k_overlap_items(uint64_t n, const int32_t *memb, const uint32_t *nmemb,
+                                const TileRec *rec, int lo, uint32_t *off, uint2 *items)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t m = nmemb[i];
+  int w = INT32_MIN;
+  for (uint32_t k = 0; k < m; k++) w = max(w, memb[MAX_MEMB * i + k]);
+  uint32_t c = 0;
+  const uint32_t base = items ? off[i] : 0;
+  for (uint32_t k = 0; k < m; k++) {
+    const int t = memb[MAX_MEMB * i + k];
+    if (t != w && rec[t - lo].hs != rec[w - lo].hs) {
+      if (items) items[base + c] = make_uint2((uint32_t)i, (uint32_t)(t - lo));
+      c++;
+    }
+  }
+  if (!items) off[i] = c;
+}
+
+__global__ void k_item_ground(uint64_t nitems, const uint2 *items, const uint8_t *item_cls,
+                              int32_t *nground)
+{
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nitems && item_cls[k] == 2) atomicAdd(&nground[items[k].y], 1);
 }
 
 __global__ void __launch_bounds__(256) k_sum_hist(const uint32_t *partial, uint32_t nblocks,
@@ -2174,6 +2245,9 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
     A.tab = c->d_tan; A.K = c->kconst; A.cls = cls; A.cls_in = d_class_out; A.nslow = nslow;
     A.edges = c->d_edges;
     A.atan_exact = ctr + 1;
+    A.items = nullptr;
+    A.n_items = 0;
+    A.item_cls = nullptr;
     WB_CHECK(hipEventRecord(ev[4], st));
     static const bool count_work = getenv("WB_CLASSIFY_COUNTERS") != nullptr;
     if (count_work) {
@@ -2210,13 +2284,61 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
     if (n > 0) {
       const unsigned nbk = wb_blocks(n, 256);
       uint32_t *partial = (uint32_t *)wb_alloc(c, 12ull * nbk);
-      hipLaunchKernelGGL(k_count_ground, dim3(nbk), dim3(256), 0, st, n, cls, memb, nmemb, g.lo,
-                         nground, partial, want_tiles ? 1 : 0);
+      hipLaunchKernelGGL(k_count_ground, dim3(nbk), dim3(256), 0, st, n, cls, memb, nmemb, rec,
+                         g.lo, nground, partial, want_tiles ? 1 : 0);
       hipLaunchKernelGGL(k_sum_hist, dim3(1), dim3(256), 0, st, partial, nbk, ctr + 2);
       wb_free(c, partial, 12ull * nbk);
       if (t->ndup)
         hipLaunchKernelGGL(k_dup_class, dim3(wb_blocks(t->ndup, 256)), dim3(256), 0, st, t->ndup,
                            t->dup_pairs, d_class_out);
+    }
+    if (want_tiles && n > 0) {
+      // Tile::nGround of the tiles that do not decide a point they hold:
+      // the ground test again with their hyperboloid size, for the
+      // (point, tile) pairs whose size differs from the deciding tile's
+      uint32_t *off = (uint32_t *)wb_alloc(c, 4 * n + 4);
+      hipLaunchKernelGGL(k_overlap_items, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, memb,
+                         nmemb, rec, g.lo, off, nullptr);
+      const uint64_t nitems = wb_exclusive_scan_u32(c, off, n);
+      if (nitems > 0xffffffffull) {
+        wb_free(c, off, 4 * n + 4);
+        throw std::string("more than 2^32-1 (point, tile) pairs to recount");
+      }
+      if (nitems) {
+        uint2 *items = (uint2 *)wb_alloc(c, 8 * nitems);
+        uint8_t *icls = (uint8_t *)wb_alloc(c, nitems + 16);
+        HeavyRec *iheavy = (HeavyRec *)wb_alloc(c, sizeof(HeavyRec) * (nitems + 1));
+        // the pass's own counters: exact gap points, handed-over items,
+        // emulated atan2i steps (the statistics keep the main pass's)
+        unsigned long long *ict = (unsigned long long *)wb_alloc(c, 32);
+        WB_CHECK(hipMemsetAsync(ict, 0, 32, st));
+        hipLaunchKernelGGL(k_overlap_items, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, memb,
+                           nmemb, rec, g.lo, off, items);
+        ClassifyArgs B = A;
+        B.items = items;
+        B.n_items = nitems;
+        B.item_cls = icls;
+        B.heavy = iheavy;
+        B.nslow = (uint32_t *)ict;
+        B.nheavy = (uint32_t *)ict + 1;
+        B.atan_exact = ict + 1;
+        B.counters = nullptr;
+        hipLaunchKernelGGL((k_ground<false, WB_GROUND_MINW>), dim3(wb_blocks(nitems, 256)),
+                           dim3(256), 0, st, B);
+        uint32_t nh = 0;
+        WB_CHECK(hipMemcpyAsync(&nh, B.nheavy, 4, hipMemcpyDeviceToHost, st));
+        WB_CHECK(hipStreamSynchronize(st));
+        if (nh)
+          hipLaunchKernelGGL(k_ground_heavy<false>, dim3(nh), dim3(64), 0, st, B, iheavy, nh);
+        hipLaunchKernelGGL(k_item_ground, dim3(wb_blocks(nitems, 256)), dim3(256), 0, st, nitems,
+                           items, icls, nground);
+        WB_CHECK(hipStreamSynchronize(st));
+        wb_free(c, items, 8 * nitems);
+        wb_free(c, icls, nitems + 16);
+        wb_free(c, iheavy, sizeof(HeavyRec) * (nitems + 1));
+        wb_free(c, ict, 32);
+      }
+      wb_free(c, off, 4 * n + 4);
     }
   }
   WB_CHECK(hipEventRecord(ev[3], st));
