@@ -373,7 +373,7 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         if int(os.environ.get("RANK", "0")) == 0:
             print("bench.py runs on one GPU only: the classification of one tile is not "
-                  "sharded across GPUs in this build (see DESIGN.md, 'Sharding').", flush=True)
+                  "sharded across GPUs.", flush=True)
         sys.exit(2)
     if args.workload == "las20m":
         return main_las(args)
@@ -394,7 +394,7 @@ def main():
     kern = kernel_times(stats)
     dominant = max(kern, key=kern.get)
     passes = int(st["sort_passes"])
-    # algorithmic bytes per launch (SURVEY 8(d) decomposition): the ground
+    # algorithmic bytes per launch (compulsory traffic per point): the ground
     # test reads each point once (12 B as int32) and writes its class byte;
     # the sort moves (u64 key, u32 index) records, 24 B per point per pass,
     # plus the 8-B key read of each digit histogram.

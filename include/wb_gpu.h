@@ -10,8 +10,10 @@
  * decoded exactly as LasHeader::readPoint does (las.cpp:808):
  *   x = (offset_x + scale_x * x_int) * unit
  * and every geometric test runs in FP64 with the reference's operation
- * order, so results equal the reference's single-threaded CPU path bit for
- * bit (see the parity notes in DESIGN.md).
+ * order.  Results are bit-identical to the oracle, a single-threaded CPU
+ * restatement of the reference's path with its scheduling-dependent orders
+ * pinned (points of a leaf in input order, the last flowsnake tile deciding
+ * an overlap).
  */
 #ifndef WB_GPU_H
 #define WB_GPU_H
