@@ -29,6 +29,9 @@ WORKLOADS = {
     "tile20m_high": ("tile20m_high: the tile20m pulses, terrain, buildings and trees with its "
                      "2% outliers all 1-50 m above the surface instead of half of them below "
                      "(about 55% ground instead of 0.3%), int32 at 0.001 m"),
+    "tile200m": ("tile200m: 3 km x 3 km, 17 pulses/m^2 (~205M points), seed 4000, an urban "
+                 "half (60 buildings per 200 m x 200 m) beside a forested half (1200 trees per "
+                 "200 m x 200 m, 44% multi-return pulses), int32 at 0.001 m"),
     "tile1m": ("tile1m: 200 m x 200 m, 25 pulses/m^2 (~1.2M points), seed 1000, "
                "6-sinusoid ground, 40 box buildings, 300 cone trees, int32 at 0.001 m"),
     "las20m": ("las20m: the tile20m points as one LAS 1.4 point-format-1 file (28-byte "

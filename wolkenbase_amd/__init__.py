@@ -472,7 +472,7 @@ class Octree:
         got = C.c_uint64()
         self.ctx.check(lib().wb_tree_leaves(self.h, buf, nl, C.byref(got)), "wb_tree_leaves")
         a = np.ctypeslib.as_array(buf)[:nl]
-        return dict(center=np.array([list(r["center"]) for r in a]) if nl else np.zeros((0, 3)),
+        return dict(center=np.array(a["center"], dtype=np.float64).reshape(nl, 3),
                     side=a["side"].copy(), first=a["first"].astype(np.int64),
                     count=a["count"].astype(np.int64), depth=a["depth"].astype(np.int32))
 

@@ -5,7 +5,8 @@
  * Scene: ground = sum of 6 sinusoids (amplitude 0.5-5 m, wavelength
  * 20-150 m) + N(0, 0.03 m); optional embankments (ramps of 20-45 deg) and
  * isolated outliers 1-50 m off the surface (a given share of them below it,
- * the rest above); box buildings (10-30 m sides, 4-15 m tall,
+ * the rest above); optionally an urban part (buildings only, x below a
+ * split) beside a forested part (trees only); box buildings (10-30 m sides, 4-15 m tall,
  * points on roofs and walls); cone trees (3-20 m, 1-3 returns per pulse,
  * last return often on the ground); optional thin linear features (0.2 m
  * wide curbs and wires).  Feature counts are given per 200 m x 200 m and
@@ -29,6 +30,7 @@ typedef struct {
   double linear_frac;
   double origin_x, origin_y;
   double outlier_low;     /* share of the outliers placed below the surface */
+  double split_x;         /* > 0: buildings only at x < split_x, trees only at x >= split_x */
 } wb_synth_config;
 
 typedef struct {
@@ -162,7 +164,8 @@ static void build_scene(scene_t *sc, const wb_synth_config *cfg, rng_t *r)
   double *bx1 = malloc(sizeof(double) * (sc->nb + 1)), *by1 = malloc(sizeof(double) * (sc->nb + 1));
   for (int i = 0; i < sc->nb; i++) {
     double sx = unifab(r, 10, 30), sy = unifab(r, 10, 30), h = unifab(r, 4, 15);
-    double cx = unifab(r, 0.05 * W + sx / 2, 0.95 * W - sx / 2);
+    double xhi = cfg->split_x > 0 ? fmin(0.95 * W, cfg->split_x) : 0.95 * W;
+    double cx = unifab(r, 0.05 * W + sx / 2, xhi - sx / 2);
     double cy = unifab(r, 0.05 * W + sy / 2, 0.95 * W - sy / 2);
     bldg_t *b = &sc->b[i];
     b->x0 = cx - sx / 2; b->x1 = cx + sx / 2; b->y0 = cy - sy / 2; b->y1 = cy + sy / 2;
@@ -178,7 +181,7 @@ static void build_scene(scene_t *sc, const wb_synth_config *cfg, rng_t *r)
     tree_t *t = &sc->t[i];
     t->h = unifab(r, 3, 20);
     t->r = t->h * unifab(r, 0.2, 0.35);
-    t->cx = unifab(r, 0, W);
+    t->cx = cfg->split_x > 0 ? unifab(r, cfg->split_x, W) : unifab(r, 0, W);
     t->cy = unifab(r, 0, W);
     t->base = ground(sc, t->cx, t->cy);
     tx0[i] = t->cx - t->r; tx1[i] = t->cx + t->r; ty0[i] = t->cy - t->r; ty1[i] = t->cy + t->r;

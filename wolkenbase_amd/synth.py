@@ -16,7 +16,7 @@ class _Config(C.Structure):
                 ("buildings_per_ref", C.c_double), ("trees_per_ref", C.c_double),
                 ("n_embankments", C.c_int), ("outlier_frac", C.c_double),
                 ("linear_frac", C.c_double), ("origin_x", C.c_double), ("origin_y", C.c_double),
-                ("outlier_low", C.c_double)]
+                ("outlier_low", C.c_double), ("split_x", C.c_double)]
 
 
 class _Cloud(C.Structure):
@@ -63,19 +63,27 @@ PRESETS = {
                     embankments=5, outliers=0.02, linear=0.0),
     "tile20m_high": dict(seed=2000, extent=900.0, density=25.0, buildings=40, trees=300,
                          embankments=5, outliers=0.02, linear=0.0, outlier_low=0.0),
+    # 3 km x 3 km, about 200M points: an urban half (60 buildings per
+    # 200 m x 200 m of it) and a forested half (1200 trees per 200 m x 200 m,
+    # crowns over about 60% of it, so about 40% of its pulses return more
+    # than once)
+    "tile200m": dict(seed=4000, extent=3000.0, density=17.0, buildings=30, trees=600,
+                     embankments=0, outliers=0.0, linear=0.0, split_x=1500.0),
     "tile20m_linear": dict(seed=2000, extent=900.0, density=25.0, buildings=40, trees=300,
                            embankments=5, outliers=0.02, linear=0.05),
 }
 
 
 def generate(seed=1000, extent=200.0, density=25.0, buildings=40, trees=300, embankments=0,
-             outliers=0.0, linear=0.0, origin=(500000.0, 4400000.0), outlier_low=0.5):
+             outliers=0.0, linear=0.0, origin=(500000.0, 4400000.0), outlier_low=0.5,
+             split_x=0.0):
     """Returns a dict: xi, yi, zi (int32), intensity, returns, truth, scale,
     offset, bbmin, bbmax (float64[3]).  outlier_low: the share of the
-    outliers placed below the surface."""
+    outliers placed below the surface; split_x > 0: buildings only west of
+    x = split_x (m from the tile's west edge), trees only east of it."""
     L = _load()
     cfg = _Config(seed, extent, density, buildings, trees, embankments, outliers, linear,
-                  origin[0], origin[1], outlier_low)
+                  origin[0], origin[1], outlier_low, split_x)
     cl = _Cloud()
     L.wb_synth_generate(C.byref(cfg), C.byref(cl))
     n = cl.n
