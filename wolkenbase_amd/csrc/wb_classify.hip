@@ -1447,7 +1447,7 @@ enum {
 #define HEAVY_A_GROUPS 2048u
 #endif
 #ifndef HEAVY_B_GROUPS
-#define HEAVY_B_GROUPS 256u
+#define HEAVY_B_GROUPS 1024u
 #endif
 
 // The ground test of classify.cpp:128-165 for one point per lane, in two
@@ -1645,17 +1645,34 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
             Ho.cz = __shfl(H.cz, o, WAVE);
             Ho.por2 = __shfl(H.por2, o, WAVE);
             const bool valid = js < ce - c0 && (!hi_half || two);
-            const bool in = valid & ((((allm >> o) & 1) != 0) | wb::hyp_in(Ho, sx, sy, sz)) &
-                            ((sx != Ho.cx) | (sy != Ho.cy));
+            bool in = valid & ((((allm >> o) & 1) != 0) | wb::hyp_in(Ho, sx, sy, sz)) &
+                      ((sx != Ho.cx) | (sy != Ho.cy));
+            // A direction outside the owner's care wedges is not binned: its
+            // bin is occupied or in an empty run of fewer than 24 bins, runs
+            // only shrink, so it never lies in a run of 24 or more, and the
+            // runs of 24 or more (the verdict, and the occupied bins that
+            // border them) come out the same without it.
+            if (in) {
+              const int ot = (int)(threadIdx.x & ~(WAVE - 1)) + o;
+              const int nw = LW.n[ot];
+              if (nw >= 0) {
+                const float ax[2] = {LW.ax[0][ot], LW.ax[1][ot]}, ay[2] = {LW.ay[0][ot], LW.ay[1][ot]};
+                const float bx[2] = {LW.bx[0][ot], LW.bx[1][ot]}, by[2] = {LW.by[0][ot], LW.by[1][ot]};
+                const float u = (float)(sx - Ho.cx), v = (float)(sy - Ho.cy);
+                in = wb::wedge_hit2(nw, ax, ay, bx, by, u, u, v, v);
+              }
+            }
             uint64_t bit = 0;
             if (in) {
               if (COUNT) cnt[CT_DIR_B]++;
               bit = 1ull << wb::dir_bin(sy - Ho.cy, sx - Ho.cx, ttan, tcos, tsin, A.K, &nexact);
             }
-            uint64_t r1, r2;
-            half_or(bit, r1, r2);
-            if ((int)lane == o1) occ |= r1;
-            if ((int)lane == o2 && two) occ |= r2;
+            if (__ballot(bit != 0)) {
+              uint64_t r1, r2;
+              half_or(bit, r1, r2);
+              if ((int)lane == o1) occ |= r1;
+              if ((int)lane == o2 && two) occ |= r2;
+            }
           }
           c0 = ce;
         }
@@ -1734,7 +1751,9 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
 // A: the lowest point of each group) or slots (walk B, and C, the exact
 // border pass for a longest empty run of 24 or 25 bins) per step, with the
 // early exit checked after every step.
+#ifndef HB_LEAVES
 #define HB_LEAVES 256
+#endif
 // The heavy walk's stack.  Its levels never decrease from bottom to top
 // (the children of the i-th popped node sit above those of the (i+1)-th, one
 // level deeper than their parents).  So a round that pops a level-(m-1) node

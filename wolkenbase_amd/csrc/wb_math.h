@@ -667,6 +667,11 @@ WB_HD bool hyp_in_cube(const Shape &s, double cx, double cy, double cz, double s
 {
   double h = side / 2;
   double x0 = cx - h, x1 = cx + h, y0 = cy - h, y1 = cy + h, z0 = cz - h, z1 = cz + h;
+  // the corner likeliest to fail first (farthest from the axis in x and y,
+  // on the face away from the vertex side): when it is out, so is the AND of
+  // all eight, which is then taken without the other seven
+  const double fx = s.cx - cx >= 0 ? x0 : x1, fy = s.cy - cy >= 0 ? y0 : y1;
+  if (!hyp_in(s, fx, fy, s.slope > 0 ? z1 : z0)) return false;
   bool all = true;
   for (int n = 0; n < 8; n++) {
     double qx = (n & 1) ? x1 : x0, qy = (n & 2) ? y1 : y0, qz = (n & 4) ? z1 : z0;
