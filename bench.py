@@ -453,6 +453,12 @@ def main():
         m = run_tile("tile20m_high", max(3, args.steps // 2), max(1, args.warmup // 2), ctx)
         mk = kernel_times(m["stats"])
         mst = m["stats"][-1]
+        mtraffic = None
+        tfile = os.path.join(ROOT, "profiles", "traffic_k_ground_tile20m_high.txt")
+        if os.path.exists(tfile):
+            for line in open(tfile):
+                if line.startswith("hbm_bytes_per_launch"):
+                    mtraffic = float(line.split("=")[1])
         out["ground_mix"] = {
             "workload": WORKLOADS["tile20m_high"], "n_points": m["n"],
             "value": round(m["value"], 1), "unit": "points/s",
@@ -462,6 +468,9 @@ def main():
             "p99_ms": round(percentile(m["lat"], 99), 3),
             "k_ground_ms": round(mk["k_ground"], 3),
             "k_ground_gbs": round(13.0 * m["n"] / (mk["k_ground"] * 1e-3) / 1e9, 2),
+            "k_ground_traffic": mtraffic,
+            "k_ground_traffic_gbs": (round(mtraffic / (mk["k_ground"] * 1e-3) / 1e9, 1)
+                                     if mtraffic else None),
             "breakdown_ms": breakdown(m["stats"]),
             "classes": classes(mst),
             "exact_paths": {"gap_points": int(mst["exact_gap_points"]),
