@@ -212,6 +212,19 @@ int wb_classify(wb_tree *tree, const double bbmin[3], const double bbmax[3],
                 const wb_classify_params *params, uint8_t *class_out, int flags,
                 wb_tile *tiles_out, uint64_t tiles_cap, wb_stats *stats);
 
+/* --- several trees classified together (a serving loop's small tiles):
+ *     wb_classify on each tree with its own header corners (bbmin and bbmax:
+ *     3 doubles per tree), the trees' ground tests in one launch of the
+ *     ground kernel and one of the wave-per-point kernel, so small tiles
+ *     share a full device.  No counterpart in the reference, which
+ *     classifies one cloud at a time (threads.cpp:619-660).  Class bytes
+ *     and statistics equal those of wb_classify on each tree alone.  Every
+ *     tree belongs to the same context; class_out[k]: n_k bytes (host, or
+ *     device with WB_DEVICE_INPUT); stats (may be NULL): ntrees records. */
+int wb_classify_batch(wb_tree *const *trees, int ntrees, const double *bbmin,
+                      const double *bbmax, const wb_classify_params *params,
+                      uint8_t *const *class_out, int flags, wb_stats *stats);
+
 /* the same pass with the tiling taken from the corners of several LAS
  * headers (x, y, z each): the BoundRect of them all, as sizeToFit builds it
  * (wolkencanvas.cpp:136-146, 501-519) */

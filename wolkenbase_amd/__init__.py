@@ -145,6 +145,9 @@ SIGNATURES = {
     "wb_count_points_in": (C.c_int, [_vp, C.c_int, _dp, C.c_uint64, _u64p]),
     "wb_count_points_in_noalloc": (C.c_int, [_vp, C.c_int, _dp, C.c_uint64, _u64p]),
     "wb_points_in": (C.c_int, [_vp, C.c_int, _dp, C.c_uint64, _u64p, _u32p, C.c_uint64]),
+    "wb_classify_batch": (C.c_int, [C.POINTER(_vp), C.c_int, _dp, _dp,
+                                    C.POINTER(ClassifyParams), C.POINTER(_vp), C.c_int,
+                                    C.POINTER(Stats)]),
     "wb_classify": (C.c_int, [_vp, _dp, _dp, C.POINTER(ClassifyParams), _vp, C.c_int,
                               C.POINTER(TileRecord), C.c_uint64, C.POINTER(Stats)]),
     "wb_tile_count": (C.c_int, [_dp, _dp, C.c_double, _u64p]),
@@ -571,6 +574,32 @@ def classify(tree, bbmin, bbmax, params=None, want_tiles=True, out=None):
     tree.ctx.check(rc, "wb_classify")
     tarr = np.ctypeslib.as_array(tiles)[:ntiles].copy() if want_tiles else None
     return cls, tarr, st.as_dict()
+
+
+def classify_batch(trees, bbmins, bbmaxs, params=None, outs=None):
+    """classify() of several trees of one context in one pass, their ground
+    tests in one kernel launch.  bbmins, bbmaxs: one header box per tree.
+    outs: None (host class bytes are returned) or one device tensor (uint8,
+    n) per tree.  Returns (list of class bytes, list of stats dicts)."""
+    params = params or default_params()
+    k = len(trees)
+    if k == 0:
+        return [], []
+    a = _arr(np.asarray(bbmins, dtype=np.float64).reshape(k, 3), np.float64)
+    b = _arr(np.asarray(bbmaxs, dtype=np.float64).reshape(k, 3), np.float64)
+    handles = (_vp * k)(*[t.h for t in trees])
+    if outs is not None:
+        cls = list(outs)
+        ptrs = (_vp * k)(*[o.data_ptr() for o in cls])
+        flags = WB_DEVICE_INPUT
+    else:
+        cls = [np.zeros(t.n_input, dtype=np.uint8) for t in trees]
+        ptrs = (_vp * k)(*[c.ctypes.data for c in cls])
+        flags = 0
+    st = (Stats * k)()
+    rc = lib().wb_classify_batch(handles, k, _dptr(a), _dptr(b), C.byref(params), ptrs, flags, st)
+    trees[0].ctx.check(rc, "wb_classify_batch")
+    return cls, [st[j].as_dict() for j in range(k)]
 
 
 def scan_tiles(tree, bbmin, bbmax, params=None):

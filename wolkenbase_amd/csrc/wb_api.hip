@@ -480,14 +480,12 @@ int wb_tile_count_corners(const double *corners, int ncorners, double tile_size,
   return WB_OK;
 }
 
-// wb_classify and wb_scan_tiles: the pass up to POSTSCAN, and CLASSIFY unless
-// scan_only
-static int run_pass(wb_tree *t, const double *corners, int ncorners,
-                    const wb_classify_params *p, uint8_t *class_out, int flags,
-                    wb_tile *tiles_out, uint64_t tiles_cap, wb_stats *stats, bool scan_only)
+// the checks every classification pass makes of its parameters and tiling;
+// WB_OK or the code (the message on the tree's context)
+static int check_pass(wb_tree *t, const double *corners, int ncorners,
+                      const wb_classify_params *p, uint64_t *ntiles_out)
 {
   wb_ctx *c = t->ctx;
-  if (!scan_only && !class_out) return fail(c, "no class output", WB_EINVAL);
   if (!(p->tile_size > 0) || !(p->max_slope > 0) || !(p->min_hs >= 0) || !isfinite(p->thickness))
     return fail(c, "bad classification parameters", WB_EINVAL);
   if (p->fit_precision < WB_FIT_FP64 || p->fit_precision > WB_FIT_FP32_COMP)
@@ -505,6 +503,49 @@ static int run_pass(wb_tree *t, const double *corners, int ncorners,
   uint64_t ntiles;
   wb_tile_count_corners(corners, ncorners, p->tile_size, &ntiles);
   if (ntiles > 0xffffffffull) return fail(c, "too many tiles", WB_ERANGE);
+  *ntiles_out = ntiles;
+  return WB_OK;
+}
+
+static void fill_stats(wb_stats *stats, const wb_tree *t, const ClassifyOut &co, uint64_t ntiles,
+                       double ms_total)
+{
+  memset(stats, 0, sizeof *stats);
+  stats->n_unclassified = co.hist[0];
+  stats->n_class1 = co.hist[1];
+  stats->n_class2 = co.hist[2];
+  stats->ntiles = ntiles;
+  stats->nonempty_tiles = co.nonempty;
+  stats->sector_borderline = co.borderline;
+  stats->exact_gap_points = co.nslow;
+  stats->atan_exact_calls = co.atan_exact;
+  stats->heavy_points = co.nheavy;
+  stats->sort_passes = t->sort_passes;
+  stats->maxdepth = t->maxdepth;
+  stats->nleaves = t->nleaves;
+  stats->nnodes = t->nnodes;
+  stats->ms_build = t->ms_build;
+  stats->ms_sort = t->ms_sort;
+  stats->ms_classify_kernel = co.ms_classify_kernel;
+  stats->ms_heavy_kernel = co.ms_heavy_kernel;
+  stats->ms_tiles = co.ms_tiles;
+  stats->ms_scan = co.ms_scan;
+  stats->ms_classify = co.ms_classify;
+  stats->ms_total = ms_total;
+  stats->fit_fallbacks = co.fallbacks;
+}
+
+// wb_classify and wb_scan_tiles: the pass up to POSTSCAN, and CLASSIFY unless
+// scan_only
+static int run_pass(wb_tree *t, const double *corners, int ncorners,
+                    const wb_classify_params *p, uint8_t *class_out, int flags,
+                    wb_tile *tiles_out, uint64_t tiles_cap, wb_stats *stats, bool scan_only)
+{
+  wb_ctx *c = t->ctx;
+  if (!scan_only && !class_out) return fail(c, "no class output", WB_EINVAL);
+  uint64_t ntiles = 0;
+  const int chk = check_pass(t, corners, ncorners, p, &ntiles);
+  if (chk != WB_OK) return chk;
   if (tiles_out && tiles_cap < ntiles) return fail(c, "tile buffer too small", WB_EINVAL);
   API_TRY(c)
   WB_CHECK(hipSetDevice(c->device));
@@ -539,31 +580,7 @@ static int run_pass(wb_tree *t, const double *corners, int ncorners,
       o.tilt_y = r.tilt_y;
     }
   }
-  if (stats) {
-    memset(stats, 0, sizeof *stats);
-    stats->n_unclassified = co.hist[0];
-    stats->n_class1 = co.hist[1];
-    stats->n_class2 = co.hist[2];
-    stats->ntiles = ntiles;
-    stats->nonempty_tiles = co.nonempty;
-    stats->sector_borderline = co.borderline;
-    stats->exact_gap_points = co.nslow;
-    stats->atan_exact_calls = co.atan_exact;
-    stats->heavy_points = co.nheavy;
-    stats->sort_passes = t->sort_passes;
-    stats->maxdepth = t->maxdepth;
-    stats->nleaves = t->nleaves;
-    stats->nnodes = t->nnodes;
-    stats->ms_build = t->ms_build;
-    stats->ms_sort = t->ms_sort;
-    stats->ms_classify_kernel = co.ms_classify_kernel;
-    stats->ms_heavy_kernel = co.ms_heavy_kernel;
-    stats->ms_tiles = co.ms_tiles;
-    stats->ms_scan = co.ms_scan;
-    stats->ms_classify = co.ms_classify;
-    stats->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    stats->fit_fallbacks = co.fallbacks;
-  }
+  if (stats) fill_stats(stats, t, co, ntiles, std::chrono::duration<double, std::milli>(t1 - t0).count());
   return WB_OK;
   API_CATCH(c)
 }
@@ -583,6 +600,51 @@ int wb_classify_corners(wb_tree *t, const double *corners, int ncorners,
 {
   if (!t || !corners || ncorners <= 0 || !p || !class_out) return WB_EINVAL;
   return run_pass(t, corners, ncorners, p, class_out, flags, tiles_out, tiles_cap, stats, false);
+}
+
+int wb_classify_batch(wb_tree *const *trees, int ntrees, const double *bbmin,
+                      const double *bbmax, const wb_classify_params *p, uint8_t *const *class_out,
+                      int flags, wb_stats *stats)
+{
+  if (!trees || ntrees <= 0 || !bbmin || !bbmax || !p || !class_out) return WB_EINVAL;
+  if (!trees[0]) return WB_EINVAL;
+  wb_ctx *c = trees[0]->ctx;
+  std::vector<double> corners(6 * (size_t)ntrees);
+  std::vector<uint64_t> ntiles(ntrees);
+  for (int k = 0; k < ntrees; k++) {
+    if (!trees[k] || !class_out[k]) return fail(c, "null tree or class output", WB_EINVAL);
+    if (trees[k]->ctx != c) return fail(c, "trees of different contexts", WB_EINVAL);
+    for (int a = 0; a < 3; a++) {
+      corners[6 * k + a] = bbmin[3 * k + a];
+      corners[6 * k + 3 + a] = bbmax[3 * k + a];
+    }
+    const int chk = check_pass(trees[k], &corners[6 * k], 2, p, &ntiles[k]);
+    if (chk != WB_OK) return chk;
+  }
+  API_TRY(c)
+  WB_CHECK(hipSetDevice(c->device));
+  auto t0 = std::chrono::steady_clock::now();
+  const bool host_out = !(flags & WB_DEVICE_INPUT);
+  std::vector<uint8_t *> dcls(ntrees);
+  for (int k = 0; k < ntrees; k++)
+    dcls[k] = host_out ? (uint8_t *)wb_alloc(c, trees[k]->n_input + 16) : class_out[k];
+  std::vector<ClassifyOut> co(ntrees);
+  wb_run_classify_batch(c, trees, ntrees, corners.data(), p, dcls.data(), co.data());
+  if (host_out) {
+    for (int k = 0; k < ntrees; k++)
+      if (trees[k]->n_input)
+        WB_CHECK(hipMemcpyAsync(class_out[k], dcls[k], trees[k]->n_input, hipMemcpyDeviceToHost,
+                                c->stream));
+    WB_CHECK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < ntrees; k++) wb_free(c, dcls[k], trees[k]->n_input + 16);
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  if (stats)
+    for (int k = 0; k < ntrees; k++)
+      fill_stats(&stats[k], trees[k], co[k], ntiles[k],
+                 std::chrono::duration<double, std::milli>(t1 - t0).count());
+  return WB_OK;
+  API_CATCH(c)
 }
 
 int wb_scan_tiles(wb_tree *t, const double bbmin[3], const double bbmax[3],

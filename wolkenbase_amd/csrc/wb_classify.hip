@@ -1463,12 +1463,13 @@ enum {
 // (shape.cpp:119-125) with its axis at x, y instead of x + 0, y + 0: the
 // two differ only in the sign of a zero, which no test below can see (every
 // use is a difference that is squared, compared or passed to hypot).
-template <bool COUNT, int MINW>
-__global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
+// blk: the block's index among the blocks of its tree; stacks, LW, edges:
+// the block's shared arrays (edges: the bin-edge table, read by wedge
+// refreshes)
+template <bool COUNT>
+__device__ __forceinline__ void ground_body(const ClassifyArgs &A, uint32_t blk, WaveStack *stacks,
+                                            LaneWedges &LW, float *edges)
 {
-  __shared__ WaveStack stacks[256 / WAVE];
-  __shared__ LaneWedges LW;
-  __shared__ float edges[3 * 128];  // the bin-edge table, read by wedge refreshes
   WaveStack &S = stacks[threadIdx.x / WAVE];
   LW.n[threadIdx.x] = -1;
   for (int k = threadIdx.x; k < 3 * 128; k += 256) edges[k] = A.edges[k];
@@ -1476,7 +1477,7 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
   const TreeView &T = A.T;
   const uint32_t lane = __lane_id();
   // work index: a sorted slot, or a work item
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x;
   const bool inrange = i < (A.items ? A.n_items : T.n);
   const double *ttan = A.tab, *tcos = A.tab + 511, *tsin = A.tab + 1023;
   const uint64_t t_start = COUNT ? wall_clock64() : 0;
@@ -1723,7 +1724,7 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
       v[k] = t;
     }
     if (lane == 0) {
-      unsigned long long *ctr = A.counters + (blockIdx.x % CT_COPIES) * CT_STRIDE;
+      unsigned long long *ctr = A.counters + (blk % CT_COPIES) * CT_STRIDE;
       for (int k = 0; k < CT_WAVES; k++) atomicAdd(&ctr[k], (unsigned long long)v[k]);
       atomicAdd(&ctr[CT_WAVES], 1ull);
       const uint64_t dt = wall_clock64() - t_start;
@@ -1738,6 +1739,34 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
       atomicAdd(&ctr[CT_ITER_B], (unsigned long long)iter_b);
     }
   }
+}
+
+
+template <bool COUNT, int MINW>
+__global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
+{
+  __shared__ WaveStack stacks[256 / WAVE];
+  __shared__ LaneWedges LW;
+  __shared__ float edges[3 * 128];
+  ground_body<COUNT>(A, blockIdx.x, stacks, LW, edges);
+}
+
+// The ground tests of several trees in one launch: blocks boff[k] ..
+// boff[k+1]-1 take tree k, whose arguments are AA[k].
+template <int MINW>
+__global__ void __launch_bounds__(256, MINW) k_ground_batch(const ClassifyArgs *AA,
+                                                            const uint32_t *boff, int ntrees)
+{
+  __shared__ WaveStack stacks[256 / WAVE];
+  __shared__ LaneWedges LW;
+  __shared__ float edges[3 * 128];
+  int lo = 0, hi = ntrees;  // boff[lo] <= blockIdx.x < boff[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (boff[mid] <= blockIdx.x) lo = mid;
+    else hi = mid;
+  }
+  ground_body<false>(AA[lo], blockIdx.x - boff[lo], stacks, LW, edges);
 }
 
 // One heavy point per 64-lane block.  The shape is the same in every lane,
@@ -1763,22 +1792,29 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
 // each of levels 1..20.
 #define HB_STK (64 * WB_MAX_LEVEL - 64)
 
-template <bool COUNT>
 #ifndef WB_HEAVY_MINW
 #define WB_HEAVY_MINW 4
 #endif
-__global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs A, const HeavyRec *heavy,
-                                                     uint32_t nheavy)
+// The block's shared arrays: the walk's stack, the leaf batch (leaf,
+// bounds, group mask, prefix of counts), the care wedges of walks A and B
+// (uniform).
+struct HeavyShared {
+  int stk[HB_STK];
+  uint32_t lq[HB_LEAVES], lb[HB_LEAVES], le[HB_LEAVES], lmask[HB_LEAVES];
+  uint32_t lpre[HB_LEAVES + 1];
+  wb::Wedges wg;
+};
+
+template <bool COUNT>
+__device__ __forceinline__ void heavy_body(const ClassifyArgs &A, const HeavyRec hr, uint32_t blk,
+                                           HeavyShared &SH)
 {
-  __shared__ int stk[HB_STK];
-  __shared__ uint32_t lq[HB_LEAVES], lb[HB_LEAVES], le[HB_LEAVES], lmask[HB_LEAVES];
-  __shared__ uint32_t lpre[HB_LEAVES + 1];
-  __shared__ wb::Wedges wg;  // the care wedges of walks A and B (uniform)
-  if (blockIdx.x >= nheavy) return;
+  int *stk = SH.stk;
+  uint32_t *lq = SH.lq, *lb = SH.lb, *le = SH.le, *lmask = SH.lmask, *lpre = SH.lpre;
+  wb::Wedges &wg = SH.wg;
   const uint64_t t_start = COUNT ? wall_clock64() : 0;
   uint64_t c_leaf = 0, c_pts = 0, c_exp = 0, c_test = 0, c_in = 0;
   const TreeView &T = A.T;
-  const HeavyRec hr = heavy[blockIdx.x];
   const uint32_t i = hr.i;  // work index (k_ground hands over only lanes with a tile)
   const uint32_t lane = threadIdx.x;
   const double *ttan = A.tab, *tcos = A.tab + 511, *tsin = A.tab + 1023;
@@ -2033,7 +2069,7 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
       // per start walk: points, passes, leaves, slots, expansions, ticks,
       // slots tested, slots inside the shape
       unsigned long long *h =
-          A.counters + (blockIdx.x % CT_COPIES) * CT_STRIDE + CT_N + 8 * hr.start;
+          A.counters + (blk % CT_COPIES) * CT_STRIDE + CT_N + 8 * hr.start;
       atomicAdd(&h[0], 1ull);
       if (verdict == 1) atomicAdd(&h[1], 1ull);
       atomicAdd(&h[2], (unsigned long long)c_leaf);
@@ -2044,6 +2080,34 @@ __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs
       atomicAdd(&h[7], (unsigned long long)c_in);
     }
   }
+}
+
+
+template <bool COUNT>
+__global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs A, const HeavyRec *heavy,
+                                                                    uint32_t nheavy)
+{
+  __shared__ HeavyShared SH;
+  if (blockIdx.x >= nheavy) return;
+  heavy_body<COUNT>(A, heavy[blockIdx.x], blockIdx.x, SH);
+}
+
+// The heavy points of several trees in one launch: blocks hoff[k] ..
+// hoff[k+1]-1 take the heavy records of tree k (AA[k].heavy).
+__global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy_batch(const ClassifyArgs *AA,
+                                                                          const uint32_t *hoff,
+                                                                          int ntrees)
+{
+  __shared__ HeavyShared SH;
+  if (blockIdx.x >= hoff[ntrees]) return;
+  int lo = 0, hi = ntrees;  // hoff[lo] <= blockIdx.x < hoff[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (hoff[mid] <= blockIdx.x) lo = mid;
+    else hi = mid;
+  }
+  const ClassifyArgs &A = AA[lo];
+  heavy_body<false>(A, A.heavy[blockIdx.x - hoff[lo]], blockIdx.x, SH);
 }
 
 // an input merged into an earlier point at its location gets that point's class
@@ -2153,33 +2217,68 @@ void wb_run_points(wb_tree *t, int kind, const double *dparams, uint64_t ns, con
                      dparams, ns, doffs, dout);
 }
 
-void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
-                     const wb_classify_params *p, uint8_t *d_class_out, bool want_tiles,
-                     bool scan_only, ClassifyOut *out)
-{
-  wb_ctx *c = t->ctx;
-  hipStream_t st = c->stream;
-  const uint64_t n = t->n;
-  wb::SnakeSetup ss = wb::snake_setup_corners(corners, ncorners, p->tile_size);
+// The state of one tree's classification pass between its phases:
+// classify_prepare (tile geometry, membership, SCAN, POSTSCAN, the ground
+// test's arguments), the ground test (one tree, or several in one launch),
+// classify_finish (counts, Tile::nGround, tile records, timings, release).
+struct ClassifyRun {
+  wb_tree *t;
+  wb_ctx *c;  // allocations and stream (the tree's context, or a batch's)
+  hipStream_t st;
+  const wb_classify_params *p;
+  uint8_t *d_class_out;
+  bool want_tiles, scan_only;
+  ClassifyOut *out;
+  uint64_t n;
   TileGeo g;
+  hipEvent_t ev[8];
+  int32_t *tex, *tey, *memb, *grid;
+  uint32_t *tile_cnt, *tile_fill, *nmemb, *err, *nslow, *tile_off, *lists;
+  TileRec *rec;
+  unsigned long long *ctr, *dcount;
+  uint64_t gcells, total;
+  uint8_t *cls;
+  HeavyRec *heavy;
+  ClassifyArgs A;
+  uint32_t h_nheavy, h_nslow;
+};
+
+static bool classify_count_work()
+{
+  static const bool on = getenv("WB_CLASSIFY_COUNTERS") != nullptr;
+  return on;
+}
+
+static void classify_prepare(ClassifyRun &R, wb_tree *t, wb_ctx *c, const double *corners,
+                             int ncorners, const wb_classify_params *p, uint8_t *d_class_out,
+                             bool want_tiles, bool scan_only, ClassifyOut *out, bool counters)
+{
+  R.t = t; R.c = c; R.st = c->stream; R.p = p; R.d_class_out = d_class_out;
+  R.want_tiles = want_tiles; R.scan_only = scan_only; R.out = out;
+  R.cls = nullptr; R.heavy = nullptr; R.dcount = nullptr; R.h_nheavy = 0; R.h_nslow = 0;
+  hipStream_t st = R.st;
+  const uint64_t n = R.n = t->n;
+  wb::SnakeSetup ss = wb::snake_setup_corners(corners, ncorners, p->tile_size);
+  TileGeo &g = R.g;
   g.scx = ss.cx; g.scy = ss.cy; g.spacing = ss.spacing; g.r = ss.radius;
   g.lo = ss.lo; g.hi = ss.hi;
   g.ntiles = (uint32_t)((int64_t)ss.hi - ss.lo + 1);
-  hipEvent_t ev[8];
-  for (int k = 0; k < 8; k++) WB_CHECK(hipEventCreate(&ev[k]));
+  for (int k = 0; k < 8; k++) WB_CHECK(hipEventCreate(&R.ev[k]));
+  hipEvent_t *ev = R.ev;
   WB_CHECK(hipEventRecord(ev[0], st));
-  int32_t *tex = (int32_t *)wb_alloc(c, 4ull * g.ntiles), *tey = (int32_t *)wb_alloc(c, 4ull * g.ntiles);
-  uint32_t *tile_cnt = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
-  uint32_t *tile_fill = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
-  TileRec *rec = (TileRec *)wb_alloc(c, sizeof(TileRec) * (size_t)g.ntiles);
-  int32_t *memb = (int32_t *)wb_alloc(c, 4ull * MAX_MEMB * n + 16);
-  uint32_t *nmemb = (uint32_t *)wb_alloc(c, 4ull * n + 4);
+  int32_t *tex = R.tex = (int32_t *)wb_alloc(c, 4ull * g.ntiles);
+  int32_t *tey = R.tey = (int32_t *)wb_alloc(c, 4ull * g.ntiles);
+  uint32_t *tile_cnt = R.tile_cnt = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
+  uint32_t *tile_fill = R.tile_fill = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
+  TileRec *rec = R.rec = (TileRec *)wb_alloc(c, sizeof(TileRec) * (size_t)g.ntiles);
+  int32_t *memb = R.memb = (int32_t *)wb_alloc(c, 4ull * MAX_MEMB * n + 16);
+  uint32_t *nmemb = R.nmemb = (uint32_t *)wb_alloc(c, 4ull * n + 4);
   // err[0] error flags; then u64 counters: 0 sector borderline, 1 exact atan2i
   // steps, 2-4 class histogram, 5 non-empty tiles, 6 (two u32) exact gap
   // points and heavy points, 7 FP64 fit fallbacks
-  uint32_t *err = (uint32_t *)wb_alloc(c, 128);
-  unsigned long long *ctr = (unsigned long long *)err + 1;
-  uint32_t *nslow = (uint32_t *)(ctr + 6);
+  uint32_t *err = R.err = (uint32_t *)wb_alloc(c, 128);
+  unsigned long long *ctr = R.ctr = (unsigned long long *)err + 1;
+  uint32_t *nslow = R.nslow = (uint32_t *)(ctr + 6);
   WB_CHECK(hipMemsetAsync(err, 0, 128, st));
   WB_CHECK(hipMemsetAsync(tile_cnt, 0, 4ull * g.ntiles, st));
   WB_CHECK(hipMemsetAsync(tile_fill, 0, 4ull * g.ntiles, st));
@@ -2195,8 +2294,8 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
   g.gy0 = hb[2];
   g.gw = hb[1] - hb[0] + 1;
   g.gh = hb[3] - hb[2] + 1;
-  const uint64_t gcells = (uint64_t)g.gw * (uint64_t)g.gh;
-  int32_t *grid = (int32_t *)wb_alloc(c, 4 * gcells);
+  const uint64_t gcells = R.gcells = (uint64_t)g.gw * (uint64_t)g.gh;
+  int32_t *grid = R.grid = (int32_t *)wb_alloc(c, 4 * gcells);
   hipLaunchKernelGGL(k_fill_i32, dim3(wb_blocks(gcells, 256)), dim3(256), 0, st, grid, gcells, -1);
   g.grid = grid;
   hipLaunchKernelGGL(k_tile_grid, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey, grid);
@@ -2210,10 +2309,10 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
   wb_free(c, lwin, sizeof(LeafWin) * ((size_t)t->nleaves + 1));
   wb_free(c, lflags, (size_t)LEAF_WIN * t->nleaves + 16);
   // tile lists in canonical order
-  uint32_t *tile_off = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
+  uint32_t *tile_off = R.tile_off = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
   WB_CHECK(hipMemcpyAsync(tile_off, tile_cnt, 4ull * g.ntiles, hipMemcpyDeviceToDevice, st));
-  uint64_t total = wb_exclusive_scan_u32(c, tile_off, g.ntiles);
-  uint32_t *lists = (uint32_t *)wb_alloc(c, 4 * total + 4);
+  const uint64_t total = R.total = wb_exclusive_scan_u32(c, tile_off, g.ntiles);
+  uint32_t *lists = R.lists = (uint32_t *)wb_alloc(c, 4 * total + 4);
   if (n > 0)
     hipLaunchKernelGGL(k_tile_lists, dim3(wb_blocks(n, 256)), dim3(256), 0, st, n, memb, nmemb,
                        g.lo, t->crank, tile_off, tile_fill, lists);
@@ -2245,60 +2344,79 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
   hipLaunchKernelGGL(k_postscan, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey, rec,
                      ctr + 5);
   WB_CHECK(hipEventRecord(ev[2], st));
-  uint8_t *cls = nullptr;
-  HeavyRec *heavy = nullptr;
-  unsigned long long *dcount = nullptr;
-  uint32_t h_nheavy = 0, h_nslow = 0;
-  int32_t *nground = (int32_t *)tile_fill;  // reused once the lists are built
-  WB_CHECK(hipEventRecord(ev[4], st));
-  WB_CHECK(hipEventRecord(ev[5], st));
-  WB_CHECK(hipEventRecord(ev[6], st));
-  WB_CHECK(hipEventRecord(ev[7], st));
+  for (int k = 4; k < 8; k++) WB_CHECK(hipEventRecord(ev[k], st));
+  if (scan_only) return;
+  R.cls = (uint8_t *)wb_alloc(c, n + 16);
+  ClassifyArgs &A = R.A;
+  A.T = T; A.g = g; A.memb = memb; A.nmemb = nmemb; A.rec = rec;
+  A.thickness = p->thickness; A.max_slope = p->max_slope;
+  A.cube_wedges = t->outside_root ? 0 : 1;
+  A.tab = c->d_tan; A.K = c->kconst; A.cls = R.cls; A.cls_in = d_class_out; A.nslow = nslow;
+  A.edges = c->d_edges;
+  A.atan_exact = ctr + 1;
+  A.items = nullptr;
+  A.n_items = 0;
+  A.item_cls = nullptr;
+  if (counters) {
+    R.dcount = (unsigned long long *)wb_alloc(c, CT_STRIDE * CT_COPIES * 8);
+    WB_CHECK(hipMemsetAsync(R.dcount, 0, CT_STRIDE * CT_COPIES * 8, st));
+  }
+  A.counters = R.dcount;
+  R.heavy = (HeavyRec *)wb_alloc(c, sizeof(HeavyRec) * (n + 1));
+  A.heavy = R.heavy;
+  A.nheavy = nslow + 1;
+}
+
+// the ground test of one tree: the main kernel, then the heavy kernel for
+// the points it hands over
+static void classify_ground_single(ClassifyRun &R)
+{
+  hipStream_t st = R.st;
+  ClassifyArgs &A = R.A;
+  const bool count_work = R.dcount != nullptr;
+  WB_CHECK(hipEventRecord(R.ev[4], st));
+  if (R.n > 0) {
+    if (count_work)
+      hipLaunchKernelGGL((k_ground<true, WB_GROUND_MINW>), dim3(wb_blocks(R.n, 256)), dim3(256),
+                         0, st, A);
+    else
+      hipLaunchKernelGGL((k_ground<false, WB_GROUND_MINW>), dim3(wb_blocks(R.n, 256)), dim3(256),
+                         0, st, A);
+  }
+  WB_CHECK(hipEventRecord(R.ev[5], st));
+  WB_CHECK(hipMemcpyAsync(&R.h_nheavy, R.nslow + 1, 4, hipMemcpyDeviceToHost, st));
+  WB_CHECK(hipStreamSynchronize(st));
+  WB_CHECK(hipEventRecord(R.ev[6], st));
+  if (R.h_nheavy) {
+    if (count_work)
+      hipLaunchKernelGGL(k_ground_heavy<true>, dim3(R.h_nheavy), dim3(64), 0, st, A, R.heavy,
+                         R.h_nheavy);
+    else
+      hipLaunchKernelGGL(k_ground_heavy<false>, dim3(R.h_nheavy), dim3(64), 0, st, A, R.heavy,
+                         R.h_nheavy);
+  }
+  WB_CHECK(hipEventRecord(R.ev[7], st));
+}
+
+static void classify_finish(ClassifyRun &R)
+{
+  wb_tree *t = R.t;
+  wb_ctx *c = R.c;
+  hipStream_t st = R.st;
+  const uint64_t n = R.n;
+  const TileGeo &g = R.g;
+  hipEvent_t *ev = R.ev;
+  ClassifyOut *out = R.out;
+  const bool want_tiles = R.want_tiles, scan_only = R.scan_only;
+  int32_t *memb = R.memb;
+  uint32_t *nmemb = R.nmemb, *nslow = R.nslow, *err = R.err;
+  TileRec *rec = R.rec;
+  unsigned long long *ctr = R.ctr, *dcount = R.dcount;
+  uint8_t *cls = R.cls, *d_class_out = R.d_class_out;
+  const ClassifyArgs &A = R.A;
+  int32_t *nground = (int32_t *)R.tile_fill;  // reused once the lists are built
   if (!scan_only) {
-    // classify
-    cls = (uint8_t *)wb_alloc(c, n + 16);
-    ClassifyArgs A;
-    A.T = T; A.g = g; A.memb = memb; A.nmemb = nmemb; A.rec = rec;
-    A.thickness = p->thickness; A.max_slope = p->max_slope;
-    A.cube_wedges = t->outside_root ? 0 : 1;
-    A.tab = c->d_tan; A.K = c->kconst; A.cls = cls; A.cls_in = d_class_out; A.nslow = nslow;
-    A.edges = c->d_edges;
-    A.atan_exact = ctr + 1;
-    A.items = nullptr;
-    A.n_items = 0;
-    A.item_cls = nullptr;
-    WB_CHECK(hipEventRecord(ev[4], st));
-    static const bool count_work = getenv("WB_CLASSIFY_COUNTERS") != nullptr;
-    if (count_work) {
-      dcount = (unsigned long long *)wb_alloc(c, CT_STRIDE * CT_COPIES * 8);
-      WB_CHECK(hipMemsetAsync(dcount, 0, CT_STRIDE * CT_COPIES * 8, st));
-    }
-    A.counters = dcount;
-    heavy = (HeavyRec *)wb_alloc(c, sizeof(HeavyRec) * (n + 1));
-    A.heavy = heavy;
-    A.nheavy = nslow + 1;
-    if (n > 0) {
-      if (count_work)
-        hipLaunchKernelGGL((k_ground<true, WB_GROUND_MINW>), dim3(wb_blocks(n, 256)), dim3(256),
-                           0, st, A);
-      else
-        hipLaunchKernelGGL((k_ground<false, WB_GROUND_MINW>), dim3(wb_blocks(n, 256)), dim3(256),
-                           0, st, A);
-    }
-    WB_CHECK(hipEventRecord(ev[5], st));
-    WB_CHECK(hipMemcpyAsync(&h_nheavy, nslow + 1, 4, hipMemcpyDeviceToHost, st));
-    WB_CHECK(hipStreamSynchronize(st));
-    WB_CHECK(hipEventRecord(ev[6], st));
-    if (h_nheavy) {
-      if (count_work)
-        hipLaunchKernelGGL(k_ground_heavy<true>, dim3(h_nheavy), dim3(64), 0, st, A, heavy,
-                           h_nheavy);
-      else
-        hipLaunchKernelGGL(k_ground_heavy<false>, dim3(h_nheavy), dim3(64), 0, st, A, heavy,
-                           h_nheavy);
-    }
-    WB_CHECK(hipEventRecord(ev[7], st));
-    WB_CHECK(hipMemcpyAsync(&h_nslow, nslow, 4, hipMemcpyDeviceToHost, st));
+    WB_CHECK(hipMemcpyAsync(&R.h_nslow, nslow, 4, hipMemcpyDeviceToHost, st));
     WB_CHECK(hipMemsetAsync(nground, 0, 4ull * g.ntiles, st));
     if (n > 0) {
       const unsigned nbk = wb_blocks(n, 256);
@@ -2369,8 +2487,8 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
     out->tey.resize(g.ntiles);
     std::vector<int32_t> ng(g.ntiles, 0);
     WB_CHECK(hipMemcpyAsync(out->tiles.data(), rec, sizeof(TileRec) * (size_t)g.ntiles, hipMemcpyDeviceToHost, st));
-    WB_CHECK(hipMemcpyAsync(out->tex.data(), tex, 4ull * g.ntiles, hipMemcpyDeviceToHost, st));
-    WB_CHECK(hipMemcpyAsync(out->tey.data(), tey, 4ull * g.ntiles, hipMemcpyDeviceToHost, st));
+    WB_CHECK(hipMemcpyAsync(out->tex.data(), R.tex, 4ull * g.ntiles, hipMemcpyDeviceToHost, st));
+    WB_CHECK(hipMemcpyAsync(out->tey.data(), R.tey, 4ull * g.ntiles, hipMemcpyDeviceToHost, st));
     if (!scan_only)
       WB_CHECK(hipMemcpyAsync(ng.data(), nground, 4ull * g.ntiles, hipMemcpyDeviceToHost, st));
     WB_CHECK(hipStreamSynchronize(st));
@@ -2394,7 +2512,7 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
   out->hist[0] = hc[2]; out->hist[1] = hc[3]; out->hist[2] = hc[4];
   out->nonempty = hc[5];
   out->fallbacks = hc[7];
-  out->nslow = h_nslow;
+  out->nslow = R.h_nslow;
   if (dcount) {
     std::vector<unsigned long long> hc((size_t)CT_STRIDE * CT_COPIES);
     WB_CHECK(hipMemcpy(hc.data(), dcount, hc.size() * 8, hipMemcpyDeviceToHost));
@@ -2435,20 +2553,80 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
     wb_free(c, dcount, CT_STRIDE * CT_COPIES * 8);
   }
   out->ms_tiles = ms1; out->ms_scan = ms2; out->ms_classify = ms3;
-  wb_free(c, grid, 4 * gcells);
-  wb_free(c, tex, 4ull * g.ntiles);
-  wb_free(c, tey, 4ull * g.ntiles);
-  wb_free(c, tile_cnt, 4ull * g.ntiles + 4);
-  wb_free(c, tile_fill, 4ull * g.ntiles + 4);
+  wb_free(c, R.grid, 4 * R.gcells);
+  wb_free(c, R.tex, 4ull * g.ntiles);
+  wb_free(c, R.tey, 4ull * g.ntiles);
+  wb_free(c, R.tile_cnt, 4ull * g.ntiles + 4);
+  wb_free(c, R.tile_fill, 4ull * g.ntiles + 4);
   wb_free(c, rec, sizeof(TileRec) * (size_t)g.ntiles);
   wb_free(c, memb, 4ull * MAX_MEMB * n + 16);
   wb_free(c, nmemb, 4ull * n + 4);
-  wb_free(c, tile_off, 4ull * g.ntiles + 4);
-  wb_free(c, lists, 4 * total + 4);
+  wb_free(c, R.tile_off, 4ull * g.ntiles + 4);
+  wb_free(c, R.lists, 4 * R.total + 4);
   if (cls) wb_free(c, cls, n + 16);
-  if (heavy) wb_free(c, heavy, sizeof(HeavyRec) * (n + 1));
-  out->nheavy = h_nheavy;
+  if (R.heavy) wb_free(c, R.heavy, sizeof(HeavyRec) * (n + 1));
+  out->nheavy = R.h_nheavy;
   if (h_err[0] & 1u) { wb_free(c, err, 128); throw std::string("a point lies in more than 4 tile cylinders"); }
   if (h_err[0] & 2u) { wb_free(c, err, 128); throw std::string("a tile cylinder holds more than 1024 points"); }
   wb_free(c, err, 128);
+}
+
+void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
+                     const wb_classify_params *p, uint8_t *d_class_out, bool want_tiles,
+                     bool scan_only, ClassifyOut *out)
+{
+  ClassifyRun R;
+  classify_prepare(R, t, t->ctx, corners, ncorners, p, d_class_out, want_tiles, scan_only, out,
+                   !scan_only && classify_count_work());
+  if (!scan_only) classify_ground_single(R);
+  classify_finish(R);
+}
+
+void wb_run_classify_batch(wb_ctx *c, wb_tree *const *trees, int ntrees, const double *corners,
+                           const wb_classify_params *p, uint8_t *const *d_class_out,
+                           ClassifyOut *outs)
+{
+  hipStream_t st = c->stream;
+  std::vector<ClassifyRun> R(ntrees);
+  for (int k = 0; k < ntrees; k++)
+    classify_prepare(R[k], trees[k], c, corners + 6 * k, 2, p, d_class_out[k], false, false,
+                     &outs[k], false);
+  // one launch of the ground kernel over every tree's points, then one of
+  // the heavy kernel over every tree's handed-over points
+  std::vector<ClassifyArgs> AA(ntrees);
+  std::vector<uint32_t> boff(ntrees + 1, 0), hoff(ntrees + 1, 0);
+  for (int k = 0; k < ntrees; k++) {
+    AA[k] = R[k].A;
+    const uint64_t nb = boff[k] + (uint64_t)wb_blocks(R[k].n, 256);
+    if (nb > 0xffffffffull) throw std::string("too many points in one batch");
+    boff[k + 1] = (uint32_t)nb;
+  }
+  ClassifyArgs *dAA = (ClassifyArgs *)wb_alloc(c, sizeof(ClassifyArgs) * ntrees);
+  uint32_t *doff = (uint32_t *)wb_alloc(c, 8 * (ntrees + 1));
+  WB_CHECK(hipMemcpyAsync(dAA, AA.data(), sizeof(ClassifyArgs) * ntrees, hipMemcpyHostToDevice, st));
+  WB_CHECK(hipMemcpyAsync(doff, boff.data(), 4 * (ntrees + 1), hipMemcpyHostToDevice, st));
+  for (int k = 0; k < ntrees; k++) WB_CHECK(hipEventRecord(R[k].ev[4], st));
+  if (boff[ntrees])
+    hipLaunchKernelGGL((k_ground_batch<WB_GROUND_MINW>), dim3(boff[ntrees]), dim3(256), 0, st, dAA,
+                       doff, ntrees);
+  for (int k = 0; k < ntrees; k++) {
+    WB_CHECK(hipEventRecord(R[k].ev[5], st));
+    WB_CHECK(hipMemcpyAsync(&R[k].h_nheavy, R[k].nslow + 1, 4, hipMemcpyDeviceToHost, st));
+  }
+  WB_CHECK(hipStreamSynchronize(st));
+  for (int k = 0; k < ntrees; k++) {
+    WB_CHECK(hipEventRecord(R[k].ev[6], st));
+    hoff[k + 1] = hoff[k] + R[k].h_nheavy;
+  }
+  if (hoff[ntrees]) {
+    uint32_t *dh = doff + ntrees + 1;
+    WB_CHECK(hipMemcpyAsync(dh, hoff.data(), 4 * (ntrees + 1), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_ground_heavy_batch, dim3(hoff[ntrees]), dim3(64), 0, st, dAA, dh,
+                       ntrees);
+  }
+  for (int k = 0; k < ntrees; k++) WB_CHECK(hipEventRecord(R[k].ev[7], st));
+  WB_CHECK(hipStreamSynchronize(st));
+  wb_free(c, dAA, sizeof(ClassifyArgs) * ntrees);
+  wb_free(c, doff, 8 * (ntrees + 1));
+  for (int k = 0; k < ntrees; k++) classify_finish(R[k]);
 }

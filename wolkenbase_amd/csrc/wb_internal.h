@@ -136,6 +136,12 @@ void wb_run_points(wb_tree *t, int kind, const double *dparams, uint64_t ns, con
                    uint32_t *dout);
 // the classification pass (tile geometry, membership, SCAN, POSTSCAN and,
 // unless scan_only, CLASSIFY); class bytes to d_class_out in input order
+// several trees of one context classified together, their ground tests in
+// one launch (tiles as wb_classify with each tree's two header corners,
+// corners + 6k); d_class_out[k]: device class bytes of tree k, input order
+void wb_run_classify_batch(wb_ctx *c, wb_tree *const *trees, int ntrees, const double *corners,
+                           const wb_classify_params *p, uint8_t *const *d_class_out,
+                           ClassifyOut *outs);
 void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
                      const wb_classify_params *p, uint8_t *d_class_out, bool want_tiles,
                      bool scan_only, ClassifyOut *out);
