@@ -128,6 +128,11 @@ int wb_ctx_sync(wb_ctx *ctx);
 /* free and total device memory of a device (admission control of a serving
  * loop; no counterpart in the reference, whose store lives on disk) */
 int wb_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
+/* pinned host memory (hipHostMalloc) and its release: receive buffers of a
+ * serving loop, which copies to and from the device read and write by DMA
+ * without staging (no counterpart in the reference).  bytes > 0. */
+int wb_host_alloc(uint64_t bytes, void **out);
+void wb_host_free(void *p);
 
 /* --- root cube: Octree::sizeFit (octree.cpp:268-310) -------------------- */
 int wb_size_fit(const double *corners, int ncorners, double center[3], double *side);

@@ -128,7 +128,10 @@ _u32p = C.POINTER(C.c_uint32)
 # exported symbols of include/wb_gpu.h with their ctypes signatures
 SIGNATURES = {
     "wb_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+
     "wb_ctx_destroy": (None, [_vp]),
+    "wb_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(_vp)]),
+    "wb_host_free": (None, [_vp]),
     "wb_device_memory": (C.c_int, [C.c_int, _u64p, _u64p]),
     "wb_last_error": (C.c_char_p, [_vp]),
     "wb_ctx_sync": (C.c_int, [_vp]),
@@ -206,6 +209,54 @@ def _arr(a, dtype):
 
 def _dptr(a):
     return a.ctypes.data_as(_dp)
+
+
+def _on_device(a):
+    """True for a tensor in device memory; a host tensor (pinned or not) is
+    host memory, passed by its address without WB_DEVICE_INPUT."""
+    return bool(getattr(a, "is_cuda", False))
+
+
+def _host_arrays(arrs, dtype):
+    """Host coordinate arrays as (keep-alive objects, addresses, length):
+    numpy arrays are made contiguous in `dtype`; host tensors (e.g. pinned
+    receive buffers) are passed as they are and must already be contiguous
+    arrays of that type."""
+    if all(hasattr(a, "data_ptr") for a in arrs):
+        isz = np.dtype(dtype).itemsize
+        for a in arrs:
+            if not a.is_contiguous() or a.element_size() != isz or a.is_floating_point() != (
+                    np.dtype(dtype).kind == "f"):
+                raise ValueError(f"host tensor must be a contiguous {np.dtype(dtype)} array")
+        return arrs, [a.data_ptr() for a in arrs], int(arrs[0].numel())
+    arrs = [_arr(a, dtype) for a in arrs]
+    return arrs, [a.ctypes.data for a in arrs], len(arrs[0])
+
+
+class _PinnedBlock:
+    """Owner of one wb_host_alloc block: freed when the last array viewing
+    it goes."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def __del__(self):
+        if self.ptr:
+            lib().wb_host_free(_vp(self.ptr))
+            self.ptr = None
+
+
+def pinned_empty(n, dtype):
+    """A numpy array of n elements in pinned host memory (hipHostMalloc):
+    copies between it and the device run by DMA without staging."""
+    dtype = np.dtype(dtype)
+    nbytes = max(1, int(n) * dtype.itemsize)
+    p = _vp()
+    if lib().wb_host_alloc(nbytes, C.byref(p)) != 0:
+        raise WbError(f"wb_host_alloc({nbytes}) failed")
+    buf = (C.c_uint8 * nbytes).from_address(p.value)
+    buf.owner = _PinnedBlock(p.value)
+    return np.frombuffer(buf, dtype=dtype, count=int(n))
 
 
 class Context:
@@ -339,14 +390,12 @@ class Octree:
         self.ctx = ctx
         xf = LasXform((C.c_double * 3)(*scale), (C.c_double * 3)(*offset), unit)
         cen = _arr(center, np.float64)
-        dev = hasattr(xi, "data_ptr")
+        dev = _on_device(xi)
         if dev:
             n = int(xi.numel())
             px, py, pz = xi.data_ptr(), yi.data_ptr(), zi.data_ptr()
         else:
-            xi, yi, zi = (_arr(a, np.int32) for a in (xi, yi, zi))
-            n = len(xi)
-            px, py, pz = xi.ctypes.data, yi.ctypes.data, zi.ctypes.data
+            keep, (px, py, pz), n = _host_arrays((xi, yi, zi), np.int32)
         h = _vp()
         flags = (WB_DEVICE_INPUT if dev else 0) | (WB_BUILD_DEDUPE if dedupe else 0)
         rc = lib().wb_tree_build(ctx.h, px, py, pz, n, C.byref(xf), _dptr(cen), float(side),
@@ -363,14 +412,12 @@ class Octree:
         self.ctx = ctx
         cen = _arr(center, np.float64)
         res = _arr(resolution, np.float64)
-        dev = hasattr(x, "data_ptr")
+        dev = _on_device(x)
         if dev:
             n = int(x.numel())
             px, py, pz = x.data_ptr(), y.data_ptr(), z.data_ptr()
         else:
-            x, y, z = (_arr(a, np.float64) for a in (x, y, z))
-            n = len(x)
-            px, py, pz = x.ctypes.data, y.ctypes.data, z.ctypes.data
+            keep, (px, py, pz), n = _host_arrays((x, y, z), np.float64)
         h = _vp()
         flags = (WB_DEVICE_INPUT if dev else 0) | (WB_BUILD_DEDUPE if dedupe else 0)
         rc = lib().wb_tree_build_xyz(ctx.h, px, py, pz, n, _dptr(res), _dptr(cen), float(side),
@@ -412,7 +459,7 @@ class Octree:
         nb = C.c_uint64()
         ch = cloud.h if cloud is not None else None
         flags = 0
-        if cls is not None and hasattr(cls, "data_ptr") and getattr(cls, "is_cuda", False):
+        if cls is not None and _on_device(cls):
             if cls.numel() != self.n_input:
                 raise ValueError(f"class bytes: {cls.numel()} for {self.n_input} input points")
             cp, flags = _vp(cls.data_ptr()), WB_DEVICE_INPUT
@@ -557,15 +604,29 @@ def classify(tree, bbmin, bbmax, params=None, want_tiles=True, out=None):
     """SCAN + POSTSCAN + CLASSIFY of every flowsnake tile (scan.cpp:31-179,
     classify.cpp:96-173).  Returns (class bytes in input order, tile records
     or None, stats dict).  If `out` is a device tensor (uint8, n), the classes
-    are written there and the first return value is `out`."""
+    stay in HBM there; if it is a host buffer of n bytes (a numpy uint8 array
+    or a host tensor, e.g. pinned), they are copied there; either way the
+    first return value is `out`."""
     params = params or default_params()
     a, b = _arr(bbmin, np.float64), _arr(bbmax, np.float64)
     ntiles = tile_count(a, b, params.tile_size) if want_tiles else 0
     tiles = (TileRecord * max(ntiles, 1))() if want_tiles else None
     st = Stats()
-    if out is not None and hasattr(out, "data_ptr"):
+    if out is not None and _on_device(out):
         cls = out
         ptr, flags = out.data_ptr(), WB_DEVICE_INPUT
+    elif out is not None:
+        # a host buffer (numpy or host tensor, e.g. pinned) of n bytes
+        cls = out
+        if hasattr(out, "data_ptr"):
+            if out.numel() * out.element_size() != tree.n_input or not out.is_contiguous():
+                raise ValueError("class output: a contiguous buffer of n bytes")
+            ptr = out.data_ptr()
+        else:
+            if out.dtype != np.uint8 or out.size != tree.n_input or not out.flags.c_contiguous:
+                raise ValueError("class output: a contiguous uint8 array of n bytes")
+            ptr = out.ctypes.data
+        flags = 0
     else:
         cls = np.zeros(tree.n_input, dtype=np.uint8)
         ptr, flags = cls.ctypes.data, 0
@@ -579,8 +640,9 @@ def classify(tree, bbmin, bbmax, params=None, want_tiles=True, out=None):
 def classify_batch(trees, bbmins, bbmaxs, params=None, outs=None):
     """classify() of several trees of one context in one pass, their ground
     tests in one kernel launch.  bbmins, bbmaxs: one header box per tree.
-    outs: None (host class bytes are returned) or one device tensor (uint8,
-    n) per tree.  Returns (list of class bytes, list of stats dicts)."""
+    outs: None (host class bytes are returned), one device tensor (uint8, n)
+    per tree, or one host buffer of n bytes per tree (numpy or host tensor,
+    e.g. pinned).  Returns (list of class bytes, list of stats dicts)."""
     params = params or default_params()
     k = len(trees)
     if k == 0:
@@ -590,8 +652,18 @@ def classify_batch(trees, bbmins, bbmaxs, params=None, outs=None):
     handles = (_vp * k)(*[t.h for t in trees])
     if outs is not None:
         cls = list(outs)
-        ptrs = (_vp * k)(*[o.data_ptr() for o in cls])
-        flags = WB_DEVICE_INPUT
+        if len(cls) != k:
+            raise ValueError(f"{len(cls)} class outputs for {k} trees")
+        dev = [_on_device(o) for o in cls]
+        if any(dev) != all(dev):
+            raise ValueError("class outputs: all device tensors or all host buffers")
+        for o, t in zip(cls, trees):
+            nb = o.numel() * o.element_size() if hasattr(o, "data_ptr") else o.nbytes
+            if nb != t.n_input:
+                raise ValueError(f"class output of {nb} bytes for {t.n_input} points")
+        ptrs = (_vp * k)(*[o.data_ptr() if hasattr(o, "data_ptr") else o.ctypes.data
+                           for o in cls])
+        flags = WB_DEVICE_INPUT if all(dev) else 0
     else:
         cls = [np.zeros(t.n_input, dtype=np.uint8) for t in trees]
         ptrs = (_vp * k)(*[c.ctypes.data for c in cls])

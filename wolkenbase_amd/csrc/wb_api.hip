@@ -46,7 +46,9 @@ static void *scratch_address(int device)
 
 // ---------------------------------------------------------------------------
 // device memory cache: buffers are returned to a size-keyed pool and reused,
-// so repeated builds of equal-sized tiles do not call hipMalloc again
+// so repeated builds of tiles of similar size do not call hipMalloc again
+// (the smallest cached buffer of at least the request is taken when it is
+// no more than 1.5 times the request, 2 times below 1 MiB)
 // ---------------------------------------------------------------------------
 static size_t round_size(size_t b)
 {
@@ -58,11 +60,14 @@ static size_t round_size(size_t b)
 void *wb_alloc(wb_ctx *c, size_t bytes)
 {
   size_t b = round_size(bytes ? bytes : 1);
-  auto it = c->cache.find(b);
-  if (it != c->cache.end()) {
+  auto it = c->cache.lower_bound(b);
+  const size_t cap = b < (1u << 20) ? 2 * b : b + (b >> 1);
+  if (it != c->cache.end() && it->first <= cap) {
     void *p = it->second;
+    const size_t have = it->first;
     c->cache.erase(it);
-    c->cached_bytes -= b;
+    c->cached_bytes -= have;
+    c->held[p] = have;
     return p;
   }
   void *p = nullptr;
@@ -77,6 +82,7 @@ void *wb_alloc(wb_ctx *c, size_t bytes)
     e = hipMalloc(&p, b);
     if (e != hipSuccess) throw std::string("hipMalloc failed for ") + std::to_string(b) + " bytes";
   }
+  c->held[p] = b;
   return p;
 }
 
@@ -84,6 +90,11 @@ void wb_free(wb_ctx *c, void *p, size_t bytes)
 {
   if (!p) return;
   size_t b = round_size(bytes ? bytes : 1);
+  auto h = c->held.find(p);
+  if (h != c->held.end()) {
+    b = h->second;
+    c->held.erase(h);
+  }
   c->cache.insert({b, p});
   c->cached_bytes += b;
 }
@@ -111,6 +122,9 @@ int wb_ctx_create(int device, wb_ctx **out)
   wb_ctx *c = new wb_ctx();
   c->device = device;
   c->cached_bytes = 0;
+  c->stream = nullptr;
+  c->d_tan = nullptr;
+  c->d_edges = nullptr;
   try {
     WB_CHECK(hipSetDevice(device));
     WB_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -127,6 +141,9 @@ int wb_ctx_create(int device, wb_ctx **out)
     c->kconst = wb::atan_const();
     c->scratch = scratch_address(device);
   } catch (const std::string &m) {
+    if (c->d_tan) hipFree(c->d_tan);
+    if (c->d_edges) hipFree(c->d_edges);
+    if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return WB_EDEVICE;
   }
@@ -165,6 +182,23 @@ int wb_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes)
   *free_bytes = f;
   *total_bytes = t;
   return WB_OK;
+}
+
+int wb_host_alloc(uint64_t bytes, void **out)
+{
+  if (!out || bytes == 0) return WB_EINVAL;
+  *out = nullptr;
+  if (hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return WB_EDEVICE;
+  }
+  return WB_OK;
+}
+
+void wb_host_free(void *p)
+{
+  if (p) hipHostFree(p);
 }
 
 int wb_ctx_sync(wb_ctx *c)

@@ -21,6 +21,7 @@
 
 #include <map>
 #include <set>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -45,8 +46,10 @@ struct wb_ctx {
   // (wb_count_points_in_noalloc)
   void *scratch;
   wb::AtanConst kconst;
-  // size-keyed cache of device buffers
+  // size-keyed cache of device buffers, and the allocated size of every
+  // buffer handed out (a cached buffer up to 1.5x the request is reused)
   std::multimap<size_t, void *> cache;
+  std::unordered_map<void *, size_t> held;
   size_t cached_bytes;
   // trees, LAS files and clouds made on this context and not yet freed
   std::set<struct wb_tree *> trees;
