@@ -99,6 +99,23 @@ void wb_free(wb_ctx *c, void *p, size_t bytes)
   c->cached_bytes += b;
 }
 
+hipEvent_t wb_event_get(wb_ctx *c)
+{
+  if (!c->evpool.empty()) {
+    hipEvent_t e = c->evpool.back();
+    c->evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  WB_CHECK(hipEventCreate(&e));
+  return e;
+}
+
+void wb_event_put(wb_ctx *c, hipEvent_t e)
+{
+  if (e) c->evpool.push_back(e);
+}
+
 static int fail(wb_ctx *c, const std::string &msg, int code)
 {
   if (c) c->err = msg;
@@ -110,6 +127,16 @@ static int fail(wb_ctx *c, const std::string &msg, int code)
   }                                                                \
   catch (const std::string &m) { return fail(ctx, m, WB_EDEVICE); } \
   catch (...) { return fail(ctx, "unexpected exception", WB_EDEVICE); }
+
+// a tree whose build failed: its buffers back to the pool, its events
+// destroyed
+static void discard_tree(wb_ctx *c, wb_tree *t)
+{
+  for (auto &o : t->owned) wb_free(c, o.first, o.second);
+  for (hipEvent_t e : t->evb)
+    if (e) hipEventDestroy(e);
+  delete t;
+}
 
 extern "C" {
 
@@ -125,6 +152,7 @@ int wb_ctx_create(int device, wb_ctx **out)
   c->stream = nullptr;
   c->d_tan = nullptr;
   c->d_edges = nullptr;
+  c->hbox = nullptr;
   try {
     WB_CHECK(hipSetDevice(device));
     WB_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -140,7 +168,10 @@ int wb_ctx_create(int device, wb_ctx **out)
     WB_CHECK(hipMemcpy(c->d_edges, edges, sizeof edges, hipMemcpyHostToDevice));
     c->kconst = wb::atan_const();
     c->scratch = scratch_address(device);
+    WB_CHECK(hipHostMalloc((void **)&c->hbox, 4 * WB_HBOX_WORDS, hipHostMallocDefault));
+    memset(c->hbox, 0, 4 * WB_HBOX_WORDS);
   } catch (const std::string &m) {
+    if (c->hbox) hipHostFree(c->hbox);
     if (c->d_tan) hipFree(c->d_tan);
     if (c->d_edges) hipFree(c->d_edges);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -160,10 +191,21 @@ void wb_ctx_destroy(wb_ctx *c)
   wb_release_las(c);
   for (wb_tree *t : c->trees) {
     for (auto &o : t->owned) hipFree(o.first);
+    for (hipEvent_t e : t->evb)
+      if (e) hipEventDestroy(e);
     delete t;
   }
   c->trees.clear();
+  for (auto &g : c->geo) {
+    hipFree(g.tex);
+    hipFree(g.tey);
+    hipFree(g.grid);
+  }
+  c->geo.clear();
+  for (hipEvent_t e : c->evpool) hipEventDestroy(e);
+  c->evpool.clear();
   for (auto &kv : c->cache) hipFree(kv.second);
+  if (c->hbox) hipHostFree(c->hbox);
   hipFree(c->d_tan);
   hipFree(c->d_edges);
   hipStreamDestroy(c->stream);
@@ -262,12 +304,13 @@ int wb_tree_build(wb_ctx *c, const int32_t *x, const int32_t *y, const int32_t *
     wb_build_tree(t, dx, dy, dz, nullptr, nullptr, nullptr, (flags & WB_BUILD_DEDUPE) != 0);
   } catch (...) {
     if (tmp) wb_free(c, tmp, 12 * n);
-    for (auto &o : t->owned) wb_free(c, o.first, o.second);
-    delete t;
+    discard_tree(c, t);
     throw;
   }
   if (tmp) wb_free(c, tmp, 12 * n);
-  WB_CHECK(hipStreamSynchronize(c->stream));
+  // the call returns while the build's last kernels run (later calls on the
+  // context are ordered after them); host input is copied by then
+  if (tmp) WB_CHECK(hipEventSynchronize(t->evb[0]));
   c->trees.insert(t);
   *out = t;
   return WB_OK;
@@ -317,12 +360,11 @@ int wb_tree_build_xyz(wb_ctx *c, const double *x, const double *y, const double 
     wb_build_tree(t, nullptr, nullptr, nullptr, dx, dy, dz, (flags & WB_BUILD_DEDUPE) != 0);
   } catch (...) {
     if (tmp) wb_free(c, tmp, 24 * n);
-    for (auto &o : t->owned) wb_free(c, o.first, o.second);
-    delete t;
+    discard_tree(c, t);
     throw;
   }
   if (tmp) wb_free(c, tmp, 24 * n);
-  WB_CHECK(hipStreamSynchronize(c->stream));
+  if (tmp) WB_CHECK(hipEventSynchronize(t->evb[0]));
   c->trees.insert(t);
   *out = t;
   return WB_OK;
@@ -349,6 +391,8 @@ int wb_tree_free(wb_tree *t)
   wb_ctx *c = t->ctx;
   hipStreamSynchronize(c->stream);
   for (auto &o : t->owned) wb_free(c, o.first, o.second);
+  for (hipEvent_t e : t->evb)
+    if (e) hipEventDestroy(e);
   c->trees.erase(t);
   delete t;
   return WB_OK;
@@ -558,6 +602,7 @@ static void fill_stats(wb_stats *stats, const wb_tree *t, const ClassifyOut &co,
   stats->maxdepth = t->maxdepth;
   stats->nleaves = t->nleaves;
   stats->nnodes = t->nnodes;
+  wb_tree_read_timing(const_cast<wb_tree *>(t));
   stats->ms_build = t->ms_build;
   stats->ms_sort = t->ms_sort;
   stats->ms_classify_kernel = co.ms_classify_kernel;

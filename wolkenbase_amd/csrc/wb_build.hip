@@ -97,15 +97,28 @@ static void scan_rec(wb_ctx *c, uint32_t *data, uint64_t n)
   wb_free(c, bs, sizeof(uint32_t) * (nb + 1));
 }
 
+// total = the last element before the scan + the last after it, written by
+// the device into the context's pinned words
+__global__ void k_scan_total(const uint32_t *last_in, const uint32_t *last_out, uint32_t *host)
+{
+  if (threadIdx.x == 0) {
+    const uint64_t t = (uint64_t)*last_in + *last_out;
+    host[0] = (uint32_t)t;
+    host[1] = (uint32_t)(t >> 32);
+  }
+}
+
 uint64_t wb_exclusive_scan_u32(wb_ctx *c, uint32_t *data, uint64_t n)
 {
   if (n == 0) return 0;
-  uint32_t last_in = 0, last_out = 0;
-  WB_CHECK(hipMemcpyAsync(&last_in, data + n - 1, 4, hipMemcpyDeviceToHost, c->stream));
+  uint32_t *keep = (uint32_t *)wb_alloc(c, 4);
+  WB_CHECK(hipMemcpyAsync(keep, data + n - 1, 4, hipMemcpyDeviceToDevice, c->stream));
   scan_rec(c, data, n);
-  WB_CHECK(hipMemcpyAsync(&last_out, data + n - 1, 4, hipMemcpyDeviceToHost, c->stream));
+  uint32_t *h = c->hbox + WB_HBOX_SCAN;
+  hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, c->stream, keep, data + n - 1, h);
   WB_CHECK(hipStreamSynchronize(c->stream));
-  return (uint64_t)last_in + last_out;
+  wb_free(c, keep, 4);
+  return (uint64_t)h[0] | ((uint64_t)h[1] << 32);
 }
 
 // ---------------------------------------------------------------------------
@@ -965,8 +978,13 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
 
   t->key = wb_talloc<uint64_t>(t, n);
   t->sidx = wb_talloc<uint32_t>(t, n);
-  hipEvent_t evb[4];
-  for (int k = 0; k < 4; k++) WB_CHECK(hipEventCreate(&evb[k]));
+  // timing events, owned by the tree and read once the stream has passed
+  // them (wb_tree_read_timing): the build returns without waiting for its
+  // last kernels
+  hipEvent_t *evb = t->evb;
+  for (int k = 0; k < 4; k++)
+    if (!evb[k]) WB_CHECK(hipEventCreateWithFlags(&evb[k], hipEventDefault));
+  t->evb_read = false;
   int passes = t->sort_passes > 0 ? t->sort_passes : default_passes(t->side, n_input, dedupe);
   const uint32_t node_cap = (uint32_t)std::min<uint64_t>(n / 64 + 1024, 0x7fffffff);
   t->node_cube = wb_talloc<double>(t, 4 * (size_t)node_cap);
@@ -1190,13 +1208,22 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
       wb_free(c, node_zlow, 8 * (size_t)t->nnodes);
     }
     WB_CHECK(hipEventRecord(evb[3], c->stream));
-    WB_CHECK(hipEventSynchronize(evb[3]));
-    float a = 0, b = 0;
-    hipEventElapsedTime(&a, evb[0], evb[3]);
-    hipEventElapsedTime(&b, evb[1], evb[2]);
-    t->ms_build = a;
-    t->ms_sort = b;
+  } else {
+    t->evb_read = true;
   }
   if (src.packed) wb_free(c, src.packed, pk_bytes);
-  for (int k = 0; k < 4; k++) hipEventDestroy(evb[k]);
+}
+
+void wb_tree_read_timing(wb_tree *t)
+{
+  if (t->evb_read || !t->evb[3]) return;
+  float a = 0, b = 0;
+  if (hipEventElapsedTime(&a, t->evb[0], t->evb[3]) != hipSuccess ||
+      hipEventElapsedTime(&b, t->evb[1], t->evb[2]) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  t->ms_build = a;
+  t->ms_sort = b;
+  t->evb_read = true;
 }

@@ -29,6 +29,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <algorithm>
+
 #include "wb_flowsnake.h"
 #include "wb_internal.h"
 #include "../../include/wb_gpu.h"
@@ -267,6 +269,16 @@ __global__ void k_fill_i32(int32_t *p, uint64_t n, int32_t v)
 {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+__global__ void k_bounds_init(int *b)
+{
+  if (threadIdx.x == 0) {
+    b[0] = INT32_MAX;
+    b[1] = INT32_MIN;
+    b[2] = INT32_MAX;
+    b[3] = INT32_MIN;
+  }
 }
 
 __global__ void k_tile_grid(TileGeo g, const int32_t *tex, const int32_t *tey, int32_t *grid)
@@ -1446,6 +1458,8 @@ enum {
 #ifndef HEAVY_A_GROUPS
 #define HEAVY_A_GROUPS 2048u
 #endif
+// blocks of the heavy kernel at most: 4 waves on each SIMD of 256 CUs
+#define HEAVY_GRID 4096u
 #ifndef HEAVY_B_GROUPS
 #define HEAVY_B_GROUPS 1024u
 #endif
@@ -2083,13 +2097,26 @@ __device__ __forceinline__ void heavy_body(const ClassifyArgs &A, const HeavyRec
 }
 
 
+// The handed-over points, as many as the ground kernel counted (*nheavy,
+// read on the device, so the host does not wait for the count): block b
+// takes record b, then the next record not yet taken (*next counts the
+// records taken beyond the first gridDim.x; zero before the launch).
 template <bool COUNT>
 __global__ void __launch_bounds__(64, WB_HEAVY_MINW) k_ground_heavy(ClassifyArgs A, const HeavyRec *heavy,
-                                                                    uint32_t nheavy)
+                                                                    const uint32_t *nheavy,
+                                                                    uint32_t *next)
 {
   __shared__ HeavyShared SH;
-  if (blockIdx.x >= nheavy) return;
-  heavy_body<COUNT>(A, heavy[blockIdx.x], blockIdx.x, SH);
+  __shared__ uint32_t take;
+  const uint32_t nh = *nheavy;
+  uint32_t r = blockIdx.x;
+  while (r < nh) {
+    heavy_body<COUNT>(A, heavy[r], r, SH);
+    __syncthreads();
+    if (threadIdx.x == 0) take = atomicAdd(next, 1u) + gridDim.x;
+    __syncthreads();
+    r = take;
+  }
 }
 
 // The heavy points of several trees in one launch: blocks hoff[k] ..
@@ -2190,7 +2217,7 @@ __global__ void __launch_bounds__(256) k_sum_hist(const uint32_t *partial, uint3
                                                    unsigned long long *hist3)
 {
   uint64_t s[3] = {0, 0, 0};
-  for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x)
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nblocks; b += gridDim.x * blockDim.x)
     for (int k = 0; k < 3; k++) s[k] += partial[3 * (uint64_t)b + k];
   for (int k = 0; k < 3; k++) {
     uint64_t v = s[k];
@@ -2241,7 +2268,55 @@ struct ClassifyRun {
   HeavyRec *heavy;
   ClassifyArgs A;
   uint32_t h_nheavy, h_nslow;
+  uint32_t *heavy_next;  // records the heavy kernel has taken beyond its grid
 };
+
+// The flowsnake coordinates of tiles lo .. lo+ntiles-1 and the address
+// grid over them (k_tile_geo, k_tile_grid) depend on nothing else, so the
+// context keeps those of its last WB_GEO_CACHE tilings: a pass over a tiling
+// seen before neither recomputes them nor waits for the grid's bounds.
+static const wb_ctx::TileGeoCache &tile_geometry(wb_ctx *c, const TileGeo &g0)
+{
+  for (size_t k = 0; k < c->geo.size(); k++)
+    if (c->geo[k].lo == g0.lo && c->geo[k].ntiles == g0.ntiles) {
+      if (k + 1 != c->geo.size()) std::rotate(c->geo.begin() + k, c->geo.begin() + k + 1, c->geo.end());
+      return c->geo.back();
+    }
+  hipStream_t st = c->stream;
+  wb_ctx::TileGeoCache e;
+  e.lo = g0.lo;
+  e.ntiles = g0.ntiles;
+  e.tex = (int32_t *)wb_alloc(c, 4ull * g0.ntiles);
+  e.tey = (int32_t *)wb_alloc(c, 4ull * g0.ntiles);
+  int *dbounds = (int *)wb_alloc(c, 16);
+  hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, st, dbounds);
+  hipLaunchKernelGGL(k_tile_geo, dim3(wb_blocks(g0.ntiles, 256)), dim3(256), 0, st, g0, e.tex,
+                     e.tey, dbounds);
+  int *hb = (int *)(c->hbox + WB_HBOX_GEO);
+  WB_CHECK(hipMemcpyAsync(hb, dbounds, 16, hipMemcpyDeviceToHost, st));
+  WB_CHECK(hipStreamSynchronize(st));
+  wb_free(c, dbounds, 16);
+  TileGeo g = g0;
+  e.gx0 = g.gx0 = hb[0];
+  e.gy0 = g.gy0 = hb[2];
+  e.gw = g.gw = hb[1] - hb[0] + 1;
+  e.gh = g.gh = hb[3] - hb[2] + 1;
+  const uint64_t gcells = (uint64_t)g.gw * (uint64_t)g.gh;
+  e.grid = (int32_t *)wb_alloc(c, 4 * gcells);
+  hipLaunchKernelGGL(k_fill_i32, dim3(wb_blocks(gcells, 256)), dim3(256), 0, st, e.grid, gcells, -1);
+  hipLaunchKernelGGL(k_tile_grid, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, e.tex, e.tey,
+                     e.grid);
+  if (c->geo.size() >= WB_GEO_CACHE) {
+    // the oldest tiling's arrays go back to the pool (stream-ordered reuse)
+    wb_ctx::TileGeoCache &o = c->geo.front();
+    wb_free(c, o.tex, 4ull * o.ntiles);
+    wb_free(c, o.tey, 4ull * o.ntiles);
+    wb_free(c, o.grid, 4 * (uint64_t)o.gw * (uint64_t)o.gh);
+    c->geo.erase(c->geo.begin());
+  }
+  c->geo.push_back(e);
+  return c->geo.back();
+}
 
 static bool classify_count_work()
 {
@@ -2263,11 +2338,12 @@ static void classify_prepare(ClassifyRun &R, wb_tree *t, wb_ctx *c, const double
   g.scx = ss.cx; g.scy = ss.cy; g.spacing = ss.spacing; g.r = ss.radius;
   g.lo = ss.lo; g.hi = ss.hi;
   g.ntiles = (uint32_t)((int64_t)ss.hi - ss.lo + 1);
-  for (int k = 0; k < 8; k++) WB_CHECK(hipEventCreate(&R.ev[k]));
+  for (int k = 0; k < 8; k++) R.ev[k] = wb_event_get(c);
   hipEvent_t *ev = R.ev;
   WB_CHECK(hipEventRecord(ev[0], st));
-  int32_t *tex = R.tex = (int32_t *)wb_alloc(c, 4ull * g.ntiles);
-  int32_t *tey = R.tey = (int32_t *)wb_alloc(c, 4ull * g.ntiles);
+  const wb_ctx::TileGeoCache &geo = tile_geometry(c, g);
+  int32_t *tex = R.tex = geo.tex;
+  int32_t *tey = R.tey = geo.tey;
   uint32_t *tile_cnt = R.tile_cnt = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
   uint32_t *tile_fill = R.tile_fill = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
   TileRec *rec = R.rec = (TileRec *)wb_alloc(c, sizeof(TileRec) * (size_t)g.ntiles);
@@ -2279,26 +2355,16 @@ static void classify_prepare(ClassifyRun &R, wb_tree *t, wb_ctx *c, const double
   uint32_t *err = R.err = (uint32_t *)wb_alloc(c, 128);
   unsigned long long *ctr = R.ctr = (unsigned long long *)err + 1;
   uint32_t *nslow = R.nslow = (uint32_t *)(ctr + 6);
+  R.heavy_next = err + 18;
   WB_CHECK(hipMemsetAsync(err, 0, 128, st));
   WB_CHECK(hipMemsetAsync(tile_cnt, 0, 4ull * g.ntiles, st));
   WB_CHECK(hipMemsetAsync(tile_fill, 0, 4ull * g.ntiles, st));
   TreeView T = view(t);
-  int *dbounds = (int *)wb_alloc(c, 16);
-  int hb[4] = {INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN};
-  WB_CHECK(hipMemcpyAsync(dbounds, hb, 16, hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_tile_geo, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey, dbounds);
-  WB_CHECK(hipMemcpyAsync(hb, dbounds, 16, hipMemcpyDeviceToHost, st));
-  WB_CHECK(hipStreamSynchronize(st));
-  wb_free(c, dbounds, 16);
-  g.gx0 = hb[0];
-  g.gy0 = hb[2];
-  g.gw = hb[1] - hb[0] + 1;
-  g.gh = hb[3] - hb[2] + 1;
-  const uint64_t gcells = R.gcells = (uint64_t)g.gw * (uint64_t)g.gh;
-  int32_t *grid = R.grid = (int32_t *)wb_alloc(c, 4 * gcells);
-  hipLaunchKernelGGL(k_fill_i32, dim3(wb_blocks(gcells, 256)), dim3(256), 0, st, grid, gcells, -1);
-  g.grid = grid;
-  hipLaunchKernelGGL(k_tile_grid, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey, grid);
+  g.gx0 = geo.gx0;
+  g.gy0 = geo.gy0;
+  g.gw = geo.gw;
+  g.gh = geo.gh;
+  g.grid = R.grid = geo.grid;
   LeafWin *lwin = (LeafWin *)wb_alloc(c, sizeof(LeafWin) * ((size_t)t->nleaves + 1));
   uint8_t *lflags = (uint8_t *)wb_alloc(c, (size_t)LEAF_WIN * t->nleaves + 16);
   if (t->nleaves)
@@ -2384,16 +2450,17 @@ static void classify_ground_single(ClassifyRun &R)
                          0, st, A);
   }
   WB_CHECK(hipEventRecord(R.ev[5], st));
-  WB_CHECK(hipMemcpyAsync(&R.h_nheavy, R.nslow + 1, 4, hipMemcpyDeviceToHost, st));
-  WB_CHECK(hipStreamSynchronize(st));
   WB_CHECK(hipEventRecord(R.ev[6], st));
-  if (R.h_nheavy) {
+  if (R.n > 0) {
+    // at most one block per handed-over point, and no more than fit on the
+    // device at once; the count is read on the device
+    const unsigned nb = (unsigned)std::min<uint64_t>(R.n, HEAVY_GRID);
     if (count_work)
-      hipLaunchKernelGGL(k_ground_heavy<true>, dim3(R.h_nheavy), dim3(64), 0, st, A, R.heavy,
-                         R.h_nheavy);
+      hipLaunchKernelGGL(k_ground_heavy<true>, dim3(nb), dim3(64), 0, st, A, R.heavy, A.nheavy,
+                         R.heavy_next);
     else
-      hipLaunchKernelGGL(k_ground_heavy<false>, dim3(R.h_nheavy), dim3(64), 0, st, A, R.heavy,
-                         R.h_nheavy);
+      hipLaunchKernelGGL(k_ground_heavy<false>, dim3(nb), dim3(64), 0, st, A, R.heavy, A.nheavy,
+                         R.heavy_next);
   }
   WB_CHECK(hipEventRecord(R.ev[7], st));
 }
@@ -2416,14 +2483,14 @@ static void classify_finish(ClassifyRun &R)
   const ClassifyArgs &A = R.A;
   int32_t *nground = (int32_t *)R.tile_fill;  // reused once the lists are built
   if (!scan_only) {
-    WB_CHECK(hipMemcpyAsync(&R.h_nslow, nslow, 4, hipMemcpyDeviceToHost, st));
     WB_CHECK(hipMemsetAsync(nground, 0, 4ull * g.ntiles, st));
     if (n > 0) {
       const unsigned nbk = wb_blocks(n, 256);
       uint32_t *partial = (uint32_t *)wb_alloc(c, 12ull * nbk);
       hipLaunchKernelGGL(k_count_ground, dim3(nbk), dim3(256), 0, st, n, cls, memb, nmemb, rec,
                          g.lo, nground, partial, want_tiles ? 1 : 0);
-      hipLaunchKernelGGL(k_sum_hist, dim3(1), dim3(256), 0, st, partial, nbk, ctr + 2);
+      hipLaunchKernelGGL(k_sum_hist, dim3(std::min(wb_blocks(nbk, 256), 64u)), dim3(256), 0, st,
+                         partial, nbk, ctr + 2);
       wb_free(c, partial, 12ull * nbk);
       if (t->ndup)
         hipLaunchKernelGGL(k_dup_class, dim3(wb_blocks(t->ndup, 256)), dim3(256), 0, st, t->ndup,
@@ -2462,11 +2529,9 @@ static void classify_finish(ClassifyRun &R)
         B.counters = nullptr;
         hipLaunchKernelGGL((k_ground<false, WB_GROUND_MINW>), dim3(wb_blocks(nitems, 256)),
                            dim3(256), 0, st, B);
-        uint32_t nh = 0;
-        WB_CHECK(hipMemcpyAsync(&nh, B.nheavy, 4, hipMemcpyDeviceToHost, st));
-        WB_CHECK(hipStreamSynchronize(st));
-        if (nh)
-          hipLaunchKernelGGL(k_ground_heavy<false>, dim3(nh), dim3(64), 0, st, B, iheavy, nh);
+        hipLaunchKernelGGL(k_ground_heavy<false>,
+                           dim3((unsigned)std::min<uint64_t>(nitems, HEAVY_GRID)), dim3(64), 0, st,
+                           B, iheavy, B.nheavy, (uint32_t *)ict + 4);
         hipLaunchKernelGGL(k_item_ground, dim3(wb_blocks(nitems, 256)), dim3(256), 0, st, nitems,
                            items, icls, nground);
         WB_CHECK(hipStreamSynchronize(st));
@@ -2479,8 +2544,8 @@ static void classify_finish(ClassifyRun &R)
     }
   }
   WB_CHECK(hipEventRecord(ev[3], st));
-  uint32_t h_err[32];
-  WB_CHECK(hipMemcpyAsync(h_err, err, 128, hipMemcpyDeviceToHost, st));
+  uint32_t *herr = c->hbox + WB_HBOX_ERR;
+  WB_CHECK(hipMemcpyAsync(herr, err, 128, hipMemcpyDeviceToHost, st));
   if (want_tiles) {
     out->tiles.resize(g.ntiles);
     out->tex.resize(g.ntiles);
@@ -2495,6 +2560,11 @@ static void classify_finish(ClassifyRun &R)
     for (uint32_t k = 0; k < g.ntiles; k++) out->tiles[k].nground = ng[k];
   }
   WB_CHECK(hipStreamSynchronize(st));
+  uint32_t h_err[32];
+  std::copy(herr, herr + 32, h_err);
+  // the exact-gap and heavy point counts (nslow[0], nslow[1])
+  R.h_nslow = h_err[14];
+  R.h_nheavy = h_err[15];
   float ms1 = 0, ms2 = 0, ms3 = 0;
   hipEventElapsedTime(&ms1, ev[0], ev[1]);
   hipEventElapsedTime(&ms2, ev[1], ev[2]);
@@ -2505,7 +2575,7 @@ static void classify_finish(ClassifyRun &R)
   float ms5 = 0;
   hipEventElapsedTime(&ms5, ev[6], ev[7]);
   out->ms_heavy_kernel = ms5;
-  for (int k = 0; k < 8; k++) hipEventDestroy(ev[k]);
+  for (int k = 0; k < 8; k++) wb_event_put(c, ev[k]);
   unsigned long long *hc = (unsigned long long *)(h_err + 2);
   out->borderline = hc[0];
   out->atan_exact = hc[1];
@@ -2553,9 +2623,6 @@ static void classify_finish(ClassifyRun &R)
     wb_free(c, dcount, CT_STRIDE * CT_COPIES * 8);
   }
   out->ms_tiles = ms1; out->ms_scan = ms2; out->ms_classify = ms3;
-  wb_free(c, R.grid, 4 * R.gcells);
-  wb_free(c, R.tex, 4ull * g.ntiles);
-  wb_free(c, R.tey, 4ull * g.ntiles);
   wb_free(c, R.tile_cnt, 4ull * g.ntiles + 4);
   wb_free(c, R.tile_fill, 4ull * g.ntiles + 4);
   wb_free(c, rec, sizeof(TileRec) * (size_t)g.ntiles);

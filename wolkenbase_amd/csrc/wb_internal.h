@@ -32,6 +32,13 @@
 #define WB_RECORDS 537  // octree.h:40-41: a leaf holds at most 537 locations
 #define WB_MAX_LEVEL 21
 #define WB_CHUNK 32
+#define WB_HBOX_WORDS 1024
+// word offsets of the pinned host words of each use
+#define WB_HBOX_SCAN 0     // wb_exclusive_scan_u32: total (2 words)
+#define WB_HBOX_GEO 16     // tile grid bounds (4 words)
+#define WB_HBOX_ERR 32     // classification flags and counters (32 words)
+#define WB_HBOX_BATCH 64   // batch classification: heavy counts per tree
+#define WB_GEO_CACHE 4
 
 struct wb_stream_alloc;
 
@@ -51,6 +58,21 @@ struct wb_ctx {
   std::multimap<size_t, void *> cache;
   std::unordered_map<void *, size_t> held;
   size_t cached_bytes;
+  // pinned host words the kernels and copies of one call fill for the host
+  // (no staging copy, no implicit wait): WB_HBOX_WORDS words
+  uint32_t *hbox;
+  // tile geometry of recent tilings (flowsnake coordinates of every tile
+  // and the address grid over them), a pure function of the first counter
+  // and the number of tiles; most recent last
+  struct TileGeoCache {
+    int lo;
+    uint32_t ntiles;
+    int32_t *tex, *tey, *grid;
+    int gx0, gy0, gw, gh;
+  };
+  std::vector<TileGeoCache> geo;
+  // timing events kept for reuse (creating one costs host time per call)
+  std::vector<hipEvent_t> evpool;
   // trees, LAS files and clouds made on this context and not yet freed
   std::set<struct wb_tree *> trees;
   std::set<struct wb_las *> las;
@@ -75,6 +97,10 @@ struct wb_tree {
   int sort_passes, maxdepth;
   uint32_t nleaves, nnodes;
   float ms_build, ms_sort;  // device time of the last build (events)
+  // the build's timing events (start, sort start, sort end, end), read
+  // once the stream has passed them; null when there are none
+  hipEvent_t evb[4];
+  bool evb_read;
   // device arrays
   uint64_t *key;
   uint32_t *sidx;
@@ -101,6 +127,11 @@ struct wb_tree {
 
 // device allocation through the context cache
 void *wb_alloc(wb_ctx *c, size_t bytes);
+// a timing event from the context's pool, and its return
+hipEvent_t wb_event_get(wb_ctx *c);
+void wb_event_put(wb_ctx *c, hipEvent_t e);
+// the build's device times, once the stream has passed its last event
+void wb_tree_read_timing(wb_tree *t);
 void wb_free(wb_ctx *c, void *p, size_t bytes);
 template <typename T>
 T *wb_talloc(wb_tree *t, size_t count)
