@@ -268,6 +268,8 @@ def main_las(args):
         "store_outside_timed": store,
         "classes": {str(k): v for k, v in st["classes"].items()},
     }
+    if args.workload == "tile20m" and not args.no_serving:
+        out["serving"] = serving_summary(args.serving_requests)
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline on a {args.cpu_window:g} m window ...")
         out["cpu_baseline"] = cpu_baseline_las(cloud, args.cpu_window)
@@ -336,6 +338,31 @@ def run_tile(name, steps, warmup, ctx, pmc_calibrate=False):
                 value=n * steps / elapsed)
 
 
+def serving_summary(nreq):
+    """A short run of the serving loop on the serving stream's shape (48
+    lognormal strata of tile size, median 2M points; Poisson arrivals at 0.7
+    of the measured capacity; 4 workers): served points/s, p50/p99 latency
+    and the comparison with a 20M-pulse tile of the same generator alone.
+    The full run (128 strata, 2000 requests) is `python -m
+    wolkenbase_amd.serve`.  A failure is reported, not raised."""
+    try:
+        from wolkenbase_amd import serve, synth
+
+        t0 = time.perf_counter()
+        pool = serve.pin_tiles(serve.stream_pool(48, 3000, 4e7))
+        ref = serve.pin_tiles([synth.generate(**serve.REF_TILE)])[0]
+        log(f"[bench] serving pool generated in {time.perf_counter() - t0:.1f} s; "
+            f"{nreq} requests ...")
+        r = serve.load_test(pool, slots=4, load=0.7, nreq=nreq, seed=3000, ref_tile=ref)
+        keep = ("requests", "pool_tiles", "pool_points", "single_tile", "capacity_points_per_s",
+                "served_points_per_s", "served_over_single_tile", "p50_ms", "p99_ms",
+                "p99_over_single_tile_ms", "p50_over_isolated", "p99_over_isolated",
+                "mismatched_results")
+        return {k: r[k] for k in keep}
+    except Exception as e:  # reported in the line, the headline stands
+        return {"error": repr(e)}
+
+
 def kernel_times(stats):
     def avg(f):
         return sum(s[f] for s in stats) / len(stats)
@@ -369,6 +396,9 @@ def main():
     ap.add_argument("--store", action="store_true",
                     help="las20m: also time the store's blocks into pinned host memory")
     ap.add_argument("--cpu-window", type=float, default=60.0)
+    ap.add_argument("--no-serving", action="store_true",
+                    help="tile20m: skip the short serving-stream measurement")
+    ap.add_argument("--serving-requests", type=int, default=400)
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="run one 256 MiB device copy first, so a --pmc profile of this "
                          "command can scale FETCH_SIZE/WRITE_SIZE against known bytes")
@@ -476,6 +506,8 @@ def main():
             "exact_paths": {"gap_points": int(mst["exact_gap_points"]),
                             "heavy_points": int(mst["heavy_points"])},
         }
+    if args.workload == "tile20m" and not args.no_serving:
+        out["serving"] = serving_summary(args.serving_requests)
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline on a {args.cpu_window:g} m window ...")
         out["cpu_baseline"] = cpu_baseline(cloud, args.cpu_window)
