@@ -1,6 +1,7 @@
 """The device math (wb_math.h, wb_flowsnake.h, wb_host.cpp) built for the
 host, pinned against glibc and the oracle without a GPU."""
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -463,3 +464,45 @@ def test_border_wedge_pruning_is_conservative(hm):
                 for qx, qy in ((x0, y0), (x0, y1), (x1, y0), (x1, y1), (cx, cy)):
                     assert hm.wbh_dir_bin(qy, qx) not in care, (hex(occ), e, off)
     assert len(masks) > 200 and rejected > 500
+
+
+def test_membership_candidates_cover_every_tile_within_reach():
+    """k_membership takes as candidates the lattice point nearest to a point
+    (cube rounding of its Eisenstein coordinates) and that point's six
+    neighbours.  Every tile centre within the candidate bound (spacing*41/71
+    with its margin) must be among them: checked against all lattice points
+    within two steps, on random points and on points placed at hexagon
+    vertices (three-way ties of the rounding) within 1e-12 relative."""
+    rng = np.random.default_rng(71)
+    s = 1.518
+    lim = s * 41 / 71 * (1 + 1e-9) + 1e-9
+    k34 = math.sqrt(3) / 2
+
+    def centre(a, b):
+        return s * (a - b / 2), s * (b * k34)
+
+    pts = [tuple(p) for p in rng.uniform(-60, 60, (20000, 2))]
+    for _ in range(4000):
+        a, b = (int(v) for v in rng.integers(-30, 30, 2))
+        cx, cy = centre(a, b)
+        ang = int(rng.integers(0, 12)) * math.pi / 6
+        rr = s / math.sqrt(3) * (1 + rng.normal() * 1e-12)
+        pts.append((cx + rr * math.cos(ang), cy + rr * math.sin(ang)))
+    for x, y in pts:
+        fb = (y / s) / k34
+        fa = x / s + fb / 2
+        rx, ry, rz = np.rint(fa - fb), np.rint(fb), np.rint(-fa)
+        ddx, ddy, ddz = abs(rx - (fa - fb)), abs(ry - fb), abs(rz + fa)
+        if ddy > ddx and ddy > ddz:
+            ry = -rx - rz
+        elif ddz >= ddx and ddz >= ddy:
+            rz = -rx - ry
+        X, Y = int(-rz), int(ry)
+        seven = {(X + da, Y + db) for da, db in
+                 [(0, 0), (1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (-1, -1)]}
+        b0, a0 = round(fb), round(fa)
+        for b in range(b0 - 3, b0 + 4):
+            for a in range(a0 - 3, a0 + 4):
+                cx, cy = centre(a, b)
+                if (cx - x) ** 2 + (cy - y) ** 2 <= lim * lim:
+                    assert (a, b) in seven, (x, y, a, b)

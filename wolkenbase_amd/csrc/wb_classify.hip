@@ -404,17 +404,28 @@ __global__ void k_membership(TreeView T, TileGeo g, const LeafWin *win, const ui
   const uint32_t leaf = T.pt_leaf[i];
   const double *lc = T.leaf_cube + 4 * leaf;
   const LeafWin wl = win[leaf];
-  double u = (x - g.scx) / g.spacing, v = (y - g.scy) / g.spacing;
-  double fy = v / kSqrt34;
-  int Y = (int)__builtin_rint(fy);
-  int X = (int)__builtin_rint(u + fy / 2);
+  // The lattice point nearest to p (Eisenstein coordinates a + b*omega,
+  // centre spacing*(a - b/2, b*sqrt3/2)) by rounding the cube coordinates
+  // (a - b, b, -a) and fixing the one rounded farthest.  A centre within the
+  // candidate bound (spacing*41/71, a hair over the hexagon's circumradius
+  // spacing/sqrt3) of p lies within 1.16 spacings of that point, so it is the
+  // point itself or one of its six neighbours (the next ring is sqrt3
+  // spacings away); a rounding near a tie that picks a neighbour of the
+  // nearest point still leaves every such centre in the seven.
+  const double u = (x - g.scx) / g.spacing, v = (y - g.scy) / g.spacing;
+  const double fb = v / kSqrt34, fa = u + fb / 2;
+  double rx = __builtin_rint(fa - fb), ry = __builtin_rint(fb), rz = __builtin_rint(-fa);
+  const double ddx = fabs(rx - (fa - fb)), ddy = fabs(ry - fb), ddz = fabs(rz + fa);
+  if (ddy > ddx && ddy > ddz) ry = -rx - rz;
+  else if (ddz >= ddx && ddz >= ddy) rz = -rx - ry;
+  const int X = (int)-rz, Y = (int)ry;
   const double lim = g.r * (1 + 1e-9) + 1e-9;
   int nm = 0;
   int32_t mem[MAX_MEMB];
-  for (int b = -2; b <= 2; b++)
-    for (int a = -2; a <= 2; a++) {
-      if (a - b > 2 || b - a > 2) continue;
-      int ex = X + a, ey = Y + b;
+  constexpr int na[7] = {0, 1, -1, 0, 0, 1, -1}, nb[7] = {0, 0, 0, 1, -1, 1, -1};
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+      int ex = X + na[k], ey = Y + nb[k];
       double cx, cy;
       tile_center(g, ex, ey, &cx, &cy);
       double dx = cx - x, dy = cy - y;
