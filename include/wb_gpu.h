@@ -125,6 +125,13 @@ void wb_ctx_destroy(wb_ctx *ctx);
 const char *wb_last_error(wb_ctx *ctx);
 /* synchronise the context's stream */
 int wb_ctx_sync(wb_ctx *ctx);
+/* on != 0: the context's long-running kernels (the ground test and its
+ * wave-per-point pass) run on resident grids that leave room on every CU,
+ * so kernels of other contexts that run meanwhile are not held until they
+ * end (a serving loop with one context per worker); off (the default): the
+ * full device.  Results are the same either way.  No counterpart in the
+ * reference. */
+int wb_ctx_set_sharing(wb_ctx *ctx, int on);
 /* free and total device memory of a device (admission control of a serving
  * loop; no counterpart in the reference, whose store lives on disk) */
 int wb_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);

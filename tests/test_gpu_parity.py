@@ -157,6 +157,47 @@ def test_classify_matches_oracle(ctx, cfg):
         assert ((m > 64) & (m <= 128)).any() and ((m > 128) & (m <= 256)).any() and (m > 256).any()
 
 
+def test_classify_sharing_context_matches_oracle():
+    """A context in sharing mode (the resident ground kernel, capped grids)
+    gives the oracle's class bytes and tile records."""
+    cloud = synth.generate(seed=22, extent=30.0, density=20.0, embankments=2, outliers=0.02)
+    sctx = W.Context(0, sharing=True)
+    try:
+        tree, _, _ = build_both(sctx, cloud)
+        cls, tiles, _ = W.classify(tree, cloud["bbmin"], cloud["bbmax"])
+        rcls, rtiles, _ = O.run(cloud)
+        for f in ("npoints", "nground", "tree_flags", "hs"):
+            assert np.array_equal(tiles[f], rtiles[f]), f
+        assert int((cls != rcls).sum()) == 0
+    finally:
+        sctx.close()
+
+
+def test_classify_sharing_context_equals_default(ctx):
+    """On a cloud with more 64-point units (6.9k) than the capped resident
+    grid has waves (4 blocks per CU), so waves go on to take units from the
+    counter, sharing mode gives the default mode's class bytes, tile records
+    and handed-over point counts."""
+    cloud = synth.generate(seed=28, extent=100.0, density=40.0, buildings=60, trees=500,
+                           outliers=0.01)
+    assert len(cloud["xi"]) > 4 * 256 * 4 * 64
+    center, side = W.Octree.size_fit(cloud["bbmin"], cloud["bbmax"])
+    res = []
+    for c in (ctx, W.Context(0, sharing=True)):
+        tree = W.Octree.build(c, cloud["xi"], cloud["yi"], cloud["zi"], cloud["scale"],
+                              cloud["offset"], center, side)
+        res.append(W.classify(tree, cloud["bbmin"], cloud["bbmax"]))
+        tree.free()
+        if c is not ctx:
+            c.close()
+    (c0, t0, s0), (c1, t1, s1) = res
+    assert np.array_equal(c0, c1)
+    for f in t0.dtype.names:
+        assert np.array_equal(t0[f], t1[f]), f
+    assert s0["heavy_points"] == s1["heavy_points"] > 0
+    assert s0["exact_gap_points"] == s1["exact_gap_points"]
+
+
 # classification parameters other than the defaults (the hyperboloid's
 # vertex drop and slope, the tile spacing, the floor of the hyperboloid size)
 PARAMS = [dict(thickness=0.05), dict(max_slope=0.7), dict(max_slope=1.5), dict(tile_size=2.0),

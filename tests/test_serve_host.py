@@ -17,7 +17,7 @@ from wolkenbase_amd import serve
 class _Ctx:
     opened = 0
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, sharing=False):
         _Ctx.opened += 1
 
     def close(self):
@@ -41,7 +41,7 @@ def fake_device(monkeypatch):
 
 
 def _fake_classify(calls):
-    def classify_tile(ctx, tile, params):
+    def classify_tile(ctx, tile, params, times=None):
         calls.append(("one", len(tile["xi"])))
         if tile.get("bad"):
             raise ValueError("bad tile")
@@ -60,7 +60,7 @@ def test_context_failure_raises_in_constructor(monkeypatch, fake_device):
     made = []
 
     class Bad(_Ctx):
-        def __init__(self, device=0):
+        def __init__(self, device=0, sharing=False):
             if len(made) == 2:
                 raise RuntimeError("device lost")
             made.append(1)
@@ -77,7 +77,7 @@ def test_dead_worker_fails_pending_and_later_requests(monkeypatch, fake_device):
     class Fatal(BaseException):
         pass
 
-    def classify_tile(ctx, tile, params):
+    def classify_tile(ctx, tile, params, times=None):
         gate.wait(5)
         raise Fatal("worker killed")
 
@@ -118,12 +118,12 @@ def _hold_worker(monkeypatch, calls):
     gate = threading.Event()
     one, batch = _fake_classify(calls)
 
-    def one_gated(ctx, tile, params):
+    def one_gated(ctx, tile, params, times=None):
         if len(calls) == 0:
             calls.append(("hold", len(tile["xi"])))
             gate.wait(5)
             return np.ones(len(tile["xi"]), np.uint8)
-        return one(ctx, tile, params)
+        return one(ctx, tile, params, times)
 
     monkeypatch.setattr(serve, "classify_tile", one_gated)
     monkeypatch.setattr(serve, "classify_tiles", batch)

@@ -130,6 +130,7 @@ SIGNATURES = {
     "wb_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
 
     "wb_ctx_destroy": (None, [_vp]),
+    "wb_ctx_set_sharing": (C.c_int, [_vp, C.c_int]),
     "wb_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(_vp)]),
     "wb_host_free": (None, [_vp]),
     "wb_device_memory": (C.c_int, [C.c_int, _u64p, _u64p]),
@@ -260,14 +261,20 @@ def pinned_empty(n, dtype):
 
 
 class Context:
-    """wb_ctx: one HIP device + stream + device memory pool."""
+    """wb_ctx: one HIP device + stream + device memory pool.  sharing: the
+    long-running kernels leave room on every CU for other contexts' kernels
+    (wb_ctx_set_sharing); results are the same."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, sharing=False):
         h = _vp()
         rc = lib().wb_ctx_create(device, C.byref(h))
         if rc != 0:
             raise WbError(f"wb_ctx_create({device}) failed with {rc}")
         self.h = h
+        if sharing and lib().wb_ctx_set_sharing(h, 1) != 0:
+            lib().wb_ctx_destroy(h)
+            self.h = None
+            raise WbError("wb_ctx_set_sharing failed")
         self.trees = weakref.WeakSet()
         self.las = weakref.WeakSet()  # LasFile and Cloud objects
 

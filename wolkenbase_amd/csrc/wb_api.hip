@@ -169,6 +169,8 @@ int wb_ctx_create(int device, wb_ctx **out)
     c->kconst = wb::atan_const();
     c->scratch = scratch_address(device);
     WB_CHECK(hipHostMalloc((void **)&c->hbox, 4 * WB_HBOX_WORDS, hipHostMallocDefault));
+    WB_CHECK(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
+    c->sharing = false;
     memset(c->hbox, 0, 4 * WB_HBOX_WORDS);
   } catch (const std::string &m) {
     if (c->hbox) hipHostFree(c->hbox);
@@ -241,6 +243,13 @@ int wb_host_alloc(uint64_t bytes, void **out)
 void wb_host_free(void *p)
 {
   if (p) hipHostFree(p);
+}
+
+int wb_ctx_set_sharing(wb_ctx *c, int on)
+{
+  if (!c) return WB_EINVAL;
+  c->sharing = on != 0;
+  return WB_OK;
 }
 
 int wb_ctx_sync(wb_ctx *c)

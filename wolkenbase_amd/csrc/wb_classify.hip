@@ -1488,21 +1488,27 @@ enum {
 // (shape.cpp:119-125) with its axis at x, y instead of x + 0, y + 0: the
 // two differ only in the sign of a zero, which no test below can see (every
 // use is a difference that is squared, compared or passed to hypot).
-// blk: the block's index among the blocks of its tree; stacks, LW, edges:
-// the block's shared arrays (edges: the bin-edge table, read by wedge
-// refreshes)
+// unit: the index of the wave's 64 work items (items 64*unit ..); stacks,
+// LW, edges: the block's shared arrays (edges: the bin-edge table, read by
+// wedge refreshes, loaded by ground_setup).  No barrier: the waves of a
+// block run their units independently.
+__device__ __forceinline__ void ground_setup(const ClassifyArgs &A, float *edges)
+{
+  for (int k = threadIdx.x; k < 3 * 128; k += blockDim.x) edges[k] = A.edges[k];
+  __syncthreads();
+}
+
 template <bool COUNT>
-__device__ __forceinline__ void ground_body(const ClassifyArgs &A, uint32_t blk, WaveStack *stacks,
-                                            LaneWedges &LW, float *edges)
+__device__ __forceinline__ void ground_body(const ClassifyArgs &A, uint64_t unit,
+                                            WaveStack *stacks, LaneWedges &LW, const float *edges)
 {
   WaveStack &S = stacks[threadIdx.x / WAVE];
   LW.n[threadIdx.x] = -1;
-  for (int k = threadIdx.x; k < 3 * 128; k += 256) edges[k] = A.edges[k];
-  __syncthreads();
+  const uint32_t blk = (uint32_t)unit;  // spreads the instrumented counters
   const TreeView &T = A.T;
   const uint32_t lane = __lane_id();
   // work index: a sorted slot, or a work item
-  const uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x;
+  const uint64_t i = unit * WAVE + lane;
   const bool inrange = i < (A.items ? A.n_items : T.n);
   const double *ttan = A.tab, *tcos = A.tab + 511, *tsin = A.tab + 1023;
   const uint64_t t_start = COUNT ? wall_clock64() : 0;
@@ -1773,7 +1779,34 @@ __global__ void __launch_bounds__(256, MINW) k_ground(ClassifyArgs A)
   __shared__ WaveStack stacks[256 / WAVE];
   __shared__ LaneWedges LW;
   __shared__ float edges[3 * 128];
-  ground_body<COUNT>(A, blockIdx.x, stacks, LW, edges);
+  ground_setup(A, edges);
+  ground_body<COUNT>(A, (uint64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE, stacks, LW,
+                     edges);
+}
+
+// The same test on a grid that stays resident: each wave takes its first
+// unit of 64 items by its place in the grid, then the next unit no wave has
+// taken (*next counts the units taken beyond the grid's; zero before the
+// launch), until none is left.  Launched with fewer blocks than fit on the
+// device, it leaves room on every CU for the kernels of other streams,
+// which a kernel of many short blocks does not: each freed slot goes to its
+// next block.
+template <int MINW>
+__global__ void __launch_bounds__(256, MINW) k_ground_resident(ClassifyArgs A, uint32_t *next)
+{
+  __shared__ WaveStack stacks[256 / WAVE];
+  __shared__ LaneWedges LW;
+  __shared__ float edges[3 * 128];
+  ground_setup(A, edges);
+  const uint64_t nunits = ((A.items ? A.n_items : A.T.n) + WAVE - 1) / WAVE;
+  const uint32_t waves = gridDim.x * (256 / WAVE);
+  uint64_t unit = (uint64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;
+  while (unit < nunits) {
+    ground_body<false>(A, unit, stacks, LW, edges);
+    uint32_t u = 0;
+    if (__lane_id() == 0) u = atomicAdd(next, 1u);
+    unit = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)u) + waves;
+  }
 }
 
 // The ground tests of several trees in one launch: blocks boff[k] ..
@@ -1791,7 +1824,9 @@ __global__ void __launch_bounds__(256, MINW) k_ground_batch(const ClassifyArgs *
     if (boff[mid] <= blockIdx.x) lo = mid;
     else hi = mid;
   }
-  ground_body<false>(AA[lo], blockIdx.x - boff[lo], stacks, LW, edges);
+  ground_setup(AA[lo], edges);
+  ground_body<false>(AA[lo], (uint64_t)(blockIdx.x - boff[lo]) * (256 / WAVE) + threadIdx.x / WAVE,
+                     stacks, LW, edges);
 }
 
 // One heavy point per 64-lane block.  The shape is the same in every lane,
@@ -2453,19 +2488,26 @@ static void classify_ground_single(ClassifyRun &R)
   const bool count_work = R.dcount != nullptr;
   WB_CHECK(hipEventRecord(R.ev[4], st));
   if (R.n > 0) {
+    const uint32_t nb = wb_blocks(R.n, 256);
     if (count_work)
-      hipLaunchKernelGGL((k_ground<true, WB_GROUND_MINW>), dim3(wb_blocks(R.n, 256)), dim3(256),
-                         0, st, A);
+      hipLaunchKernelGGL((k_ground<true, WB_GROUND_MINW>), dim3(nb), dim3(256), 0, st, A);
+    else if (R.c->sharing)
+      // 4 of the 6 blocks a CU holds: a wave slot with 192 VGPRs and 2
+      // blocks' LDS stay free on every SIMD for other contexts' kernels
+      hipLaunchKernelGGL((k_ground_resident<WB_GROUND_MINW>),
+                         dim3(std::min<uint32_t>(nb, 4u * (uint32_t)R.c->ncu)), dim3(256), 0, st,
+                         A, R.err + 19);
     else
-      hipLaunchKernelGGL((k_ground<false, WB_GROUND_MINW>), dim3(wb_blocks(R.n, 256)), dim3(256),
-                         0, st, A);
+      hipLaunchKernelGGL((k_ground<false, WB_GROUND_MINW>), dim3(nb), dim3(256), 0, st, A);
   }
   WB_CHECK(hipEventRecord(R.ev[5], st));
   WB_CHECK(hipEventRecord(R.ev[6], st));
   if (R.n > 0) {
     // at most one block per handed-over point, and no more than fit on the
     // device at once; the count is read on the device
-    const unsigned nb = (unsigned)std::min<uint64_t>(R.n, HEAVY_GRID);
+    // sharing: 8 blocks per CU, about half the LDS the kernel could hold
+    const unsigned nb = (unsigned)std::min<uint64_t>(
+        R.n, R.c->sharing ? 8ull * (uint64_t)R.c->ncu : (uint64_t)HEAVY_GRID);
     if (count_work)
       hipLaunchKernelGGL(k_ground_heavy<true>, dim3(nb), dim3(64), 0, st, A, R.heavy, A.nheavy,
                          R.heavy_next);

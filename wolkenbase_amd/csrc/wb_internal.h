@@ -58,6 +58,10 @@ struct wb_ctx {
   std::multimap<size_t, void *> cache;
   std::unordered_map<void *, size_t> held;
   size_t cached_bytes;
+  // compute units of the device, and whether the context's long kernels
+  // leave room on every CU for other contexts' (wb_ctx_set_sharing)
+  int ncu;
+  bool sharing;
   // pinned host words the kernels and copies of one call fill for the host
   // (no staging copy, no implicit wait): WB_HBOX_WORDS words
   uint32_t *hbox;

@@ -65,6 +65,7 @@ class Request:
         self.done = threading.Event()
         self.t_arrive = self.t_start = self.t_end = None
         self.batch = 1
+        self.times = {}
         self.cls = None
         self.error = None
 
@@ -73,10 +74,12 @@ class Server:
     """slots: workers (each a context and stream); small_slots of them take
     only requests of at most small_points points.  policy "ratio" (highest
     response ratio next), "fifo" (arrival order) or "sjf" (fewest points
-    first)."""
+    first).  sharing: the workers' contexts in sharing mode
+    (wb_ctx_set_sharing)."""
 
     def __init__(self, slots=4, device=0, mem_fraction=0.8, params=None, policy="ratio",
-                 small_slots=1, small_points=1_500_000, batch_points=4_000_000, max_batch=16):
+                 small_slots=1, small_points=1_500_000, batch_points=4_000_000, max_batch=16,
+                 sharing=False):
         if policy not in ("ratio", "fifo", "sjf"):
             raise ValueError(f"unknown policy {policy!r}")
         _, total = W.device_memory(device)
@@ -99,7 +102,11 @@ class Server:
         ctxs = []
         try:
             for _ in range(slots):
-                ctxs.append(W.Context(device))
+                # sharing: the workers' long kernels leave room on every CU,
+                # so a small tile's kernels are not held behind one large
+                # tile's (with several large tiles at once the device fills
+                # again)
+                ctxs.append(W.Context(device, sharing=True) if sharing else W.Context(device))
         except BaseException:
             for c in ctxs:
                 c.close()
@@ -203,7 +210,7 @@ class Server:
                     r.batch = len(reqs)
                 if len(reqs) == 1:
                     try:
-                        reqs[0].cls = classify_tile(ctx, reqs[0].tile, self.params)
+                        reqs[0].cls = classify_tile(ctx, reqs[0].tile, self.params, reqs[0].times)
                     except Exception as e:  # reported with the request
                         reqs[0].error = e
                 else:
@@ -215,7 +222,7 @@ class Server:
                         # a bad tile fails only its own request
                         for r in reqs:
                             try:
-                                r.cls = classify_tile(ctx, r.tile, self.params)
+                                r.cls = classify_tile(ctx, r.tile, self.params, r.times)
                             except Exception as e:
                                 r.error = e
                 t1 = time.perf_counter()
@@ -242,14 +249,21 @@ def _build(ctx, tile):
                           tile["offset"], center, side)
 
 
-def classify_tile(ctx, tile, params):
-    """Build the tile's octree and classify it; returns class bytes (input order)."""
+def classify_tile(ctx, tile, params, times=None):
+    """Build the tile's octree and classify it; returns class bytes (input
+    order).  times: a dict that receives the wall time of the build call and
+    of the classification call (s)."""
+    t0 = time.perf_counter()
     tree = _build(ctx, tile)
+    t1 = time.perf_counter()
     try:
         cls, _, _ = W.classify(tree, tile["bbmin"], tile["bbmax"], params, want_tiles=False,
                                out=np.empty(tree.n_input, dtype=np.uint8))
     finally:
         tree.free()
+    if times is not None:
+        times["build"] = t1 - t0
+        times["classify"] = time.perf_counter() - t1
     return cls
 
 
@@ -329,7 +343,7 @@ def pin_tiles(pool):
 
 
 def load_test(pool, slots, load, nreq, seed, device=0, policy="ratio", small_slots=1,
-              small_points=1_500_000, batch_points=4_000_000, ref_tile=None):
+              small_points=1_500_000, batch_points=4_000_000, ref_tile=None, sharing=False):
     """Each pool tile timed alone (one context, nothing else on the device),
     the saturated capacity (every pool tile submitted at once), then `nreq`
     requests drawn uniformly from the pool arriving as a Poisson process at
@@ -361,7 +375,8 @@ def load_test(pool, slots, load, nreq, seed, device=0, policy="ratio", small_slo
     single_pool = npts.sum() / sum(iso_t)
     single_ref = ref_n / ref_s
     srv = Server(slots=slots, device=device, params=params, policy=policy,
-                 small_slots=small_slots, small_points=small_points, batch_points=batch_points)
+                 small_slots=small_slots, small_points=small_points, batch_points=batch_points,
+                 sharing=sharing)
     try:
         # capacity: every pool tile submitted at once
         t0 = time.perf_counter()
@@ -403,6 +418,7 @@ def load_test(pool, slots, load, nreq, seed, device=0, policy="ratio", small_slo
     return {
         "metric": "serving: classified points/s and p50/p99 tile latency",
         "slots": slots, "small_slots": srv.small_slots, "policy": policy, "load": load,
+        "sharing": bool(sharing),
         "requests": nreq, "pool_tiles": len(pool),
         "pool_points": {"min": int(npts.min()), "median": int(np.median(npts)),
                         "max": int(npts.max()), "mean": int(npts.mean())},
@@ -422,7 +438,8 @@ def load_test(pool, slots, load, nreq, seed, device=0, policy="ratio", small_slo
         "isolated_ms": {"min": round(1e3 * min(iso_t), 3), "max": round(1e3 * max(iso_t), 3)},
         "slowest_1pct": [{"points": r.n, "isolated_ms": round(1e3 * iso_t[r.tag], 2),
                           "wait_ms": round(1e3 * (r.t_start - r.t_arrive), 2),
-                          "run_ms": round(1e3 * (r.t_end - r.t_start), 2), "batch": r.batch}
+                          "run_ms": round(1e3 * (r.t_end - r.t_start), 2), "batch": r.batch,
+                          **{k + "_ms": round(1e3 * v, 2) for k, v in r.times.items()}}
                          for r in worst],
         "batched_requests": int((batches > 1).sum()),
         "mean_batch": round(float(batches.mean()), 2),
@@ -451,6 +468,8 @@ def main(argv=None):
     ap.add_argument("--policy", choices=("ratio", "fifo", "sjf"), default="ratio")
     ap.add_argument("--pageable", action="store_true",
                     help="keep the tiles in pageable host memory instead of pinned buffers")
+    ap.add_argument("--sharing", action="store_true",
+                    help="workers' contexts in sharing mode (resident, capped long kernels)")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="hardware queues per process (GPU_MAX_HW_QUEUES; 0 keeps the "
                          "environment's)")
@@ -465,7 +484,7 @@ def main(argv=None):
         ref = pin_tiles([ref])[0]
     res = load_test(pool, args.slots, args.load, args.requests, args.seed, policy=args.policy,
                     small_slots=args.small_slots, small_points=int(args.small_points),
-                    batch_points=int(args.batch_points), ref_tile=ref)
+                    batch_points=int(args.batch_points), ref_tile=ref, sharing=args.sharing)
     res["host_buffers"] = "pageable" if args.pageable else "pinned"
     res["hw_queues"] = os.environ.get("GPU_MAX_HW_QUEUES", "runtime default")
     print(json.dumps(res), flush=True)
