@@ -17,6 +17,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GPU_SO = os.path.join(HERE, "libwb_gpu.so")
+# another build of the library (e.g. one compiled with different settings,
+# to compare two builds on one device); the in-tree build by default
+GPU_SO = os.environ.get("WB_GPU_LIB", GPU_SO)
 
 WB_SHAPE_CYLINDER, WB_SHAPE_SPHERE, WB_SHAPE_HYPERBOLOID = 0, 1, 2
 WB_SHAPE_NPARAMS = 5

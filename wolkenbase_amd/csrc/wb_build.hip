@@ -86,13 +86,45 @@ __global__ void k_scan_down(uint32_t *data, uint64_t n, const uint32_t *boff)
   }
 }
 
+// exclusive scan of n <= SCAN1_MAX values by one block of 1024 threads, each
+// taking a run of consecutive values (the block sums of a larger scan)
+#define SCAN1_MAX 65536
+__global__ void __launch_bounds__(1024) k_scan_one(uint32_t *data, uint32_t n)
+{
+  __shared__ uint32_t wsum[16];
+  const uint32_t per = (n + 1023) / 1024, b = threadIdx.x * per;
+  uint32_t s = 0;
+  for (uint32_t k = 0; k < per; k++)
+    if (b + k < n) s += data[b + k];
+  uint32_t inc = s;  // inclusive scan within the wave
+  for (int d = 1; d < WAVE; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d, WAVE);
+    if ((int)lane_id() >= d) inc += o;
+  }
+  const int w = threadIdx.x / WAVE;
+  if ((int)lane_id() == WAVE - 1) wsum[w] = inc;
+  __syncthreads();
+  uint32_t off = 0;
+  for (int k = 0; k < w; k++) off += wsum[k];
+  uint32_t ex = off + inc - s;
+  for (uint32_t k = 0; k < per; k++)
+    if (b + k < n) {
+      const uint32_t v = data[b + k];
+      data[b + k] = ex;
+      ex += v;
+    }
+}
+
 static void scan_rec(wb_ctx *c, uint32_t *data, uint64_t n)
 {
+  if (n <= SCAN1_MAX) {
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, c->stream, data, (uint32_t)n);
+    return;
+  }
   unsigned nb = wb_blocks(n, 1024);
   uint32_t *bs = (uint32_t *)wb_alloc(c, sizeof(uint32_t) * (nb + 1));
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c->stream, data, n, bs);
-  if (nb > 1) scan_rec(c, bs, nb);
-  else WB_CHECK(hipMemsetAsync(bs, 0, sizeof(uint32_t), c->stream));
+  scan_rec(c, bs, nb);
   hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, c->stream, data, n, bs);
   wb_free(c, bs, sizeof(uint32_t) * (nb + 1));
 }
