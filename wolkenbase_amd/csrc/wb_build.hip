@@ -9,6 +9,7 @@
 // z*4+y*2+x), and the DFS visits children 0..7, so DFS order is ascending
 // order of the Morton key built from those compare bits.
 #include "wb_internal.h"
+#include "wb_wave.h"
 
 #include <string.h>
 
@@ -887,23 +888,12 @@ __global__ void k_dup_compact(const uint64_t *key, const uint32_t *sidx, const u
   }
 }
 
-// Per-leaf canonical order, input order inside each leaf: a bitonic sort of
-// the leaf's input indices by a wave per leaf, in registers: leaves of
+// Per-leaf canonical order, input order inside each leaf: a sort of the
+// leaf's input indices by a wave per leaf, in registers: leaves of
 // 64(K/2)+1 .. 64K points (K = 1: up to 64) hold K keys per lane, element
-// e = 64s + lane in slot s; the bitonic network's exchanges across lanes
-// are shuffles, those within a lane (partner distance 64 or more) register
-// swaps.  No barrier, no LDS.  Keys (input indices) are distinct.
-// lane (lane ^ j)'s value, j < 64 a constant of the unrolled network: xor 1,
-// 2 and 8 stay inside the row of 16 lanes (quad_perm 1,0,3,2 and 2,3,0,1,
-// row_ror 8), other distances go through ds_bpermute
-__device__ __attribute__((always_inline)) inline uint32_t lane_xor(uint32_t v, int j)
-{
-  if (j == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xb1, 0xf, 0xf, false);
-  if (j == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4e, 0xf, 0xf, false);
-  if (j == 8) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xf, 0xf, false);
-  return (uint32_t)__shfl_xor((int)v, j, 64);
-}
-
+// e = 64s + lane in slot s, sorted by the register network of wb_wave.h up
+// to the next power of two of the leaf size.  No barrier, no LDS.  Keys
+// (input indices) are distinct.
 template <int K>
 __global__ void __launch_bounds__(256) k_leaf_canon_reg(const uint32_t *leaf_start, uint32_t nl,
                                                         const uint32_t *sidx, uint32_t *canon,
@@ -920,40 +910,7 @@ __global__ void __launch_bounds__(256) k_leaf_canon_reg(const uint32_t *leaf_sta
       key[s] = e < m ? sidx[b + e] : 0xffffffffu;
       val[s] = b + e;
     }
-#pragma unroll
-    for (int k = 2; k <= 64 * K; k <<= 1) {
-#pragma unroll
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        if (j < 64) {
-#pragma unroll
-          for (int s = 0; s < K; s++) {
-            const uint32_t e = 64u * s + lane;
-            // (the 16-slot network is not unrolled: there j is not a constant)
-            const uint32_t ok = K <= 8 ? lane_xor(key[s], j) : (uint32_t)__shfl_xor((int)key[s], j, 64);
-            const uint32_t ov = K <= 8 ? lane_xor(val[s], j) : (uint32_t)__shfl_xor((int)val[s], j, 64);
-            const bool up = (e & (uint32_t)k) == 0, low = (e & (uint32_t)j) == 0;
-            // the lower element of a pair keeps the smaller key when sorting up
-            const bool take = (low == up) ? ok < key[s] : ok > key[s];
-            key[s] = take ? ok : key[s];
-            val[s] = take ? ov : val[s];
-          }
-        } else {
-#pragma unroll
-          for (int s = 0; s < K; s++) {
-            const int s2 = s ^ (j / 64);
-            if (s2 > s) {
-              const uint32_t e = 64u * s + lane;
-              const bool up = (e & (uint32_t)k) == 0;
-              if ((key[s] > key[s2]) == up) {
-                const uint32_t tk = key[s], tv = val[s];
-                key[s] = key[s2]; val[s] = val[s2];
-                key[s2] = tk; val[s2] = tv;
-              }
-            }
-          }
-        }
-      }
-    }
+    wbw::wave_sort<K, true>(key, val, wbw::pow2_ceil(m));
 #pragma unroll
     for (int s = 0; s < K; s++) {
       const uint32_t e = 64u * s + lane;

@@ -33,6 +33,7 @@
 
 #include "wb_flowsnake.h"
 #include "wb_internal.h"
+#include "wb_wave.h"
 #include "../../include/wb_gpu.h"
 
 #define WAVE 64
@@ -844,31 +845,12 @@ __device__ __attribute__((always_inline)) inline void scan_reg(const TreeView &T
     const uint32_t e = s * WAVE + lane;
     c[s] = e < m ? lists[off + e] : 0xffffffffu;
   }
-  for (uint32_t k = 2; k <= (uint32_t)(WAVE * K); k <<= 1)
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      if (j < WAVE) {
-#pragma unroll
-        for (int s = 0; s < K; s++) {
-          const uint32_t e = s * WAVE + lane;
-          const uint32_t o = (uint32_t)__shfl_xor((int)c[s], (int)j, WAVE);
-          const bool up = (e & k) == 0, low = (e & j) == 0;
-          c[s] = (low == up) ? min(c[s], o) : max(c[s], o);
-        }
-      } else {
-        const uint32_t js = j / WAVE;
-#pragma unroll
-        for (int s = 0; s < K; s++) {
-#pragma unroll
-          for (int s2 = 0; s2 < K; s2++) {
-            if (s2 != (s | (int)js) || (s & (int)js)) continue;
-            const uint32_t e = s * WAVE + lane;
-            const bool up = (e & k) == 0;
-            const uint32_t a = c[s], b = c[s2];
-            if ((a > b) == up) { c[s] = b; c[s2] = a; }
-          }
-        }
-      }
-    }
+  {
+    // canonical order: the m ranks ascending (pads last), sorted only as far
+    // as the next power of two
+    uint32_t unused[K];
+    wbw::wave_sort<K, false>(c, unused, wbw::pow2_ceil(m));
+  }
   double cx, cy;
   tile_center(g, ex, ey, &cx, &cy);
   const double r = g.r;
@@ -973,9 +955,53 @@ __device__ __attribute__((always_inline)) inline void scan_reg(const TreeView &T
   }
 }
 
+// Size classes of the scan: 0 = 1..64 points, 1 = 65-128, 2 = 129-256,
+// 3 = more.  One thread per tile: an empty tile gets its record here, every
+// other tile goes on its class's list (class 0 from the front of la, class
+// 3 from its back, class 1 from the front of lb, class 2 from its back;
+// ncls[c] the lengths), so each class kernel visits only its own tiles.
+// The order inside a list does not matter: every tile's record depends on
+// that tile alone.
+__global__ void k_scan_classes(uint32_t ntiles, const uint32_t *tile_cnt, uint32_t *la,
+                               uint32_t *lb, uint32_t *ncls, TileRec *rec)
+{
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t m = t < ntiles ? tile_cnt[t] : 0u;
+  if (t < ntiles && m == 0) {
+    TileRec r = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    rec[t] = r;
+  }
+  const int c = m == 0 ? -1 : m <= WAVE ? 0 : m <= 2 * WAVE ? 1 : m <= 4 * WAVE ? 2 : 3;
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint64_t mask = __ballot(c == k);
+    if (!mask) continue;
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(&ncls[k], (uint32_t)__popcll(mask));
+    base = (uint32_t)__shfl((int)base, leader, WAVE);
+    if (c != k) continue;
+    const uint32_t pos = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+    if (k == 0) la[pos] = t;
+    else if (k == 1) lb[pos] = t;
+    else if (k == 2) lb[ntiles - 1 - pos] = t;
+    else la[ntiles - 1 - pos] = t;
+  }
+}
+
+// A class list: entry i at list[step * i] (step +1 for a list filled from
+// the front of its buffer, -1 from the back), n entries.
+struct ClassList {
+  const uint32_t *list;
+  int32_t step;
+  const uint32_t *n;
+  __device__ uint32_t at(uint32_t i) const { return list[(int64_t)step * i]; }
+};
+
 // One wave per tile, over the tiles of one size class: up to 64, 65-128 and
-// 129-256 points (K = 1, 2, 4 points per lane; the K = 1 kernel also writes
-// the empty tiles).  Each class has its own register budget.
+// 129-256 points (K = 1, 2, 4 points per lane).  Each class has its own
+// register budget.  The next tile's number is read before this tile's scan.
 #ifndef WB_SCAN_MINW
 #define WB_SCAN_MINW 5
 #endif
@@ -985,21 +1011,19 @@ template <int K, bool F32>
 __global__ void __launch_bounds__(64, WB_SCAN_MINW) k_scan_reg(TreeView T, TileGeo g, const int32_t *tex,
                                                  const int32_t *tey, const uint32_t *tile_off,
                                                  const uint32_t *tile_cnt, const uint32_t *lists,
-                                                 double min_hs, int prec, TileRec *rec,
-                                                 unsigned long long *ctr)
+                                                 ClassList cl, double min_hs, int prec,
+                                                 TileRec *rec, unsigned long long *ctr)
 {
-  for (uint32_t t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
-    const uint32_t m = tile_cnt[t];
-    if (K == 1 && m == 0) {
-      if (threadIdx.x == 0) {
-        TileRec r = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        rec[t] = r;
-      }
-      continue;
-    }
-    const uint32_t below = K == 1 ? 0u : (uint32_t)(WAVE * K / 2);  // the next smaller class
-    if (m <= below || m > (uint32_t)(WAVE * K)) continue;
-    scan_reg<K, F32>(T, g, tex[t], tey[t], tile_off[t], m, lists, min_hs, prec, rec + t, ctr);
+  const uint32_t n = *cl.n;
+  uint32_t i = blockIdx.x;
+  if (i >= n) return;
+  uint32_t t = cl.at(i);
+  for (; i < n; i += gridDim.x) {
+    const uint32_t inext = i + gridDim.x;
+    const uint32_t tnext = inext < n ? cl.at(inext) : 0u;
+    scan_reg<K, F32>(T, g, tex[t], tey[t], tile_off[t], tile_cnt[t], lists, min_hs, prec, rec + t,
+                     ctr);
+    t = tnext;
   }
 }
 
@@ -1111,14 +1135,15 @@ __device__ void scan_big(const TreeView &T, const TileGeo &g, int ex, int ey, ui
 __global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_t *tex,
                                              const int32_t *tey, const uint32_t *tile_off,
                                              const uint32_t *tile_cnt, const uint32_t *lists,
-                                             double min_hs, int prec, TileRec *rec, uint32_t *err,
-                                             unsigned long long *ctr)
+                                             ClassList cl, double min_hs, int prec, TileRec *rec,
+                                             uint32_t *err, unsigned long long *ctr)
 {
   __shared__ uint32_t cp[SCAN_CAP];
   __shared__ int histo[7];
-  for (uint32_t t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
+  const uint32_t n = *cl.n;
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const uint32_t t = cl.at(i);
     const uint32_t m = tile_cnt[t];
-    if (m <= 4 * WAVE) continue;
     if (m > SCAN_CAP) {
       if (threadIdx.x == 0) atomicOr(err, 2u);
       continue;
@@ -2397,7 +2422,8 @@ static void classify_prepare(ClassifyRun &R, wb_tree *t, wb_ctx *c, const double
   uint32_t *nmemb = R.nmemb = (uint32_t *)wb_alloc(c, 4ull * n + 4);
   // err[0] error flags; then u64 counters: 0 sector borderline, 1 exact atan2i
   // steps, 2-4 class histogram, 5 non-empty tiles, 6 (two u32) exact gap
-  // points and heavy points, 7 FP64 fit fallbacks
+  // points and heavy points, 7 FP64 fit fallbacks; u32 18-19 work counters of
+  // the heavy and resident ground kernels, 20-23 the scan's class list lengths
   uint32_t *err = R.err = (uint32_t *)wb_alloc(c, 128);
   unsigned long long *ctr = R.ctr = (unsigned long long *)err + 1;
   uint32_t *nslow = R.nslow = (uint32_t *)(ctr + 6);
@@ -2430,28 +2456,35 @@ static void classify_prepare(ClassifyRun &R, wb_tree *t, wb_ctx *c, const double
                        g.lo, t->crank, tile_off, tile_fill, lists);
   WB_CHECK(hipEventRecord(ev[1], st));
   {
-    // one wave per tile of each size class; grid-stride over the tiles
+    // the size classes' tile lists, then one wave per tile of each class,
+    // grid-stride over its list
+    uint32_t *la = tile_fill;  // free until the ground counts
+    uint32_t *lb = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
+    uint32_t *ncls = err + 20;
     const unsigned nb = (unsigned)std::min<uint64_t>(g.ntiles, 1u << 16);
     const int prec = p->fit_precision;
     if (nb) {
+      hipLaunchKernelGGL(k_scan_classes, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st,
+                         g.ntiles, tile_cnt, la, lb, ncls, rec);
+      const ClassList c1 = {la, 1, ncls}, c2 = {lb, 1, ncls + 1};
+      const ClassList c4 = {lb + g.ntiles - 1, -1, ncls + 2}, cbig = {la + g.ntiles - 1, -1, ncls + 3};
+#define WB_SCAN_CLASSES(F32)                                                                      \
+  hipLaunchKernelGGL((k_scan_reg<1, F32>), dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,   \
+                     tile_cnt, lists, c1, p->min_hs, prec, rec, ctr);                             \
+  hipLaunchKernelGGL((k_scan_reg<2, F32>), dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,   \
+                     tile_cnt, lists, c2, p->min_hs, prec, rec, ctr);                             \
+  hipLaunchKernelGGL((k_scan_reg<4, F32>), dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,   \
+                     tile_cnt, lists, c4, p->min_hs, prec, rec, ctr)
       if (prec == WB_FIT_FP64) {
-        hipLaunchKernelGGL((k_scan_reg<1, false>), dim3(nb), dim3(64), 0, st, T, g, tex, tey,
-                           tile_off, tile_cnt, lists, p->min_hs, prec, rec, ctr);
-        hipLaunchKernelGGL((k_scan_reg<2, false>), dim3(nb), dim3(64), 0, st, T, g, tex, tey,
-                           tile_off, tile_cnt, lists, p->min_hs, prec, rec, ctr);
-        hipLaunchKernelGGL((k_scan_reg<4, false>), dim3(nb), dim3(64), 0, st, T, g, tex, tey,
-                           tile_off, tile_cnt, lists, p->min_hs, prec, rec, ctr);
+        WB_SCAN_CLASSES(false);
       } else {
-        hipLaunchKernelGGL((k_scan_reg<1, true>), dim3(nb), dim3(64), 0, st, T, g, tex, tey,
-                           tile_off, tile_cnt, lists, p->min_hs, prec, rec, ctr);
-        hipLaunchKernelGGL((k_scan_reg<2, true>), dim3(nb), dim3(64), 0, st, T, g, tex, tey,
-                           tile_off, tile_cnt, lists, p->min_hs, prec, rec, ctr);
-        hipLaunchKernelGGL((k_scan_reg<4, true>), dim3(nb), dim3(64), 0, st, T, g, tex, tey,
-                           tile_off, tile_cnt, lists, p->min_hs, prec, rec, ctr);
+        WB_SCAN_CLASSES(true);
       }
+#undef WB_SCAN_CLASSES
       hipLaunchKernelGGL(k_scan, dim3(std::min(nb, 4096u)), dim3(64), 0, st, T, g, tex, tey,
-                         tile_off, tile_cnt, lists, p->min_hs, prec, rec, err, ctr);
+                         tile_off, tile_cnt, lists, cbig, p->min_hs, prec, rec, err, ctr);
     }
+    wb_free(c, lb, 4ull * g.ntiles + 4);
   }
   hipLaunchKernelGGL(k_postscan, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey, rec,
                      ctr + 5);
