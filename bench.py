@@ -363,6 +363,69 @@ def serving_summary(nreq):
         return {"error": repr(e)}
 
 
+def query_kernels(ctx, cloud, nq, seed=7):
+    """Batched OctStore::countPointsIn / pointsIn (octree.cpp:1214-1264) over
+    the tile's octree: nq spheres of radius 1 m and nq cylinders of radius
+    0.877 m (the tile cylinder of a 1 m tiling of a 900 m cube) centred on
+    input points drawn at random (seeded), and the index pass of pointsIn
+    for the spheres.  Kernel times from the library's HIP events.
+    Algorithmic bytes per query: the 40-B shape in, the 8-B count out and
+    the 24-B FP64 coordinates of every point inside it (the least a kernel
+    reads to decide it); the index pass adds 4 B per index written."""
+    import numpy as np
+
+    import wolkenbase_amd as W
+
+    center, side = W.Octree.size_fit(cloud["bbmin"], cloud["bbmax"])
+    tree = W.Octree.build(ctx, cloud["xi"], cloud["yi"], cloud["zi"], cloud["scale"],
+                          cloud["offset"], center, side)
+    rng = np.random.default_rng(seed)
+    pick = rng.integers(0, len(cloud["xi"]), nq)
+    c = [cloud["offset"][k] + cloud["scale"][k] * cloud[a][pick].astype(np.float64)
+         for k, a in enumerate(("xi", "yi", "zi"))]
+    zero = np.zeros(nq)
+    batches = {
+        "sphere_r1m": (W.WB_SHAPE_SPHERE, np.stack([c[0], c[1], c[2], zero + 1.0, zero], 1)),
+        "cylinder_r0.877m": (W.WB_SHAPE_CYLINDER,
+                             np.stack([c[0], c[1], zero + 0.877, zero, zero], 1)),
+    }
+    out = {}
+    try:
+        for name, (kind, prm) in batches.items():
+            tree.count_points_in((kind, prm))  # warm-up
+            ms, wall = [], []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                cnt = tree.count_points_in((kind, prm))
+                wall.append(time.perf_counter() - t0)
+                ms.append(tree.query_ms()[0])
+            kms = sorted(ms)[1]
+            inside = float(cnt.sum())
+            alg = 48.0 * nq + 24.0 * inside
+            gbs = alg / (kms * 1e-3) / 1e9
+            out[name] = {"queries": nq, "mean_points_in": round(inside / nq, 2),
+                         "kernel": "k_count_points_in", "kernel_ms": round(kms, 3),
+                         "queries_per_s": round(nq / (kms * 1e-3), 1),
+                         "call_ms_host_in_out": round(1e3 * sorted(wall)[1], 3),
+                         "algorithmic_bytes": alg, "gbs": round(gbs, 1),
+                         "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 5)}
+        kind, prm = batches["sphere_r1m"]
+        npq = min(nq, 1 << 18)
+        idx = tree.points_in((kind, prm[:npq]))
+        pms = tree.query_ms()[1]
+        total = float(sum(len(a) for a in idx))
+        alg = 48.0 * npq + 28.0 * total
+        gbs = alg / (pms * 1e-3) / 1e9
+        out["points_in_sphere_r1m"] = {"queries": npq, "indices": int(total),
+                                       "kernel": "k_points_in", "kernel_ms": round(pms, 3),
+                                       "queries_per_s": round(npq / (pms * 1e-3), 1),
+                                       "algorithmic_bytes": alg, "gbs": round(gbs, 1),
+                                       "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 5)}
+    finally:
+        tree.free()
+    return out
+
+
 def kernel_times(stats):
     def avg(f):
         return sum(s[f] for s in stats) / len(stats)
@@ -399,6 +462,10 @@ def main():
     ap.add_argument("--no-serving", action="store_true",
                     help="tile20m: skip the short serving-stream measurement")
     ap.add_argument("--serving-requests", type=int, default=400)
+    ap.add_argument("--no-queries", action="store_true",
+                    help="tile20m: skip the batched shape-query measurement")
+    ap.add_argument("--queries", type=int, default=1 << 20,
+                    help="tile20m: shapes per batched query measurement")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="run one 256 MiB device copy first, so a --pmc profile of this "
                          "command can scale FETCH_SIZE/WRITE_SIZE against known bytes")
@@ -477,6 +544,16 @@ def main():
                         "atan_x87_steps": int(st["atan_exact_calls"]),
                         "sector_borderline": int(st["sector_borderline"])},
     }
+    # the radix sort (K2) against the same peak: 24 B per point and pass
+    # moved, plus the 8-B key read of each pass's digit histogram
+    sort_alg = (24.0 + 8.0) * passes * n
+    sort_gbs = sort_alg / (kern["radix_sort"] * 1e-3) / 1e9
+    out["kernels"] = {"radix_sort": {"passes": passes, "ms": round(kern["radix_sort"], 3),
+                                     "algorithmic_bytes": sort_alg, "gbs": round(sort_gbs, 1),
+                                     "frac_of_8TBs": round(sort_gbs / HBM_PEAK_GBS, 5)}}
+    if args.workload == "tile20m" and not args.no_queries:
+        log("[bench] batched shape queries ...")
+        out["kernels"].update(query_kernels(ctx, cloud, args.queries))
     if args.workload == "tile20m" and not args.no_mix:
         # the same tile with every outlier above the surface: a realistic
         # ground share, where most points need the exhaustive search

@@ -194,6 +194,12 @@ int wb_count_points_in(wb_tree *tree, int kind, const double *params, uint64_t n
                        uint64_t *counts);
 int wb_points_in(wb_tree *tree, int kind, const double *params, uint64_t nshapes,
                  uint64_t *offsets, uint32_t *idx, uint64_t cap);
+/* device time (ms, HIP events on the context stream) of the kernels of the
+ * tree's last wb_count_points_in / wb_points_in call: the counting kernel,
+ * and wb_points_in's index kernel (0 when the call wrote no indices).  No
+ * counterpart in the reference (its queries are not timed); for
+ * measurement. */
+int wb_tree_query_ms(wb_tree *tree, double *count_ms, double *points_ms);
 
 /* --- wb_count_points_in with no device allocation.  No counterpart in the
  *     reference: for callers that cannot allocate device memory (code under

@@ -134,6 +134,62 @@ def test_count_and_points_in(ctx, kind):
         assert np.array_equal(lists[s].astype(np.int64), ref.points_in(kind, P[s])), s
 
 
+def _noalloc_counts(tree, kind, P):
+    """The wave-per-shape kernel (the no-allocation entry point, which takes
+    the shapes in the caller's order), in batches of its maximum."""
+    import ctypes as C
+
+    out = np.zeros(len(P), dtype=np.uint64)
+    step = 65536
+    for a in range(0, len(P), step):
+        blk = np.ascontiguousarray(P[a:a + step])
+        o = np.zeros(len(blk), dtype=np.uint64)
+        rc = W.lib().wb_count_points_in_noalloc(
+            tree.h, kind, blk.ctypes.data_as(C.POINTER(C.c_double)), len(blk),
+            o.ctypes.data_as(C.POINTER(C.c_uint64)))
+        assert rc == 0
+        out[a:a + step] = o
+    return out
+
+
+@pytest.mark.parametrize("kind", [W.WB_SHAPE_CYLINDER, W.WB_SHAPE_SPHERE, W.WB_SHAPE_HYPERBOLOID])
+def test_batched_queries_large_batch(ctx, kind):
+    """Many shapes in random order (the lane-per-shape kernels visit them in
+    Morton order of their centres): counts equal the wave-per-shape kernel's
+    for all of them and the oracle's for a sample; index lists equal the
+    oracle's, and their lengths the counts.  Edge shapes: far outside the
+    cloud, radius zero, and one that holds every leaf whole."""
+    cloud = synth.generate(seed=31, extent=80.0, density=25.0, outliers=0.01)
+    tree, ref, (x, y, z) = build_both(ctx, cloud)
+    rng = np.random.default_rng(100 + kind)
+    P = random_shapes(rng, x, y, z, kind, 20000)
+    far = P[:3].copy()
+    far[:, 0] += 1e4
+    zero = P[3:6].copy()
+    if kind == W.WB_SHAPE_HYPERBOLOID:
+        zero[:, 3] = 0.0
+        big = np.array([[x.mean(), y.mean(), z.max() + 1e4, 1.0, 1.0]])
+    elif kind == W.WB_SHAPE_SPHERE:
+        zero[:, 3] = 0.0
+        big = np.array([[x.mean(), y.mean(), z.mean(), 1e5, 0.0]])
+    else:
+        zero[:, 2] = 0.0
+        big = np.array([[x.mean(), y.mean(), 1e5, 0.0, 0.0]])
+    P = np.concatenate([P, far, zero, big])
+    counts = tree.count_points_in((kind, P))
+    assert np.array_equal(counts, _noalloc_counts(tree, kind, P))
+    assert counts[-1] == len(x)
+    assert (counts[-7:-4] == 0).all()
+    for s in list(rng.integers(0, len(P), 150)) + list(range(len(P) - 7, len(P))):
+        assert counts[s] == ref.count_points_in(kind, P[s]), (s, P[s])
+    sub = np.concatenate([P[:3000], P[-7:]])
+    lists = tree.points_in((kind, sub))
+    assert [len(a) for a in lists] == [int(c) for c in np.concatenate([counts[:3000],
+                                                                       counts[-7:]])]
+    for s in list(rng.integers(0, 3000, 80)) + list(range(3000, 3007)):
+        assert np.array_equal(lists[s].astype(np.int64), ref.points_in(kind, sub[s])), s
+
+
 # the third cloud is dense enough for tiles of 65-128, 129-256 and more than
 # 256 points (each size class of the scan has its own kernel)
 @pytest.mark.parametrize("cfg", [dict(seed=21, extent=24.0, density=25.0),

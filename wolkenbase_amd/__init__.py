@@ -147,6 +147,7 @@ SIGNATURES = {
     "wb_tree_duplicates": (C.c_int, [_vp, _u64p, _u32p]),
     "wb_tree_free": (C.c_int, [_vp]),
     "wb_tree_info": (C.c_int, [_vp, _u64p, _u64p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "wb_tree_query_ms": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "wb_tree_leaves": (C.c_int, [_vp, C.POINTER(Leaf), C.c_uint64, _u64p]),
     "wb_tree_order": (C.c_int, [_vp, _u32p]),
     "wb_count_points_in": (C.c_int, [_vp, C.c_int, _dp, C.c_uint64, _u64p]),
@@ -560,6 +561,14 @@ class Octree:
                                                         out.ctypes.data_as(_u64p)),
                        "wb_count_points_in_noalloc")
         return out
+
+    def query_ms(self):
+        """Device time (ms) of the kernels of the last count_points_in /
+        points_in call on this tree: (counting kernel, index kernel)."""
+        a, b = C.c_double(0), C.c_double(0)
+        self.ctx.check(lib().wb_tree_query_ms(self.h, C.byref(a), C.byref(b)),
+                       "wb_tree_query_ms")
+        return a.value, b.value
 
     def points_in(self, shapes):
         """OctStore::pointsIn (unsorted) for a batch: list of input-index arrays

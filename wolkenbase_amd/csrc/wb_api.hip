@@ -288,6 +288,7 @@ int wb_tree_build(wb_ctx *c, const int32_t *x, const int32_t *y, const int32_t *
   t->side = root_side;
   t->sort_passes = 0;
   t->ms_build = t->ms_sort = 0;
+  t->ms_query_count = t->ms_query_points = 0;
   if (xf->unit != 1.0) {
     delete t;
     return fail(c, "only unit == 1 is supported (metres)", WB_ERANGE);
@@ -352,6 +353,7 @@ int wb_tree_build_xyz(wb_ctx *c, const double *x, const double *y, const double 
   t->side = root_side;
   t->sort_passes = 0;
   t->ms_build = t->ms_sort = 0;
+  t->ms_query_count = t->ms_query_points = 0;
   API_TRY(c)
   WB_CHECK(hipSetDevice(c->device));
   const double *dx = x, *dy = y, *dz = z;
@@ -404,6 +406,14 @@ int wb_tree_free(wb_tree *t)
     if (e) hipEventDestroy(e);
   c->trees.erase(t);
   delete t;
+  return WB_OK;
+}
+
+int wb_tree_query_ms(wb_tree *t, double *count_ms, double *points_ms)
+{
+  if (!t) return WB_EINVAL;
+  if (count_ms) *count_ms = t->ms_query_count;
+  if (points_ms) *points_ms = t->ms_query_points;
   return WB_OK;
 }
 
@@ -463,6 +473,23 @@ int wb_tree_order(wb_tree *t, uint32_t *order)
 
 static int check_kind(int kind) { return kind == WB_SHAPE_CYLINDER || kind == WB_SHAPE_SPHERE || kind == WB_SHAPE_HYPERBOLOID; }
 
+// Events around a query's kernels on the context stream; ms() after the
+// stream has passed the second one.
+struct QueryTimer {
+  wb_ctx *c;
+  hipEvent_t e[2];
+  explicit QueryTimer(wb_ctx *c_) : c(c_) { e[0] = wb_event_get(c); e[1] = wb_event_get(c); }
+  ~QueryTimer() { wb_event_put(c, e[0]); wb_event_put(c, e[1]); }
+  void start() { WB_CHECK(hipEventRecord(e[0], c->stream)); }
+  void stop() { WB_CHECK(hipEventRecord(e[1], c->stream)); }
+  float ms()
+  {
+    float v = 0;
+    WB_CHECK(hipEventElapsedTime(&v, e[0], e[1]));
+    return v;
+  }
+};
+
 int wb_count_points_in(wb_tree *t, int kind, const double *params, uint64_t ns, uint64_t *counts)
 {
   if (!t || (!params && ns) || (!counts && ns)) return WB_EINVAL;
@@ -475,9 +502,16 @@ int wb_count_points_in(wb_tree *t, int kind, const double *params, uint64_t ns, 
   double *dp = (double *)wb_alloc(c, pb);
   uint64_t *dc = (uint64_t *)wb_alloc(c, 8 * ns);
   WB_CHECK(hipMemcpyAsync(dp, params, pb, hipMemcpyHostToDevice, c->stream));
-  wb_run_count(t, kind, dp, ns, dc);
+  QueryTimer qt(c);
+  qt.start();
+  uint32_t *order = wb_query_order(t, kind, dp, ns);
+  wb_run_count(t, kind, dp, ns, dc, order);
+  qt.stop();
   WB_CHECK(hipMemcpyAsync(counts, dc, 8 * ns, hipMemcpyDeviceToHost, c->stream));
   WB_CHECK(hipStreamSynchronize(c->stream));
+  t->ms_query_count = qt.ms();
+  t->ms_query_points = 0;
+  wb_free(c, order, 4 * ns);
   wb_free(c, dp, pb);
   wb_free(c, dc, 8 * ns);
   return WB_OK;
@@ -505,7 +539,7 @@ int wb_count_points_in_noalloc(wb_tree *t, int kind, const double *params, uint6
   uint64_t *dc = (uint64_t *)((unsigned char *)c->scratch + WB_NOALLOC_PARAM_BYTES);
   WB_CHECK(hipMemcpyAsync(dp, params, sizeof(double) * WB_SHAPE_NPARAMS * ns,
                           hipMemcpyHostToDevice, c->stream));
-  wb_run_count(t, kind, dp, ns, dc);
+  wb_run_count(t, kind, dp, ns, dc, nullptr);
   WB_CHECK(hipMemcpyAsync(counts, dc, 8 * ns, hipMemcpyDeviceToHost, c->stream));
   WB_CHECK(hipStreamSynchronize(c->stream));
   return WB_OK;
@@ -526,25 +560,36 @@ int wb_points_in(wb_tree *t, int kind, const double *params, uint64_t ns, uint64
   double *dp = (double *)wb_alloc(c, pb);
   uint64_t *dc = (uint64_t *)wb_alloc(c, 8 * (ns + 1));
   WB_CHECK(hipMemcpyAsync(dp, params, pb, hipMemcpyHostToDevice, c->stream));
-  wb_run_count(t, kind, dp, ns, dc);
+  QueryTimer qt(c);
+  qt.start();
+  uint32_t *order = wb_query_order(t, kind, dp, ns);
+  wb_run_count(t, kind, dp, ns, dc, order);
+  qt.stop();
   std::vector<uint64_t> cnt(ns);
   WB_CHECK(hipMemcpyAsync(cnt.data(), dc, 8 * ns, hipMemcpyDeviceToHost, c->stream));
   WB_CHECK(hipStreamSynchronize(c->stream));
+  t->ms_query_count = qt.ms();
+  t->ms_query_points = 0;
   for (uint64_t s = 0; s < ns; s++) offsets[s + 1] = offsets[s] + cnt[s];
   if (idx) {
     uint64_t total = offsets[ns];
     if (cap < total) {
+      wb_free(c, order, 4 * ns);
       wb_free(c, dp, pb);
       wb_free(c, dc, 8 * (ns + 1));
       return fail(c, "index buffer too small", WB_EINVAL);
     }
     WB_CHECK(hipMemcpyAsync(dc, offsets, 8 * (ns + 1), hipMemcpyHostToDevice, c->stream));
     uint32_t *dout = (uint32_t *)wb_alloc(c, 4 * total + 4);
-    wb_run_points(t, kind, dp, ns, dc, dout);
+    qt.start();
+    wb_run_points(t, kind, dp, ns, dc, dout, order);
+    qt.stop();
     WB_CHECK(hipMemcpyAsync(idx, dout, 4 * total, hipMemcpyDeviceToHost, c->stream));
     WB_CHECK(hipStreamSynchronize(c->stream));
+    t->ms_query_points = qt.ms();
     wb_free(c, dout, 4 * total + 4);
   }
+  wb_free(c, order, 4 * ns);
   wb_free(c, dp, pb);
   wb_free(c, dc, 8 * (ns + 1));
   return WB_OK;

@@ -198,6 +198,294 @@ __global__ void __launch_bounds__(64) k_points_in(TreeView T, int kind, const do
 }
 
 // ---------------------------------------------------------------------------
+// Batched countPointsIn / pointsIn with a lane per shape.  The shapes are
+// taken in Morton order of their centres (k_query_keys + the radix sort), so
+// the 64 shapes of a wave are neighbours and share most of their leaves.
+// A wave walks the octree once for its 64 shapes: findBlocks
+// (octree.cpp:234-251) with an explicit stack whose entries carry the mask of
+// lanes for which the node and every ancestor intersect the lane's shape, so
+// each lane sees exactly the leaves findBlocks returns for its own shape, in
+// the same (DFS) order.  A leaf whose eight corners are all in a lane's shape
+// counts whole for that lane (octree.cpp:1224-1226, 1252-1254); otherwise the
+// lane tests the leaf's points, skipping 32-point groups whose bounding box
+// no point of the shape can reach (wb::shape_may_hit_box: exact).  A group's
+// points are read once per wave with one vector load and handed to every
+// lane by readlane.
+// ---------------------------------------------------------------------------
+// pending entries at one time: at most 7 siblings per level on the path plus
+// 8 children (internal nodes sit at levels 0..WB_MAX_LEVEL)
+#define WB_QSTK (7 * (WB_MAX_LEVEL + 1) + 8 + 2)
+struct QueryStack {
+  int code[WB_QSTK];
+  uint32_t lo[WB_QSTK], hi[WB_QSTK];
+};
+
+template <int KIND>
+__device__ inline wb::Shape query_shape(const double *p)
+{
+  if (KIND == wb::SHAPE_CYLINDER) return wb::make_cylinder(p[0], p[1], p[2]);
+  if (KIND == wb::SHAPE_SPHERE) return wb::make_sphere(p[0], p[1], p[2], p[3]);
+  return wb::make_hyperboloid(p[0], p[1], p[2], p[3], p[4]);
+}
+
+// Octree::cube (octree.cpp:349-358) of child k of the cube (cx, cy, cz, side):
+// the operations the build used for the stored cubes
+__device__ inline void child_cube(double cx, double cy, double cz, double side, int k,
+                                  double &x, double &y, double &z, double &sd)
+{
+  x = cx + (double)(2 * (k & 1) - 1) * side / 4;
+  y = cy + (double)(2 * ((k >> 1) & 1) - 1) * side / 4;
+  z = cz + (double)(2 * (k >> 2) - 1) * side / 4;
+  sd = side / 2;
+}
+
+// The walk: leaf(l, on) for every leaf some lane reaches, in DFS order; `on`
+// tells whether this lane's findBlocks returns it.
+template <class Leaf>
+__device__ inline void query_walk(const TreeView &T, const wb::Shape &S, bool active,
+                                  QueryStack &K, Leaf &&leaf)
+{
+  if (T.nleaves == 0) return;
+  const uint64_t m0 = __ballot(active && wb::shape_intersect(S, T.node_cube[0], T.node_cube[1],
+                                                             T.node_cube[2], T.node_cube[3]));
+  if (!m0) return;
+  K.code[0] = 0;
+  K.lo[0] = (uint32_t)m0;
+  K.hi[0] = (uint32_t)(m0 >> 32);
+  int sp = 1;
+  const uint32_t lane = __lane_id();
+  while (sp > 0) {
+    sp--;
+    const int code = rfl(K.code[sp]);
+    const uint32_t mlo = (uint32_t)rfl((int)K.lo[sp]), mhi = (uint32_t)rfl((int)K.hi[sp]);
+    const bool on = ((lane < 32 ? mlo >> lane : mhi >> (lane - 32)) & 1u) != 0;
+    if (code < 0) {
+      leaf(-2 - code, on);
+      continue;
+    }
+    const double *nc = T.node_cube + 4 * (size_t)code;
+    const double cx = nc[0], cy = nc[1], cz = nc[2], side = nc[3];
+    const int *chp = T.node_child + 8 * (size_t)code;
+    int ck[8];
+    uint64_t mk[8];
+    uint32_t pushed = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) ck[k] = rfl(chp[k]);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      mk[k] = 0;
+      if (ck[k] != -1) {
+        double x, y, z, sd;
+        child_cube(cx, cy, cz, side, k, x, y, z, sd);
+        mk[k] = __ballot(on && wb::shape_intersect(S, x, y, z, sd));
+        if (mk[k]) pushed |= 1u << k;
+      }
+    }
+    // child 0 on top: a child's entry sits below those of the smaller indices
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (mk[k]) {
+        const int pos = sp + __popc(pushed >> (k + 1));
+        K.code[pos] = ck[k];
+        K.lo[pos] = (uint32_t)mk[k];
+        K.hi[pos] = (uint32_t)(mk[k] >> 32);
+      }
+    }
+    sp += __popc(pushed);
+  }
+}
+
+// Morton key of a shape's centre in the tree's root cube (21 bits per axis,
+// z, y, x interleaved as the octree's child index): the visiting order of the
+// lane-per-shape kernels.  A cylinder has no z.
+__device__ inline uint32_t query_cell(double d, double inv_cell)
+{
+  const double u = d * inv_cell;
+  if (!(u >= 0.0)) return 0;
+  if (u >= (double)(1u << WB_MAX_LEVEL)) return (1u << WB_MAX_LEVEL) - 1;
+  return (uint32_t)u;
+}
+
+__device__ inline uint64_t spread3(uint32_t v)
+{
+  uint64_t x = v & 0x1fffffull;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+__global__ void k_query_keys(const double *params, uint64_t ns, int kind, double lox, double loy,
+                             double loz, double inv_cell, uint64_t *key, uint32_t *val)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const double *p = params + WB_SHAPE_NPARAMS * i;
+  const uint32_t cx = query_cell(p[0] - lox, inv_cell), cy = query_cell(p[1] - loy, inv_cell);
+  const uint32_t cz = kind == wb::SHAPE_CYLINDER ? 0u : query_cell(p[2] - loz, inv_cell);
+  key[i] = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+  val[i] = (uint32_t)i;
+}
+
+// lane o's value of v (o wave-uniform)
+__device__ inline uint64_t rdl64(uint64_t v, uint32_t o)
+{
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)o);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)o);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// lane o's shape (o wave-uniform), read from the lane that holds it
+__device__ inline wb::Shape lane_shape(const wb::Shape &S, uint32_t o)
+{
+  wb::Shape R = S;
+  R.cx = rdl(S.cx, o);
+  R.cy = rdl(S.cy, o);
+  R.cz = rdl(S.cz, o);
+  R.r = rdl(S.r, o);
+  R.por2 = rdl(S.por2, o);
+  R.slope = rdl(S.slope, o);
+  return R;
+}
+
+// countPointsIn of shapes order[64w .. 64w+63] by wave w.  Inside a leaf,
+// each 32-point group is loaded once (lanes 0-31 and 32-63 both hold it)
+// and tested against the shapes of the lanes that may reach it ("owners"),
+// two per step: lanes 0-31 against the first owner's shape, lanes 32-63
+// against the second's, the two ballot halves counted for their owners.
+template <int KIND>
+__global__ void __launch_bounds__(256) k_query_count(TreeView T, const double *params,
+                                                     const uint32_t *order, uint64_t ns,
+                                                     uint64_t *counts)
+{
+  __shared__ QueryStack stk[4];
+  QueryStack &K = stk[threadIdx.x / WAVE];
+  const uint32_t lane = __lane_id();
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = q < ns;
+  const uint32_t si = active ? order[q] : 0u;
+  const wb::Shape S = query_shape<KIND>(params + WB_SHAPE_NPARAMS * (uint64_t)si);
+  uint64_t cnt = 0;
+  const bool hi_half = lane >= WAVE / 2;
+  query_walk(T, S, active, K, [&](int l, bool on) {
+    const double *lc = T.leaf_cube + 4 * (size_t)l;
+    const uint32_t b = T.leaf_start[l], e = T.leaf_start[l + 1];
+    if (e <= b) return;
+    const bool all = on && wb::shape_in_cube(S, lc[0], lc[1], lc[2], lc[3]);
+    if (all) cnt += e - b;
+    const bool rest = on && !all;
+    if (!__ballot(rest)) return;
+    const uint32_t g1 = (e - 1) / WB_CHUNK;
+    for (uint32_t g = b / WB_CHUNK; g <= g1; g++) {
+      const double *bx = T.chunk_box + 6 * (size_t)g;
+      const bool hit = rest && wb::shape_may_hit_box(S, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+      uint64_t own = __ballot(hit);
+      if (!own) continue;
+      const uint32_t base = g * WB_CHUNK;
+      const uint32_t j = base + (lane & (WB_CHUNK - 1));
+      const bool valid = j >= b && j < e;
+      double x = 0, y = 0, z = 0;
+      if (valid) {
+        x = T.px[j];
+        y = T.py[j];
+        z = T.pz[j];
+      }
+      while (own) {
+        const uint32_t o1 = (uint32_t)__ffsll((long long)own) - 1;
+        own &= own - 1;
+        const bool two = own != 0;
+        const uint32_t o2 = two ? (uint32_t)__ffsll((long long)own) - 1 : o1;
+        own &= own - 1;
+        const wb::Shape A = lane_shape(S, o1), B = lane_shape(S, o2);
+        const wb::Shape &H = hi_half ? B : A;
+        const uint64_t in = __ballot(valid && wb::shape_in(H, x, y, z));
+        if (lane == o1) cnt += __popcll(in & 0xffffffffull);
+        if (two && lane == o2) cnt += __popcll(in >> 32);
+      }
+    }
+  });
+  if (active) counts[si] = cnt;
+}
+
+// pointsIn of shapes order[64w .. 64w+63] by wave w: each shape's input
+// indices from offs[shape] on, leaf by leaf in DFS order and in canonical
+// (input) order inside a leaf, as k_points_in writes them.  A leaf is read
+// in canonical order 64 points at a time; each such group is tested against
+// the shape of every lane that takes the leaf (one owner per step, its
+// indices written by consecutive lanes).
+template <int KIND>
+__global__ void __launch_bounds__(256) k_query_points(TreeView T, const double *params,
+                                                      const uint32_t *order, uint64_t ns,
+                                                      const uint64_t *offs, uint32_t *out)
+{
+  __shared__ QueryStack stk[4];
+  QueryStack &K = stk[threadIdx.x / WAVE];
+  const uint32_t lane = __lane_id();
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = q < ns;
+  const uint32_t si = active ? order[q] : 0u;
+  const wb::Shape S = query_shape<KIND>(params + WB_SHAPE_NPARAMS * (uint64_t)si);
+  // the writes stay inside the shape's own range whatever the walk finds
+  uint64_t pos = active ? offs[si] : 0;
+  const uint64_t lim = active ? offs[si + 1] : 0;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  query_walk(T, S, active, K, [&](int l, bool on) {
+    const double *lc = T.leaf_cube + 4 * (size_t)l;
+    const uint32_t b = T.leaf_start[l], e = T.leaf_start[l + 1];
+    if (e <= b) return;
+    const bool all = on && wb::shape_in_cube(S, lc[0], lc[1], lc[2], lc[3]);
+    const bool rest = on && !all;
+    // bit g - g0: the lane's shape may reach 32-point group g (a leaf of at
+    // most 537 points spans at most 18 groups)
+    const uint32_t g0 = b / WB_CHUNK, g1 = (e - 1) / WB_CHUNK;
+    uint32_t gm = 0;
+    if (__ballot(rest)) {
+      for (uint32_t g = g0; g <= g1; g++) {
+        const double *bx = T.chunk_box + 6 * (size_t)g;
+        if (rest && wb::shape_may_hit_box(S, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]))
+          gm |= 1u << (g - g0);
+      }
+    }
+    const uint64_t owners = __ballot(all || gm != 0);
+    if (!owners) return;
+    for (uint32_t c0 = b; c0 < e; c0 += WAVE) {
+      const uint32_t c = c0 + lane;
+      const bool valid = c < e;
+      uint32_t slot = 0, id = 0;
+      double x = 0, y = 0, z = 0;
+      if (valid) {
+        slot = T.canon[c];
+        id = T.sidx[slot];
+        x = T.px[slot];
+        y = T.py[slot];
+        z = T.pz[slot];
+      }
+      const uint32_t gb = valid ? slot / WB_CHUNK - g0 : 0;
+      uint64_t own = owners;
+      while (own) {
+        const uint32_t o = (uint32_t)__ffsll((long long)own) - 1;
+        own &= own - 1;
+        const bool ao = __builtin_amdgcn_readlane((int)all, (int)o) != 0;
+        const uint32_t go = (uint32_t)__builtin_amdgcn_readlane((int)gm, (int)o);
+        bool in = valid && ao;
+        if (!ao) {
+          const wb::Shape H = lane_shape(S, o);
+          in = valid && ((go >> gb) & 1u) && wb::shape_in(H, x, y, z);
+        }
+        const uint64_t bal = __ballot(in);
+        if (!bal) continue;
+        const uint64_t p0 = rdl64(pos, o), l0 = rdl64(lim, o);
+        const uint64_t w = p0 + (uint64_t)__popcll(bal & lt);
+        if (in && w < l0) out[w] = id;
+        if (lane == o) pos = p0 + (uint64_t)__popcll(bal);
+      }
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------
 // Tiles
 // ---------------------------------------------------------------------------
 struct TileGeo {
@@ -2300,19 +2588,63 @@ __global__ void __launch_bounds__(256) k_sum_hist(const uint32_t *partial, uint3
 // ---------------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------------
-void wb_run_count(wb_tree *t, int kind, const double *dparams, uint64_t ns, uint64_t *dcounts)
+uint32_t *wb_query_order(wb_tree *t, int kind, const double *dparams, uint64_t ns)
+{
+  wb_ctx *c = t->ctx;
+  uint64_t *key = (uint64_t *)wb_alloc(c, 8 * ns);
+  uint32_t *val = (uint32_t *)wb_alloc(c, 4 * ns);
+  const double h = t->side / 2;
+  hipLaunchKernelGGL(k_query_keys, dim3(wb_blocks(ns, 256)), dim3(256), 0, c->stream, dparams, ns,
+                     kind, t->center[0] - h, t->center[1] - h, t->center[2] - h,
+                     (double)(1u << WB_MAX_LEVEL) / t->side, key, val);
+  // the top 32 key bits (cells of side/1024) give the locality that matters
+  wb_radix_sort_pairs(c, key, val, ns, 4);
+  wb_free(c, key, 8 * ns);
+  return val;
+}
+
+void wb_run_count(wb_tree *t, int kind, const double *dparams, uint64_t ns, uint64_t *dcounts,
+                  const uint32_t *order)
 {
   if (ns == 0) return;
-  hipLaunchKernelGGL(k_count_points_in, dim3((unsigned)ns), dim3(64), 0, t->ctx->stream, view(t),
-                     kind, dparams, ns, dcounts);
+  hipStream_t st = t->ctx->stream;
+  if (!order) {
+    hipLaunchKernelGGL(k_count_points_in, dim3((unsigned)ns), dim3(64), 0, st, view(t), kind,
+                       dparams, ns, dcounts);
+    return;
+  }
+  const dim3 grid(wb_blocks(ns, 256)), blk(256);
+  if (kind == wb::SHAPE_CYLINDER)
+    hipLaunchKernelGGL(k_query_count<wb::SHAPE_CYLINDER>, grid, blk, 0, st, view(t), dparams,
+                       order, ns, dcounts);
+  else if (kind == wb::SHAPE_SPHERE)
+    hipLaunchKernelGGL(k_query_count<wb::SHAPE_SPHERE>, grid, blk, 0, st, view(t), dparams,
+                       order, ns, dcounts);
+  else
+    hipLaunchKernelGGL(k_query_count<wb::SHAPE_HYPERBOLOID>, grid, blk, 0, st, view(t), dparams,
+                       order, ns, dcounts);
 }
 
 void wb_run_points(wb_tree *t, int kind, const double *dparams, uint64_t ns, const uint64_t *doffs,
-                   uint32_t *dout)
+                   uint32_t *dout, const uint32_t *order)
 {
   if (ns == 0) return;
-  hipLaunchKernelGGL(k_points_in, dim3((unsigned)ns), dim3(64), 0, t->ctx->stream, view(t), kind,
-                     dparams, ns, doffs, dout);
+  hipStream_t st = t->ctx->stream;
+  if (!order) {
+    hipLaunchKernelGGL(k_points_in, dim3((unsigned)ns), dim3(64), 0, st, view(t), kind, dparams,
+                       ns, doffs, dout);
+    return;
+  }
+  const dim3 grid(wb_blocks(ns, 256)), blk(256);
+  if (kind == wb::SHAPE_CYLINDER)
+    hipLaunchKernelGGL(k_query_points<wb::SHAPE_CYLINDER>, grid, blk, 0, st, view(t), dparams,
+                       order, ns, doffs, dout);
+  else if (kind == wb::SHAPE_SPHERE)
+    hipLaunchKernelGGL(k_query_points<wb::SHAPE_SPHERE>, grid, blk, 0, st, view(t), dparams,
+                       order, ns, doffs, dout);
+  else
+    hipLaunchKernelGGL(k_query_points<wb::SHAPE_HYPERBOLOID>, grid, blk, 0, st, view(t), dparams,
+                       order, ns, doffs, dout);
 }
 
 // The state of one tree's classification pass between its phases:

@@ -101,6 +101,9 @@ struct wb_tree {
   int sort_passes, maxdepth;
   uint32_t nleaves, nnodes;
   float ms_build, ms_sort;  // device time of the last build (events)
+  // device time of the last batched query's kernels (counting, and the
+  // index pass of wb_points_in; 0 when it had none)
+  float ms_query_count, ms_query_points;
   // the build's timing events (start, sort start, sort end, end), read
   // once the stream has passed them; null when there are none
   hipEvent_t evb[4];
@@ -169,9 +172,16 @@ struct ClassifyOut {
 };
 
 // ---- implemented in wb_classify.hip ----
-void wb_run_count(wb_tree *t, int kind, const double *dparams, uint64_t ns, uint64_t *dcounts);
+// batched shape queries: with order (a permutation of the shapes, from
+// wb_query_order) a lane per shape in that order; without, a wave per shape
+// (no allocation)
+void wb_run_count(wb_tree *t, int kind, const double *dparams, uint64_t ns, uint64_t *dcounts,
+                  const uint32_t *order);
 void wb_run_points(wb_tree *t, int kind, const double *dparams, uint64_t ns, const uint64_t *doffs,
-                   uint32_t *dout);
+                   uint32_t *dout, const uint32_t *order);
+// the shapes in Morton order of their centres (ns indices, allocated on the
+// context: wb_free(c, order, 4 * ns))
+uint32_t *wb_query_order(wb_tree *t, int kind, const double *dparams, uint64_t ns);
 // the classification pass (tile geometry, membership, SCAN, POSTSCAN and,
 // unless scan_only, CLASSIFY); class bytes to d_class_out in input order
 // several trees of one context classified together, their ground tests in
