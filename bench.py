@@ -363,15 +363,38 @@ def serving_summary(nreq):
         return {"error": repr(e)}
 
 
+def _points_in_flat(tree, kind, P):
+    """wb_points_in through the C ABI: (offsets, flat input indices)."""
+    import ctypes as C
+
+    import numpy as np
+
+    import wolkenbase_amd as W
+
+    L = W.lib()
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    pp = P.ctypes.data_as(C.POINTER(C.c_double))
+    offs = np.zeros(len(P) + 1, dtype=np.uint64)
+    op = offs.ctypes.data_as(C.POINTER(C.c_uint64))
+    tree.ctx.check(L.wb_points_in(tree.h, kind, pp, len(P), op, None, 0), "wb_points_in")
+    idx = np.empty(max(1, int(offs[-1])), dtype=np.uint32)
+    tree.ctx.check(L.wb_points_in(tree.h, kind, pp, len(P), op,
+                                  idx.ctypes.data_as(C.POINTER(C.c_uint32)), int(offs[-1])),
+                   "wb_points_in")
+    return offs, idx[:int(offs[-1])]
+
+
 def query_kernels(ctx, cloud, nq, seed=7):
     """Batched OctStore::countPointsIn / pointsIn (octree.cpp:1214-1264) over
     the tile's octree: nq spheres of radius 1 m and nq cylinders of radius
     0.877 m (the tile cylinder of a 1 m tiling of a 900 m cube) centred on
-    input points drawn at random (seeded), and the index pass of pointsIn
-    for the spheres.  Kernel times from the library's HIP events.
-    Algorithmic bytes per query: the 40-B shape in, the 8-B count out and
-    the 24-B FP64 coordinates of every point inside it (the least a kernel
-    reads to decide it); the index pass adds 4 B per index written."""
+    input points drawn at random (seeded); pointsIn for the spheres.  Kernel
+    times from the library's HIP events: the counting pass includes ordering
+    the shapes (k_query_keys and the radix sort of their Morton keys).
+    Algorithmic bytes of a batch: the 40-B shapes in, the 8-B counts out and
+    the 24-B FP64 coordinates of every point some shape of the batch holds,
+    read once (the union, from the batch's pointsIn); pointsIn adds 4 B per
+    index written."""
     import numpy as np
 
     import wolkenbase_amd as W
@@ -390,6 +413,7 @@ def query_kernels(ctx, cloud, nq, seed=7):
                              np.stack([c[0], c[1], zero + 0.877, zero, zero], 1)),
     }
     out = {}
+    n = tree.n_input
     try:
         for name, (kind, prm) in batches.items():
             tree.count_points_in((kind, prm))  # warm-up
@@ -400,27 +424,31 @@ def query_kernels(ctx, cloud, nq, seed=7):
                 wall.append(time.perf_counter() - t0)
                 ms.append(tree.query_ms()[0])
             kms = sorted(ms)[1]
-            inside = float(cnt.sum())
-            alg = 48.0 * nq + 24.0 * inside
+            offs, idx = _points_in_flat(tree, kind, prm)
+            pms = tree.query_ms()[1]
+            assert np.array_equal(np.diff(offs), cnt)
+            seen = np.zeros(n, dtype=bool)
+            seen[idx] = True
+            union = int(seen.sum())
+            del seen
+            alg = 48.0 * nq + 24.0 * union
             gbs = alg / (kms * 1e-3) / 1e9
-            out[name] = {"queries": nq, "mean_points_in": round(inside / nq, 2),
-                         "kernel": "k_count_points_in", "kernel_ms": round(kms, 3),
+            out[name] = {"queries": nq, "mean_points_in": round(float(cnt.sum()) / nq, 2),
+                         "distinct_points": union,
+                         "kernel": "k_query_count", "kernel_ms": round(kms, 3),
                          "queries_per_s": round(nq / (kms * 1e-3), 1),
                          "call_ms_host_in_out": round(1e3 * sorted(wall)[1], 3),
                          "algorithmic_bytes": alg, "gbs": round(gbs, 1),
                          "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 5)}
-        kind, prm = batches["sphere_r1m"]
-        npq = min(nq, 1 << 18)
-        idx = tree.points_in((kind, prm[:npq]))
-        pms = tree.query_ms()[1]
-        total = float(sum(len(a) for a in idx))
-        alg = 48.0 * npq + 28.0 * total
-        gbs = alg / (pms * 1e-3) / 1e9
-        out["points_in_sphere_r1m"] = {"queries": npq, "indices": int(total),
-                                       "kernel": "k_points_in", "kernel_ms": round(pms, 3),
-                                       "queries_per_s": round(npq / (pms * 1e-3), 1),
-                                       "algorithmic_bytes": alg, "gbs": round(gbs, 1),
-                                       "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 5)}
+            if kind == W.WB_SHAPE_SPHERE:
+                palg = 48.0 * nq + 24.0 * union + 4.0 * len(idx)
+                pgbs = palg / (pms * 1e-3) / 1e9
+                out["points_in_" + name] = {
+                    "queries": nq, "indices": int(len(idx)), "kernel": "k_query_points",
+                    "kernel_ms": round(pms, 3), "queries_per_s": round(nq / (pms * 1e-3), 1),
+                    "algorithmic_bytes": palg, "gbs": round(pgbs, 1),
+                    "frac_of_8TBs": round(pgbs / HBM_PEAK_GBS, 5)}
+            del idx, offs
     finally:
         tree.free()
     return out

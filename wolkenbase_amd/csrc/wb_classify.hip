@@ -55,6 +55,17 @@ __device__ inline double rdl(double v, uint32_t l)
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// The device hands consecutive workgroups to its 8 XCDs in turn, and each
+// XCD has its own L2.  xcd_block(b, nb) renumbers the nb blocks of a grid so
+// that the blocks one XCD runs take one contiguous range of the work: kernels
+// whose neighbouring blocks read the same leaves then find them in that XCD's
+// L2.  A permutation of [0, nb): results do not depend on it.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb)
+{
+  const uint32_t x = b % 8, per = nb / 8, extra = nb % 8;
+  return x * per + (x < extra ? x : extra) + b / 8;
+}
+
 struct TreeView {
   const double *px, *py, *pz;
   const uint32_t *leaf_start;
@@ -363,7 +374,7 @@ __global__ void __launch_bounds__(256) k_query_count(TreeView T, const double *p
   __shared__ QueryStack stk[4];
   QueryStack &K = stk[threadIdx.x / WAVE];
   const uint32_t lane = __lane_id();
-  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t q = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const bool active = q < ns;
   const uint32_t si = active ? order[q] : 0u;
   const wb::Shape S = query_shape<KIND>(params + WB_SHAPE_NPARAMS * (uint64_t)si);
@@ -423,7 +434,7 @@ __global__ void __launch_bounds__(256) k_query_points(TreeView T, const double *
   __shared__ QueryStack stk[4];
   QueryStack &K = stk[threadIdx.x / WAVE];
   const uint32_t lane = __lane_id();
-  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t q = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const bool active = q < ns;
   const uint32_t si = active ? order[q] : 0u;
   const wb::Shape S = query_shape<KIND>(params + WB_SHAPE_NPARAMS * (uint64_t)si);
@@ -438,10 +449,13 @@ __global__ void __launch_bounds__(256) k_query_points(TreeView T, const double *
     const bool all = on && wb::shape_in_cube(S, lc[0], lc[1], lc[2], lc[3]);
     const bool rest = on && !all;
     // bit g - g0: the lane's shape may reach 32-point group g (a leaf of at
-    // most 537 points spans at most 18 groups)
+    // most 537 points spans at most 18 groups; a longer one is not pruned)
     const uint32_t g0 = b / WB_CHUNK, g1 = (e - 1) / WB_CHUNK;
+    const bool wide = g1 - g0 >= 32;
     uint32_t gm = 0;
-    if (__ballot(rest)) {
+    if (wide) {
+      gm = rest ? ~0u : 0u;
+    } else if (__ballot(rest)) {
       for (uint32_t g = g0; g <= g1; g++) {
         const double *bx = T.chunk_box + 6 * (size_t)g;
         if (rest && wb::shape_may_hit_box(S, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]))
@@ -462,7 +476,7 @@ __global__ void __launch_bounds__(256) k_query_points(TreeView T, const double *
         y = T.py[slot];
         z = T.pz[slot];
       }
-      const uint32_t gb = valid ? slot / WB_CHUNK - g0 : 0;
+      const uint32_t gb = valid && !wide ? slot / WB_CHUNK - g0 : 0;
       uint64_t own = owners;
       while (own) {
         const uint32_t o = (uint32_t)__ffsll((long long)own) - 1;
