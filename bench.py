@@ -524,11 +524,15 @@ def main():
     passes = int(st["sort_passes"])
     # algorithmic bytes per launch (compulsory traffic per point): the ground
     # test reads each point once (12 B as int32) and writes its class byte;
-    # the sort moves (u64 key, u32 index) records, 24 B per point per pass,
-    # plus the 8-B key read of each digit histogram.
+    # the sort moves its (key, u32 index) records, read and written once per
+    # pass, plus the key read of each digit histogram: a build of at most 4
+    # passes without the duplicate merge sorts u32 keys (16 + 4 B per point
+    # and pass), a deeper one u64 keys (24 + 8 B)
     nh = int(st["heavy_points"])
+    kb = 4 if passes <= 4 else 8
+    sort_bpp = 2.0 * (kb + 4) + kb
     alg = {"k_ground": 13.0 * n, "k_ground_heavy": 13.0 * nh,
-           "radix_sort": (24.0 + 8.0) * passes * n}
+           "radix_sort": sort_bpp * passes * n}
     achieved = alg[dominant] / (kern[dominant] * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{dominant}_{args.workload}.txt")
@@ -572,11 +576,11 @@ def main():
                         "atan_x87_steps": int(st["atan_exact_calls"]),
                         "sector_borderline": int(st["sector_borderline"])},
     }
-    # the radix sort (K2) against the same peak: 24 B per point and pass
-    # moved, plus the 8-B key read of each pass's digit histogram
-    sort_alg = (24.0 + 8.0) * passes * n
+    # the radix sort (K2) against the same peak
+    sort_alg = sort_bpp * passes * n
     sort_gbs = sort_alg / (kern["radix_sort"] * 1e-3) / 1e9
-    out["kernels"] = {"radix_sort": {"passes": passes, "ms": round(kern["radix_sort"], 3),
+    out["kernels"] = {"radix_sort": {"passes": passes, "key_bytes": kb,
+                                     "ms": round(kern["radix_sort"], 3),
                                      "algorithmic_bytes": sort_alg, "gbs": round(sort_gbs, 1),
                                      "frac_of_8TBs": round(sort_gbs / HBM_PEAK_GBS, 5)}}
     if args.workload == "tile20m" and not args.no_queries:

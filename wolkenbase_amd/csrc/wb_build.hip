@@ -305,6 +305,11 @@ void wb_radix_sort_pairs(wb_ctx *c, uint64_t *key, uint32_t *val, uint64_t n, in
   radix_sort_impl<uint64_t>(c, key, val, n, 64 - 8 * passes, passes);
 }
 
+void wb_radix_sort_u32_prefix(wb_ctx *c, uint32_t *key, uint32_t *val, uint64_t n, int passes)
+{
+  radix_sort_impl<uint32_t>(c, key, val, n, 32 - 8 * passes, passes);
+}
+
 void wb_radix_sort_u32_pairs(wb_ctx *c, uint32_t *key, uint32_t *val, uint64_t n)
 {
   radix_sort_impl<uint32_t>(c, key, val, n, 0, 4);
@@ -387,8 +392,12 @@ __device__ inline void load_packed(const Src &s, const Xform &f, uint64_t i, dou
   }
 }
 
-template <bool F64>
-__global__ void k_keygen(Src src, uint64_t n, Xform f, uint64_t *key, uint32_t *val,
+// K = uint64_t: the whole Morton key; K = uint32_t: its top 32 bits, which
+// are all a build that sorts at most 4 digits (32 bits) and merges no
+// duplicates reads after the sort (the split reads the top 3 bits per level
+// of at most 10 levels)
+template <bool F64, typename K>
+__global__ void k_keygen(Src src, uint64_t n, Xform f, K *key, uint32_t *val,
                          uint32_t *outside)
 {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -408,7 +417,7 @@ __global__ void k_keygen(Src src, uint64_t n, Xform f, uint64_t *key, uint32_t *
   uint32_t iy = cell_index(y, f.loy, f.cell, f.inv_cell);
   uint32_t iz = cell_index(z, f.loz, f.cell, f.inv_cell);
   uint64_t k = (spread3(iz) << 2) | (spread3(iy) << 1) | spread3(ix);
-  key[i] = k << 1;
+  key[i] = (K)((k << 1) >> (64 - 8 * sizeof(K)));
   val[i] = (uint32_t)i;
 }
 
@@ -434,7 +443,12 @@ struct SplitOut {
   uint32_t n_next, n_leaf, n_node, overflow, maxdepth;
 };
 
-__device__ inline uint32_t digit_at(uint64_t k, int level) { return (uint32_t)(k >> (64 - 3 * level)) & 7u; }
+// the level-`level` digit of a key whose top bit is bit 8*sizeof(K)-1
+template <typename K>
+__device__ inline uint32_t digit_at(K k, int level)
+{
+  return (uint32_t)(k >> (8 * (int)sizeof(K) - 3 * level)) & 7u;
+}
 
 // The levels are launched back to back: the number of active nodes of a
 // level is the counter the previous level incremented (act_cnt), and a level
@@ -447,7 +461,8 @@ __device__ inline uint32_t digit_at(uint64_t k, int level) { return (uint32_t)(k
 
 // One thread per active node (levels with many active nodes, each with a
 // short range): seven binary searches for the boundaries.
-__global__ void k_split_level_t(const uint64_t *key, const Active *act, const uint32_t *act_cnt,
+template <typename K>
+__global__ void k_split_level_t(const K *key, const Active *act, const uint32_t *act_cnt,
                               uint32_t act_cap, int level, int max_level, Active *next,
                               uint32_t *next_cnt, uint32_t next_cap, LeafTmp *leaves,
                               uint32_t leaf_cap, double *node_cube, int32_t *node_child,
@@ -539,8 +554,9 @@ __global__ void k_split_level_t(const uint64_t *key, const Active *act, const ui
 // node's sorted keys: 64 evenly spaced samples place every boundary between
 // two neighbouring samples, then lanes 8(c-1)..8(c-1)+7 narrow boundary c by
 // a 9-way search (8 probes per round).  Lanes 0..7 then make child c = lane.
+template <typename K>
 __global__ void __launch_bounds__(256) k_split_level(
-    const uint64_t *key, const Active *act, const uint32_t *act_cnt, uint32_t act_cap, int level,
+    const K *key, const Active *act, const uint32_t *act_cnt, uint32_t act_cap, int level,
     int max_level, Active *next, uint32_t *next_cnt, uint32_t next_cap, LeafTmp *leaves,
     uint32_t leaf_cap, double *node_cube, int32_t *node_child, int32_t *node_parent,
     uint32_t node_cap, SplitOut *out)
@@ -1002,17 +1018,33 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
     t->ndup = 0;
     t->last_of = nullptr;
     t->dup_pairs = nullptr;
+    // a build of at most 4 sort digits that merges no duplicates reads only
+    // the top 32 key bits after keygen, so it sorts (u32 key, u32 index)
+    // pairs: 8 B per point moved per pass instead of 12
+    const bool narrow = !dedupe && passes <= 4;
+    uint32_t *key32 = (uint32_t *)key0;
     if (n > 0) {
       WB_CHECK(hipEventRecord(evb[0], c->stream));
       WB_CHECK(hipMemcpyAsync(d_outside, &zero_word, 4, hipMemcpyHostToDevice, c->stream));
-      if (f64)
-        hipLaunchKernelGGL(k_keygen<true>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, src,
-                           n, f, t->key, t->sidx, d_outside);
-      else
-        hipLaunchKernelGGL(k_keygen<false>, dim3(wb_blocks(n, 256)), dim3(256), 0, c->stream, src,
-                           n, f, t->key, t->sidx, d_outside);
+      const dim3 kg(wb_blocks(n, 256)), kb(256);
+      if (narrow) {
+        if (f64)
+          hipLaunchKernelGGL((k_keygen<true, uint32_t>), kg, kb, 0, c->stream, src, n, f, key32,
+                             t->sidx, d_outside);
+        else
+          hipLaunchKernelGGL((k_keygen<false, uint32_t>), kg, kb, 0, c->stream, src, n, f, key32,
+                             t->sidx, d_outside);
+      } else {
+        if (f64)
+          hipLaunchKernelGGL((k_keygen<true, uint64_t>), kg, kb, 0, c->stream, src, n, f, t->key,
+                             t->sidx, d_outside);
+        else
+          hipLaunchKernelGGL((k_keygen<false, uint64_t>), kg, kb, 0, c->stream, src, n, f, t->key,
+                             t->sidx, d_outside);
+      }
       WB_CHECK(hipEventRecord(evb[1], c->stream));
-      wb_radix_sort_pairs(c, t->key, t->sidx, n, passes);
+      if (narrow) wb_radix_sort_u32_prefix(c, key32, t->sidx, n, passes);
+      else wb_radix_sort_pairs(c, t->key, t->sidx, n, passes);
       WB_CHECK(hipEventRecord(evb[2], c->stream));
       if (dedupe) {
         const int shift = 64 - 8 * passes;
@@ -1072,16 +1104,28 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
     }
     WB_CHECK(hipMemcpyAsync(d_cnt, hcnt, sizeof hcnt, hipMemcpyHostToDevice, c->stream));
     for (int level = 0; n > 0 && level < max_level; level++) {
-      hipLaunchKernelGGL(k_split_level, dim3(std::min<uint32_t>(wb_blocks(act_ub, 4), 512)),
-                         dim3(256), 0, c->stream,
-                         t->key, act, d_cnt + level, node_cap, level, max_level, nxt,
-                         d_cnt + level + 1, node_cap, lt, leaf_cap, t->node_cube, t->node_child,
-                         t->node_parent, node_cap, d_out);
-      if (act_ub > SPLIT_WAVE_MAX)
-        hipLaunchKernelGGL(k_split_level_t, dim3(wb_blocks(act_ub, 128)), dim3(128), 0, c->stream,
-                           t->key, act, d_cnt + level, node_cap, level, max_level, nxt,
-                           d_cnt + level + 1, node_cap, lt, leaf_cap, t->node_cube,
-                           t->node_child, t->node_parent, node_cap, d_out);
+      const dim3 gw(std::min<uint32_t>(wb_blocks(act_ub, 4), 512)), gt(wb_blocks(act_ub, 128));
+      if (narrow) {
+        hipLaunchKernelGGL(k_split_level<uint32_t>, gw, dim3(256), 0, c->stream, key32, act,
+                           d_cnt + level, node_cap, level, max_level, nxt, d_cnt + level + 1,
+                           node_cap, lt, leaf_cap, t->node_cube, t->node_child, t->node_parent,
+                           node_cap, d_out);
+        if (act_ub > SPLIT_WAVE_MAX)
+          hipLaunchKernelGGL(k_split_level_t<uint32_t>, gt, dim3(128), 0, c->stream, key32, act,
+                             d_cnt + level, node_cap, level, max_level, nxt, d_cnt + level + 1,
+                             node_cap, lt, leaf_cap, t->node_cube, t->node_child,
+                             t->node_parent, node_cap, d_out);
+      } else {
+        hipLaunchKernelGGL(k_split_level<uint64_t>, gw, dim3(256), 0, c->stream, t->key, act,
+                           d_cnt + level, node_cap, level, max_level, nxt, d_cnt + level + 1,
+                           node_cap, lt, leaf_cap, t->node_cube, t->node_child, t->node_parent,
+                           node_cap, d_out);
+        if (act_ub > SPLIT_WAVE_MAX)
+          hipLaunchKernelGGL(k_split_level_t<uint64_t>, gt, dim3(128), 0, c->stream, t->key, act,
+                             d_cnt + level, node_cap, level, max_level, nxt, d_cnt + level + 1,
+                             node_cap, lt, leaf_cap, t->node_cube, t->node_child,
+                             t->node_parent, node_cap, d_out);
+      }
       std::swap(act, nxt);
     }
     WB_CHECK(hipMemcpyAsync(&h_out, d_out, sizeof h_out, hipMemcpyDeviceToHost, c->stream));

@@ -199,6 +199,8 @@ void wb_run_classify(wb_tree *t, const double *corners, int ncorners,
 uint64_t wb_exclusive_scan_u32(wb_ctx *c, uint32_t *data, uint64_t n);
 // LSD radix sort of (u64 key, u32 val) pairs on bits [64-8*passes, 64)
 void wb_radix_sort_pairs(wb_ctx *c, uint64_t *key, uint32_t *val, uint64_t n, int passes);
+// LSD radix sort of (u32 key, u32 val) pairs on bits [32-8*passes, 32)
+void wb_radix_sort_u32_prefix(wb_ctx *c, uint32_t *key, uint32_t *val, uint64_t n, int passes);
 // LSD radix sort of (u32 key, u32 val) pairs on all 32 bits
 void wb_radix_sort_u32_pairs(wb_ctx *c, uint32_t *key, uint32_t *val, uint64_t n);
 
