@@ -329,14 +329,16 @@ __device__ inline uint64_t spread3(uint32_t v)
 }
 
 __global__ void k_query_keys(const double *params, uint64_t ns, int kind, double lox, double loy,
-                             double loz, double inv_cell, uint64_t *key, uint32_t *val)
+                             double loz, double inv_cell, uint32_t *key, uint32_t *val)
 {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns) return;
   const double *p = params + WB_SHAPE_NPARAMS * i;
   const uint32_t cx = query_cell(p[0] - lox, inv_cell), cy = query_cell(p[1] - loy, inv_cell);
   const uint32_t cz = kind == wb::SHAPE_CYLINDER ? 0u : query_cell(p[2] - loz, inv_cell);
-  key[i] = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+  // the top 32 of the 63 key bits (cells of side/1024 in x and y) give the
+  // locality that matters
+  key[i] = (uint32_t)((spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2)) >> 31);
   val[i] = (uint32_t)i;
 }
 
@@ -1264,9 +1266,15 @@ __device__ __attribute__((always_inline)) inline void scan_reg(const TreeView &T
 // ncls[c] the lengths), so each class kernel visits only its own tiles.
 // The order inside a list does not matter: every tile's record depends on
 // that tile alone.
-__global__ void k_scan_classes(uint32_t ntiles, const uint32_t *tile_cnt, uint32_t *la,
-                               uint32_t *lb, uint32_t *ncls, TileRec *rec)
+__global__ void __launch_bounds__(1024) k_scan_classes(uint32_t ntiles, const uint32_t *tile_cnt,
+                                                      uint32_t *la, uint32_t *lb, uint32_t *ncls,
+                                                      TileRec *rec)
 {
+  // slots are claimed per block: the waves' class counts summed in LDS, one
+  // atomic per class and block (atomics on the four list counters are
+  // serialised, so one per wave would cost more than the pass)
+  __shared__ uint32_t wcnt[4][1024 / WAVE];
+  __shared__ uint32_t bbase[4];
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t m = t < ntiles ? tile_cnt[t] : 0u;
   if (t < ntiles && m == 0) {
@@ -1274,22 +1282,31 @@ __global__ void k_scan_classes(uint32_t ntiles, const uint32_t *tile_cnt, uint32
     rec[t] = r;
   }
   const int c = m == 0 ? -1 : m <= WAVE ? 0 : m <= 2 * WAVE ? 1 : m <= 4 * WAVE ? 2 : 3;
-  const uint32_t lane = __lane_id();
+  const uint32_t lane = __lane_id(), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+  uint64_t mk[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    const uint64_t mask = __ballot(c == k);
-    if (!mask) continue;
-    const int leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(&ncls[k], (uint32_t)__popcll(mask));
-    base = (uint32_t)__shfl((int)base, leader, WAVE);
-    if (c != k) continue;
-    const uint32_t pos = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
-    if (k == 0) la[pos] = t;
-    else if (k == 1) lb[pos] = t;
-    else if (k == 2) lb[ntiles - 1 - pos] = t;
-    else la[ntiles - 1 - pos] = t;
+    mk[k] = __ballot(c == k);
+    if (lane == 0) wcnt[k][w] = (uint32_t)__popcll(mk[k]);
   }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = threadIdx.x;
+    uint32_t tot = 0;
+    for (uint32_t v = 0; v < nw; v++) {
+      const uint32_t x = wcnt[k][v];
+      wcnt[k][v] = tot;  // exclusive prefix over the block's waves
+      tot += x;
+    }
+    bbase[k] = tot ? atomicAdd(&ncls[k], tot) : 0u;
+  }
+  __syncthreads();
+  if (c < 0) return;
+  const uint32_t pos = bbase[c] + wcnt[c][w] + (uint32_t)__popcll(mk[c] & ((1ull << lane) - 1));
+  if (c == 0) la[pos] = t;
+  else if (c == 1) lb[pos] = t;
+  else if (c == 2) lb[ntiles - 1 - pos] = t;
+  else la[ntiles - 1 - pos] = t;
 }
 
 // A class list: entry i at list[step * i] (step +1 for a list filled from
@@ -1460,10 +1477,17 @@ __global__ void __launch_bounds__(64) k_scan(TreeView T, TileGeo g, const int32_
 __global__ void k_postscan(TileGeo g, const int32_t *tex, const int32_t *tey, TileRec *rec,
                            unsigned long long *nonempty)
 {
+  // the non-empty tiles counted per block (one atomic on the counter per
+  // block: atomics on one address are serialised)
+  __shared__ uint32_t bcount;
+  if (threadIdx.x == 0) bcount = 0;
+  __syncthreads();
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = t < g.ntiles && rec[t].npoints;
   const uint64_t lv = __ballot(live);
-  if (__lane_id() == 0 && lv) atomicAdd(nonempty, (unsigned long long)__popcll(lv));
+  if (__lane_id() == 0 && lv) atomicAdd(&bcount, (uint32_t)__popcll(lv));
+  __syncthreads();
+  if (threadIdx.x == 0 && bcount) atomicAdd(nonempty, (unsigned long long)bcount);
   if (!live) return;
   TileRec me = rec[t];
   const int r1x[6] = {1, 1, 0, -1, -1, 0}, r1y[6] = {0, 1, 1, 0, -1, -1};  // root1, eisenstein.cpp:51
@@ -2605,15 +2629,14 @@ __global__ void __launch_bounds__(256) k_sum_hist(const uint32_t *partial, uint3
 uint32_t *wb_query_order(wb_tree *t, int kind, const double *dparams, uint64_t ns)
 {
   wb_ctx *c = t->ctx;
-  uint64_t *key = (uint64_t *)wb_alloc(c, 8 * ns);
+  uint32_t *key = (uint32_t *)wb_alloc(c, 4 * ns);
   uint32_t *val = (uint32_t *)wb_alloc(c, 4 * ns);
   const double h = t->side / 2;
   hipLaunchKernelGGL(k_query_keys, dim3(wb_blocks(ns, 256)), dim3(256), 0, c->stream, dparams, ns,
                      kind, t->center[0] - h, t->center[1] - h, t->center[2] - h,
                      (double)(1u << WB_MAX_LEVEL) / t->side, key, val);
-  // the top 32 key bits (cells of side/1024) give the locality that matters
-  wb_radix_sort_pairs(c, key, val, ns, 4);
-  wb_free(c, key, 8 * ns);
+  wb_radix_sort_u32_prefix(c, key, val, ns, 4);
+  wb_free(c, key, 4 * ns);
   return val;
 }
 
@@ -2810,7 +2833,7 @@ static void classify_prepare(ClassifyRun &R, wb_tree *t, wb_ctx *c, const double
     const unsigned nb = (unsigned)std::min<uint64_t>(g.ntiles, 1u << 16);
     const int prec = p->fit_precision;
     if (nb) {
-      hipLaunchKernelGGL(k_scan_classes, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st,
+      hipLaunchKernelGGL(k_scan_classes, dim3(wb_blocks(g.ntiles, 1024)), dim3(1024), 0, st,
                          g.ntiles, tile_cnt, la, lb, ncls, rec);
       const ClassList c1 = {la, 1, ncls}, c2 = {lb, 1, ncls + 1};
       const ClassList c4 = {lb + g.ntiles - 1, -1, ncls + 2}, cbig = {la + g.ntiles - 1, -1, ncls + 3};
