@@ -785,7 +785,8 @@ __global__ void k_node_zlow(const int32_t *node_child, const double *node_cube, 
 // ---------------------------------------------------------------------------
 template <bool F64>
 __global__ void k_permute(Src src, const uint32_t *sidx, uint64_t n, Xform f, double *px,
-                          double *py, double *pz, double *box, uint32_t *low)
+                          double *py, double *pz, double *box, uint32_t *low, double *lowx,
+                          double *lowy, double *lowz)
 {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool ok = i < n;
@@ -801,15 +802,18 @@ __global__ void k_permute(Src src, const uint32_t *sidx, uint64_t n, Xform f, do
   double x0 = ok ? x : __builtin_inf(), x1 = ok ? x : -__builtin_inf();
   double y0 = ok ? y : __builtin_inf(), y1 = ok ? y : -__builtin_inf();
   double z0 = ok ? z : __builtin_inf(), z1 = ok ? z : -__builtin_inf();
-  double lz = z0;
+  double lz = z0, lx = x, ly = y;
   uint32_t li = (uint32_t)i;
 #pragma unroll
   for (int d = 1; d < WB_CHUNK; d <<= 1) {
     const double oz = __shfl_xor(lz, d, WAVE);
     const uint32_t oi = __shfl_xor(li, d, WAVE);
+    const double ox = __shfl_xor(lx, d, WAVE), oy = __shfl_xor(ly, d, WAVE);
     if (oz < lz || (oz == lz && oi < li)) {
       lz = oz;
       li = oi;
+      lx = ox;
+      ly = oy;
     }
     x0 = fmin(x0, __shfl_xor(x0, d, WAVE));
     x1 = fmax(x1, __shfl_xor(x1, d, WAVE));
@@ -823,6 +827,9 @@ __global__ void k_permute(Src src, const uint32_t *sidx, uint64_t n, Xform f, do
     double *b = box + 6 * chunk;
     b[0] = x0; b[1] = x1; b[2] = y0; b[3] = y1; b[4] = z0; b[5] = z1;
     low[chunk] = li;
+    lowx[chunk] = lx;
+    lowy[chunk] = ly;
+    lowz[chunk] = lz;
   }
 }
 
@@ -1200,17 +1207,20 @@ void wb_build_tree(wb_tree *t, const int32_t *dx, const int32_t *dy, const int32
   uint64_t nchunks = (n + WB_CHUNK - 1) / WB_CHUNK;
   t->chunk_box = wb_talloc<double>(t, 6 * nchunks + 6);
   t->chunk_low = wb_talloc<uint32_t>(t, nchunks + 1);
+  t->chunk_sx = wb_talloc<double>(t, nchunks + 1);
+  t->chunk_sy = wb_talloc<double>(t, nchunks + 1);
+  t->chunk_sz = wb_talloc<double>(t, nchunks + 1);
   if (n > 0) {
     hipLaunchKernelGGL(k_fill_pt_leaf, dim3(nl), dim3(256), 0, c->stream, t->leaf_start, nl,
                        t->pt_leaf);
     if (f64)
       hipLaunchKernelGGL(k_permute<true>, dim3(wb_blocks(nchunks * WB_CHUNK, 256)), dim3(256), 0,
                          c->stream, src, t->sidx, n, f, t->px, t->py, t->pz, t->chunk_box,
-                         t->chunk_low);
+                         t->chunk_low, t->chunk_sx, t->chunk_sy, t->chunk_sz);
     else
       hipLaunchKernelGGL(k_permute<false>, dim3(wb_blocks(nchunks * WB_CHUNK, 256)), dim3(256), 0,
                          c->stream, src, t->sidx, n, f, t->px, t->py, t->pz, t->chunk_box,
-                         t->chunk_low);
+                         t->chunk_low, t->chunk_sx, t->chunk_sy, t->chunk_sz);
     wb_free(c, src.packed, pk_bytes);
     src.packed = nullptr;
     {

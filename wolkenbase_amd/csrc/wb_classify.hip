@@ -75,6 +75,7 @@ struct TreeView {
   const int32_t *node_child, *node_parent;
   const double *chunk_box;
   const uint32_t *chunk_low;
+  const double *chunk_sx, *chunk_sy, *chunk_sz;
   const uint2 *node_crange;
   const double *child_zlow;
   const uint32_t *pt_leaf, *canon, *crank, *sidx;
@@ -88,7 +89,8 @@ static TreeView view(const wb_tree *t)
   v.px = t->px; v.py = t->py; v.pz = t->pz;
   v.leaf_start = t->leaf_start; v.leaf_cube = t->leaf_cube; v.leaf_parent = t->leaf_parent;
   v.node_cube = t->node_cube; v.node_child = t->node_child; v.node_parent = t->node_parent;
-  v.chunk_box = t->chunk_box; v.chunk_low = t->chunk_low; v.node_crange = t->node_crange;
+  v.chunk_box = t->chunk_box; v.chunk_low = t->chunk_low;
+  v.chunk_sx = t->chunk_sx; v.chunk_sy = t->chunk_sy; v.chunk_sz = t->chunk_sz; v.node_crange = t->node_crange;
   v.child_zlow = t->child_zlow;
   v.pt_leaf = t->pt_leaf; v.canon = t->canon; v.crank = t->crank;
   v.sidx = t->sidx; v.nleaves = t->nleaves; v.nnodes = t->nnodes; v.n = t->n;
@@ -1959,9 +1961,16 @@ __device__ __forceinline__ void ground_body(const ClassifyArgs &A, uint64_t unit
           const uint32_t g = gb + lane;
           double sx = 0, sy = 0, sz = 0;
           if (g < ng) {
-            uint32_t j = T.chunk_low[g0 + g];
-            if (j < b || j >= e) j = g == 0 ? b : (g0 + g) * WB_CHUNK;
-            sx = T.px[j]; sy = T.py[j]; sz = T.pz[j];
+            // the group's lowest point from the per-group copy (consecutive
+            // lanes, consecutive words); a group that starts or ends in
+            // another leaf takes its first slot in this leaf instead
+            const uint32_t j = T.chunk_low[g0 + g];
+            if (j < b || j >= e) {
+              const uint32_t jf = g == 0 ? b : (g0 + g) * WB_CHUNK;
+              sx = T.px[jf]; sy = T.py[jf]; sz = T.pz[jf];
+            } else {
+              sx = T.chunk_sx[g0 + g]; sy = T.chunk_sy[g0 + g]; sz = T.chunk_sz[g0 + g];
+            }
           }
           if (COUNT) {
             __builtin_amdgcn_s_waitcnt(0);

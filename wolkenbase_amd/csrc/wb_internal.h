@@ -11,6 +11,8 @@
 //   chunk_box[ceil(n/32)]  f64x6  bounding box of each 32-slot group
 //   chunk_low[ceil(n/32)]  u32    slot of the group's lowest point (first of
 //                                 equal z)
+//   chunk_sx/sy/sz[ceil(n/32)] f64 that point's coordinates (the ground
+//                                 test's samples, read coalesced per leaf)
 //   leaf_start[nl+1] u32, leaf_cube[nl] f64x4 (cx,cy,cz,side), leaf_parent[nl]
 //   node_cube[nn] f64x4, node_child[8*nn] i32 (>=0 node, -2-l leaf, -1 empty),
 //   node_crange[8*nn] u32x2 (slot range of each leaf child; 0,0 otherwise),
@@ -115,6 +117,7 @@ struct wb_tree {
   uint32_t *pt_leaf, *canon, *crank;
   double *chunk_box;
   uint32_t *chunk_low;
+  double *chunk_sx, *chunk_sy, *chunk_sz;
   uint32_t *leaf_start;
   double *leaf_cube;
   int32_t *leaf_parent;
