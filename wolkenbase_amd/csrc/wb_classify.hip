@@ -597,25 +597,22 @@ __global__ void k_tile_grid(TileGeo g, const int32_t *tex, const int32_t *tey, i
 
 #define MAX_MEMB 4
 
-// cnt[key] += 1 for every active lane with `has`, one atomic per distinct key
-// in the wave; returns the lane's slot (the old count plus its rank among
-// the lanes with the same key)
-__device__ inline uint32_t wave_count_add(uint32_t *cnt, bool has, int key)
+// cnt[key] += 1 for every active lane with `has`: the peer masks first, then
+// one atomic per distinct key from its first lane, none of them waited for
+// (no slot is returned)
+__device__ inline void wave_count_inc(uint32_t *cnt, bool has, int key)
 {
-  uint32_t slot = 0;
-  uint64_t todo = __ballot(has);
+  const uint32_t lane = __lane_id();
+  uint64_t todo = __ballot(has), peers = 0;
   while (todo) {
-    const int leader = __ffsll((long long)todo) - 1;
-    const int kl = __shfl(key, leader, 64);
-    const bool same_key = has && key == kl;
-    const uint64_t same = __ballot(same_key);
-    uint32_t base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(&cnt[kl], (uint32_t)__popcll(same));
-    base = (uint32_t)__shfl((int)base, leader, 64);
-    if (same_key) slot = base + (uint32_t)__popcll(same & ((1ull << __lane_id()) - 1));
-    todo &= ~same;
+    const int kl = __builtin_amdgcn_readlane(key, __ffsll((long long)todo) - 1);
+    const bool same = has && key == kl;
+    const uint64_t m = __ballot(same);
+    if (same) peers = m;
+    todo &= ~m;
   }
-  return slot;
+  if (has && __ffsll((long long)peers) - 1 == (int)lane)
+    atomicAdd(&cnt[key], (uint32_t)__popcll(peers));
 }
 
 // The same claim with the atomics of all distinct keys issued together:
@@ -766,7 +763,7 @@ __global__ void k_membership(TreeView T, TileGeo g, const LeafWin *win, const ui
   for (int k = 0; k < MAX_MEMB; k++) memb[MAX_MEMB * i + k] = k < nm ? mem[k] : INT32_MIN;
   nmemb[i] = nm;
 #pragma unroll
-  for (int k = 0; k < MAX_MEMB; k++) wave_count_add(tile_cnt, k < nm, k < nm ? mem[k] - g.lo : 0);
+  for (int k = 0; k < MAX_MEMB; k++) wave_count_inc(tile_cnt, k < nm, k < nm ? mem[k] - g.lo : 0);
 }
 
 __global__ void k_tile_lists(uint64_t n, const int32_t *memb, const uint32_t *nmemb, int lo,
