@@ -441,6 +441,13 @@ def load_test(pool, slots, load, nreq, seed, device=0, policy="ratio", small_slo
                           "run_ms": round(1e3 * (r.t_end - r.t_start), 2), "batch": r.batch,
                           **{k + "_ms": round(1e3 * v, 2) for k, v in r.times.items()}}
                          for r in worst],
+        # the requests at or above the 99th latency percentile: their tile
+        # sizes, waits and run times
+        "p99_requests": [{"points": r.n, "isolated_ms": round(1e3 * iso_t[r.tag], 2),
+                          "latency_ms": round(1e3 * (r.t_end - r.t_arrive), 2),
+                          "wait_ms": round(1e3 * (r.t_start - r.t_arrive), 2)}
+                         for r in sorted(reqs, key=lambda r: -(r.t_end - r.t_arrive))
+                         [:max(1, len(reqs) // 100)]],
         "batched_requests": int((batches > 1).sum()),
         "mean_batch": round(float(batches.mean()), 2),
         "mismatched_results": mismatched,
