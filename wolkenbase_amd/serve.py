@@ -465,8 +465,11 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description="serving load test")
     ap.add_argument("--pool", type=int, default=128, help="distinct tiles (lognormal strata)")
     ap.add_argument("--max-points", type=float, default=4e7, help="cap on a pool tile")
-    ap.add_argument("--slots", type=int, default=4)
-    ap.add_argument("--small-slots", type=int, default=1)
+    # 6 workers, 2 of them for small tiles: the shortest p99 measured (4
+    # workers: the same capacity, a longer p99 from mid-size tiles waiting
+    # for a worker; 8: longer again)
+    ap.add_argument("--slots", type=int, default=6)
+    ap.add_argument("--small-slots", type=int, default=2)
     ap.add_argument("--small-points", type=float, default=1.5e6)
     ap.add_argument("--batch-points", type=float, default=4e6)
     ap.add_argument("--load", type=float, default=0.7, help="arrival rate / capacity")
