@@ -1258,38 +1258,44 @@ __device__ __attribute__((always_inline)) inline void scan_reg(const TreeView &T
   }
 }
 
-// Size classes of the scan: 0 = 1..64 points, 1 = 65-128, 2 = 129-256,
-// 3 = more.  One thread per tile: an empty tile gets its record here, every
-// other tile goes on its class's list (class 0 from the front of la, class
-// 3 from its back, class 1 from the front of lb, class 2 from its back;
-// ncls[c] the lengths), so each class kernel visits only its own tiles.
-// The order inside a list does not matter: every tile's record depends on
-// that tile alone.
+// Size classes of the scan: 0 = 1..16 points, 1 = 17-32, 2 = 33-64,
+// 3 = 65-128, 4 = 129-256, 5 = more.  One thread per tile: an empty tile gets
+// its record here, every other tile goes on its class's list (classes 0 and
+// 5 from the front and the back of la, 2 and 4 of lb, 1 and 3 of lc; ncls[c]
+// the lengths), so each class kernel visits only its own tiles.  The order
+// inside a list does not matter: every tile's record depends on that tile
+// alone.
+#define SCAN_NCLS 6
+__device__ inline int scan_class(uint32_t m)
+{
+  return m == 0 ? -1 : m <= 16 ? 0 : m <= 32 ? 1 : m <= WAVE ? 2 : m <= 2 * WAVE ? 3 : m <= 4 * WAVE ? 4 : 5;
+}
+
 __global__ void __launch_bounds__(1024) k_scan_classes(uint32_t ntiles, const uint32_t *tile_cnt,
-                                                      uint32_t *la, uint32_t *lb, uint32_t *ncls,
-                                                      TileRec *rec)
+                                                      uint32_t *la, uint32_t *lb, uint32_t *lc,
+                                                      uint32_t *ncls, TileRec *rec)
 {
   // slots are claimed per block: the waves' class counts summed in LDS, one
-  // atomic per class and block (atomics on the four list counters are
+  // atomic per class and block (atomics on the list counters are
   // serialised, so one per wave would cost more than the pass)
-  __shared__ uint32_t wcnt[4][1024 / WAVE];
-  __shared__ uint32_t bbase[4];
+  __shared__ uint32_t wcnt[SCAN_NCLS][1024 / WAVE];
+  __shared__ uint32_t bbase[SCAN_NCLS];
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t m = t < ntiles ? tile_cnt[t] : 0u;
   if (t < ntiles && m == 0) {
     TileRec r = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     rec[t] = r;
   }
-  const int c = m == 0 ? -1 : m <= WAVE ? 0 : m <= 2 * WAVE ? 1 : m <= 4 * WAVE ? 2 : 3;
+  const int c = scan_class(m);
   const uint32_t lane = __lane_id(), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
-  uint64_t mk[4];
+  uint64_t mk[SCAN_NCLS];
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < SCAN_NCLS; k++) {
     mk[k] = __ballot(c == k);
     if (lane == 0) wcnt[k][w] = (uint32_t)__popcll(mk[k]);
   }
   __syncthreads();
-  if (threadIdx.x < 4) {
+  if (threadIdx.x < SCAN_NCLS) {
     const int k = threadIdx.x;
     uint32_t tot = 0;
     for (uint32_t v = 0; v < nw; v++) {
@@ -1302,10 +1308,14 @@ __global__ void __launch_bounds__(1024) k_scan_classes(uint32_t ntiles, const ui
   __syncthreads();
   if (c < 0) return;
   const uint32_t pos = bbase[c] + wcnt[c][w] + (uint32_t)__popcll(mk[c] & ((1ull << lane) - 1));
-  if (c == 0) la[pos] = t;
-  else if (c == 1) lb[pos] = t;
-  else if (c == 2) lb[ntiles - 1 - pos] = t;
-  else la[ntiles - 1 - pos] = t;
+  switch (c) {
+  case 0: la[pos] = t; break;
+  case 1: lc[pos] = t; break;
+  case 2: lb[pos] = t; break;
+  case 3: lc[ntiles - 1 - pos] = t; break;
+  case 4: lb[ntiles - 1 - pos] = t; break;
+  default: la[ntiles - 1 - pos] = t; break;
+  }
 }
 
 // A class list: entry i at list[step * i] (step +1 for a list filled from
@@ -1342,6 +1352,166 @@ __global__ void __launch_bounds__(64, WB_SCAN_MINW) k_scan_reg(TreeView T, TileG
     scan_reg<K, F32>(T, g, tex[t], tey[t], tile_off[t], tile_cnt[t], lists, min_hs, prec, rec + t,
                      ctr);
     t = tnext;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tiles of at most GS = 16 or 32 points, WAVE/GS of them per wave: group
+// q (lanes GS*q .. GS*q+GS-1) scans one tile with the operations of
+// scan_reg<1> in the same order (FP64 fit only).  Every exchange stays
+// inside a group: the canonical sort is the register network on GS lanes,
+// the pairwise sums are the xor butterflies of lanes_pairwise (DPP inside a
+// row, one ds_bpermute across the two rows of a 32-lane group) with each
+// group's blocks read from the lanes its own point count names, and the
+// minima, the first bottom point and the sector counts are taken per group.
+// ---------------------------------------------------------------------------
+// pairwisesum (manysum.cpp:120-154) of the first m values of this lane's
+// group of GS lanes (m may differ between groups; every lane active)
+template <int GS>
+__device__ inline double group_pairwise(double v, uint32_t m, uint32_t gbase)
+{
+  constexpr int LV = GS == 16 ? 5 : 6;
+  double L[LV];
+  L[0] = v;
+  L[1] = L[0] + dpp_d<0xb1>(L[0]);
+  L[2] = L[1] + dpp_d<0x4e>(L[1]);
+  L[3] = L[2] + dpp_d<0x141>(L[2]);
+  L[4] = L[3] + dpp_d<0x140>(L[3]);
+  if constexpr (GS == 32) L[5] = L[4] + __shfl_xor(L[4], 16, WAVE);
+  // the block of bit k of m starts at (m >> (k+1)) << (k+1); block sums are
+  // added to 0 in ascending bit order, as wave_pairwise_reg does
+  double acc = 0;
+#pragma unroll
+  for (int k = 0; k < LV; k++) {
+    const uint32_t off = (m >> (k + 1)) << (k + 1);
+    const double b = __shfl(L[k], (int)(gbase + (off & (GS - 1))), WAVE);
+    if ((m >> k) & 1u) acc = acc + b;
+  }
+  return acc;
+}
+
+// minimum / maximum over this lane's group of GS lanes
+template <int GS>
+__device__ inline double group_dmin(double v)
+{
+  v = fmin(v, dpp_d<0xb1>(v));
+  v = fmin(v, dpp_d<0x4e>(v));
+  v = fmin(v, dpp_d<0x141>(v));
+  v = fmin(v, dpp_d<0x140>(v));
+  if constexpr (GS == 32) v = fmin(v, __shfl_xor(v, 16, WAVE));
+  return v;
+}
+template <int GS>
+__device__ inline double group_dmax(double v)
+{
+  v = fmax(v, dpp_d<0xb1>(v));
+  v = fmax(v, dpp_d<0x4e>(v));
+  v = fmax(v, dpp_d<0x141>(v));
+  v = fmax(v, dpp_d<0x140>(v));
+  if constexpr (GS == 32) v = fmax(v, __shfl_xor(v, 16, WAVE));
+  return v;
+}
+
+template <int GS>
+__device__ __attribute__((always_inline)) inline void scan_group(
+    const TreeView &T, const TileGeo &g, bool live, int ex, int ey, uint32_t off, uint32_t m,
+    const uint32_t *lists, double min_hs, TileRec *out, unsigned long long *ctr)
+{
+  const uint32_t lane = __lane_id(), gl = lane % GS, gbase = lane - gl;
+  const uint64_t gmask = (GS == 64 ? ~0ull : ((1ull << GS) - 1)) << gbase;
+  uint32_t c[1] = {live && gl < m ? lists[off + gl] : 0xffffffffu};
+  {
+    // canonical order inside each group: the network on GS lanes sorts every
+    // aligned group of GS lanes on its own (pads last)
+    uint32_t unused[1];
+    wbw::wave_sort<1, false>(c, unused, GS);
+  }
+  double cx = 0, cy = 0;
+  if (live) tile_center(g, ex, ey, &cx, &cy);
+  const double r = g.r;
+  const bool mine = live && gl < m;
+  double vx = 0, vy = 0, vz = 0;
+  if (mine) {
+    const uint32_t q = T.canon[c[0]];
+    vx = T.px[q] - cx;
+    vy = T.py[q] - cy;
+    vz = T.pz[q] - 0;
+  }
+  const double sxx = group_pairwise<GS>(vx * vx, m, gbase);
+  const double syx = group_pairwise<GS>(vy * vx, m, gbase);
+  const double syy = group_pairwise<GS>(vy * vy, m, gbase);
+  const double s1x = group_pairwise<GS>(1.0 * vx, m, gbase);
+  const double s1y = group_pairwise<GS>(1.0 * vy, m, gbase);
+  const double s11 = group_pairwise<GS>(1.0 * 1.0, m, gbase);
+  const double sxz = group_pairwise<GS>(vx * vz, m, gbase);
+  const double syz = group_pairwise<GS>(vy * vz, m, gbase);
+  const double s1z = group_pairwise<GS>(1.0 * vz, m, gbase);
+  double sx = 0, sy = 0;
+  plane_tilt(sxx, syx, syy, s1x, s1y, s11, sxz, syz, s1z, &sx, &sy);
+  const double u = vz - (sy * vy + sx * vx);
+  const double bottom = group_dmin<GS>(mine ? u : __builtin_inf());
+  const double top = group_dmax<GS>(mine ? u : -__builtin_inf());
+  const uint64_t atb = __ballot(mine && u == bottom) & gmask;
+  const uint32_t first = atb ? (uint32_t)(__ffsll((long long)atb) - 1) - gbase : 0xffffffffu;
+  double bottom2 = group_dmin<GS>(mine && gl < first ? u : __builtin_inf());
+  if (wb::disinf(bottom2)) bottom2 = bottom;
+  // 7-sector histogram of the bottom layer
+  int sec = -1;
+  bool edge = false;
+  if (mine && u < bottom2 + 2 * r) {
+    int e = 0;
+    sec = wb::scan_sector(vx, vy, r, &e);
+    edge = e != 0;
+  }
+  const uint32_t nbottom = (uint32_t)__popcll(__ballot(sec >= 0) & gmask);
+  const uint32_t nedge = (uint32_t)__popcll(__ballot(edge) & gmask);
+  uint32_t hist[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) hist[k] = (uint32_t)__popcll(__ballot(sec == k) & gmask);
+  if (live && gl == 0) {
+    if (nedge) atomicAdd(&ctr[0], (unsigned long long)nedge);
+    double density = 0;
+    int tree = 0;
+    for (int i = 0; i < 7; i++) density += (int)hist[i] * (int)hist[i];
+    if (m > nbottom && density < 7) tree = 1;
+    density = sqrt(density) * kSqrt7 / (r * r) / M_PI;
+    if (m > nbottom && density < 0.5) tree = 1;
+    if (top - bottom > 1.5) tree = 1;
+    TileRec rr;
+    rr.npoints = (int32_t)m;
+    rr.nground = 0;
+    rr.tree_flags = tree;
+    rr.fit_fp64 = 0;
+    rr.density = density;
+    rr.hs_scan = sqrt(1 / density + min_hs * min_hs);
+    rr.hs = rr.hs_scan;
+    rr.height = top - bottom;
+    rr.tilt_x = sx;
+    rr.tilt_y = sy;
+    *out = rr;
+  }
+}
+
+// WAVE/GS tiles of the class list per wave, grid-stride over the list (the
+// FP64 fit; the FP32 options take these tiles through k_scan_reg<1>)
+template <int GS>
+__global__ void __launch_bounds__(64, WB_SCAN_MINW) k_scan_row(TreeView T, TileGeo g,
+                                                             const int32_t *tex, const int32_t *tey,
+                                                             const uint32_t *tile_off,
+                                                             const uint32_t *tile_cnt,
+                                                             const uint32_t *lists, ClassList cl,
+                                                             double min_hs, TileRec *rec,
+                                                             unsigned long long *ctr)
+{
+  constexpr uint32_t NG = WAVE / GS;
+  const uint32_t n = *cl.n;
+  const uint32_t grp = __lane_id() / GS;
+  for (uint32_t i0 = blockIdx.x * NG; i0 < n; i0 += gridDim.x * NG) {
+    const uint32_t i = i0 + grp;
+    const bool live = i < n;
+    const uint32_t t = live ? cl.at(i) : 0u;
+    scan_group<GS>(T, g, live, live ? tex[t] : 0, live ? tey[t] : 0, live ? tile_off[t] : 0u,
+                   live ? tile_cnt[t] : 0u, lists, min_hs, rec + t, ctr);
   }
 }
 
@@ -2798,7 +2968,7 @@ static void classify_prepare(ClassifyRun &R, wb_tree *t, wb_ctx *c, const double
   // err[0] error flags; then u64 counters: 0 sector borderline, 1 exact atan2i
   // steps, 2-4 class histogram, 5 non-empty tiles, 6 (two u32) exact gap
   // points and heavy points, 7 FP64 fit fallbacks; u32 18-19 work counters of
-  // the heavy and resident ground kernels, 20-23 the scan's class list lengths
+  // the heavy and resident ground kernels, 20-25 the scan's class list lengths
   uint32_t *err = R.err = (uint32_t *)wb_alloc(c, 128);
   unsigned long long *ctr = R.ctr = (unsigned long long *)err + 1;
   uint32_t *nslow = R.nslow = (uint32_t *)(ctr + 6);
@@ -2835,31 +3005,42 @@ static void classify_prepare(ClassifyRun &R, wb_tree *t, wb_ctx *c, const double
     // grid-stride over its list
     uint32_t *la = tile_fill;  // free until the ground counts
     uint32_t *lb = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
+    uint32_t *lc = (uint32_t *)wb_alloc(c, 4ull * g.ntiles + 4);
     uint32_t *ncls = err + 20;
     const unsigned nb = (unsigned)std::min<uint64_t>(g.ntiles, 1u << 16);
     const int prec = p->fit_precision;
     if (nb) {
       hipLaunchKernelGGL(k_scan_classes, dim3(wb_blocks(g.ntiles, 1024)), dim3(1024), 0, st,
-                         g.ntiles, tile_cnt, la, lb, ncls, rec);
-      const ClassList c1 = {la, 1, ncls}, c2 = {lb, 1, ncls + 1};
-      const ClassList c4 = {lb + g.ntiles - 1, -1, ncls + 2}, cbig = {la + g.ntiles - 1, -1, ncls + 3};
-#define WB_SCAN_CLASSES(F32)                                                                      \
-  hipLaunchKernelGGL((k_scan_reg<1, F32>), dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,   \
-                     tile_cnt, lists, c1, p->min_hs, prec, rec, ctr);                             \
-  hipLaunchKernelGGL((k_scan_reg<2, F32>), dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,   \
-                     tile_cnt, lists, c2, p->min_hs, prec, rec, ctr);                             \
-  hipLaunchKernelGGL((k_scan_reg<4, F32>), dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,   \
-                     tile_cnt, lists, c4, p->min_hs, prec, rec, ctr)
+                         g.ntiles, tile_cnt, la, lb, lc, ncls, rec);
+      const uint32_t e = g.ntiles - 1;
+      const ClassList c16 = {la, 1, ncls}, c32 = {lc, 1, ncls + 1}, c64 = {lb, 1, ncls + 2};
+      const ClassList c128 = {lc + e, -1, ncls + 3}, c256 = {lb + e, -1, ncls + 4};
+      const ClassList cbig = {la + e, -1, ncls + 5};
+#define WB_SCAN_REG(K, F32, CL)                                                                   \
+  hipLaunchKernelGGL((k_scan_reg<K, F32>), dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,   \
+                     tile_cnt, lists, CL, p->min_hs, prec, rec, ctr)
       if (prec == WB_FIT_FP64) {
-        WB_SCAN_CLASSES(false);
+        // tiles of up to 16 and 32 points four and two to a wave
+        hipLaunchKernelGGL(k_scan_row<16>, dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,
+                           tile_cnt, lists, c16, p->min_hs, rec, ctr);
+        hipLaunchKernelGGL(k_scan_row<32>, dim3(nb), dim3(64), 0, st, T, g, tex, tey, tile_off,
+                           tile_cnt, lists, c32, p->min_hs, rec, ctr);
+        WB_SCAN_REG(1, false, c64);
+        WB_SCAN_REG(2, false, c128);
+        WB_SCAN_REG(4, false, c256);
       } else {
-        WB_SCAN_CLASSES(true);
+        WB_SCAN_REG(1, true, c16);
+        WB_SCAN_REG(1, true, c32);
+        WB_SCAN_REG(1, true, c64);
+        WB_SCAN_REG(2, true, c128);
+        WB_SCAN_REG(4, true, c256);
       }
-#undef WB_SCAN_CLASSES
+#undef WB_SCAN_REG
       hipLaunchKernelGGL(k_scan, dim3(std::min(nb, 4096u)), dim3(64), 0, st, T, g, tex, tey,
                          tile_off, tile_cnt, lists, cbig, p->min_hs, prec, rec, err, ctr);
     }
     wb_free(c, lb, 4ull * g.ntiles + 4);
+    wb_free(c, lc, 4ull * g.ntiles + 4);
   }
   hipLaunchKernelGGL(k_postscan, dim3(wb_blocks(g.ntiles, 256)), dim3(256), 0, st, g, tex, tey, rec,
                      ctr + 5);
