@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(64) k_points_in(TreeView T, int kind, const do
 // lanes for which the node and every ancestor intersect the lane's shape, so
 // each lane sees exactly the leaves findBlocks returns for its own shape, in
 // the same (DFS) order.  A leaf whose eight corners are all in a lane's shape
-// counts whole for that lane (octree.cpp:1224-1226, 1252-1254); otherwise the
+// counts whole for that lane (octree.cpp:1226-1231, 1256-1257); otherwise the
 // lane tests the leaf's points, skipping 32-point groups whose bounding box
 // no point of the shape can reach (wb::shape_may_hit_box: exact).  A group's
 // points are read once per wave with one vector load and handed to every
