@@ -597,44 +597,58 @@ __global__ void k_tile_grid(TileGeo g, const int32_t *tex, const int32_t *tey, i
 
 #define MAX_MEMB 4
 
+// the lanes with `has` whose key equals this lane's (0 without `has`)
+__device__ inline uint64_t wave_peers(bool has, int key)
+{
+  uint64_t todo = __ballot(has), peers = 0;
+  while (todo) {
+    const int kl = __builtin_amdgcn_readlane(key, __ffsll((long long)todo) - 1);
+    const bool same = has && key == kl;
+    const uint64_t m = __ballot(same);
+    if (same) peers = m;
+    todo &= ~m;
+  }
+  return peers;
+}
+
 // cnt[key] += 1 for every active lane with `has`: the peer masks first, then
 // one atomic per distinct key from its first lane, none of them waited for
 // (no slot is returned)
 __device__ inline void wave_count_inc(uint32_t *cnt, bool has, int key)
 {
-  const uint32_t lane = __lane_id();
-  uint64_t todo = __ballot(has), peers = 0;
-  while (todo) {
-    const int kl = __builtin_amdgcn_readlane(key, __ffsll((long long)todo) - 1);
-    const bool same = has && key == kl;
-    const uint64_t m = __ballot(same);
-    if (same) peers = m;
-    todo &= ~m;
-  }
-  if (has && __ffsll((long long)peers) - 1 == (int)lane)
+  const uint64_t peers = wave_peers(has, key);
+  if (has && __ffsll((long long)peers) - 1 == (int)__lane_id())
     atomicAdd(&cnt[key], (uint32_t)__popcll(peers));
 }
 
-// The same claim with the atomics of all distinct keys issued together:
-// the peer masks first, then one atomic per key from its first lane, and
-// each lane's base read from that lane.  Lanes with the same key get
-// consecutive slots in lane order.
-__device__ inline uint32_t wave_claim(uint32_t *cnt, bool has, int key)
+// Slot claims of a wave, K keys per lane: for each k, one atomic on
+// cnt[key[k]] per distinct key from its first lane, each lane's base read
+// from that lane, lanes with the same key given consecutive slots in lane
+// order (slot[k], where has[k]).  The atomics of all K keys are issued
+// before any is waited for: one round trip per wave.
+template <int K>
+__device__ inline void wave_claim_n(uint32_t *cnt, const bool (&has)[K], const int (&key)[K],
+                                    uint32_t (&slot)[K])
 {
   const uint32_t lane = __lane_id();
-  uint64_t todo = __ballot(has), peers = 0;
-  while (todo) {
-    const int kl = __builtin_amdgcn_readlane(key, __ffsll((long long)todo) - 1);
-    const bool same = has && key == kl;
-    const uint64_t m = __ballot(same);
-    if (same) peers = m;
-    todo &= ~m;
+  uint64_t peers[K];
+  int first[K];
+  uint32_t base[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    peers[k] = wave_peers(has[k], key[k]);
+    first[k] = has[k] ? __ffsll((long long)peers[k]) - 1 : (int)lane;
   }
-  const int first = has ? __ffsll((long long)peers) - 1 : (int)lane;
-  uint32_t base = 0;
-  if (has && first == (int)lane) base = atomicAdd(&cnt[key], (uint32_t)__popcll(peers));
-  base = (uint32_t)__shfl((int)base, first, 64);
-  return base + (uint32_t)__popcll(peers & ((1ull << lane) - 1));
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    base[k] = 0;
+    if (has[k] && first[k] == (int)lane)
+      base[k] = atomicAdd(&cnt[key[k]], (uint32_t)__popcll(peers[k]));
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++)
+    slot[k] = (uint32_t)__shfl((int)base[k], first[k], 64) +
+              (uint32_t)__popcll(peers[k] & ((1ull << lane) - 1));
 }
 
 // The (leaf, tile) part of the membership test, once per leaf for the
@@ -773,13 +787,20 @@ __global__ void k_tile_lists(uint64_t n, const int32_t *memb, const uint32_t *nm
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t nm = nmemb[i];
+  bool has[MAX_MEMB];
+  int t[MAX_MEMB];
+  uint32_t slot[MAX_MEMB];
 #pragma unroll
   for (uint32_t k = 0; k < MAX_MEMB; k++) {
-    const bool has = k < nm;
-    const int t = has ? memb[MAX_MEMB * i + k] - lo : 0;
-    const uint32_t slot = wave_claim(tile_fill, has, t);
-    if (has) lists[tile_off[t] + slot] = crank[i];
+    has[k] = k < nm;
+    t[k] = has[k] ? memb[MAX_MEMB * i + k] - lo : 0;
   }
+  // the slot claims of all memberships in one round of atomics
+  wave_claim_n<MAX_MEMB>(tile_fill, has, t, slot);
+  const uint32_t cr = crank[i];
+#pragma unroll
+  for (uint32_t k = 0; k < MAX_MEMB; k++)
+    if (has[k]) lists[tile_off[t[k]] + slot[k]] = cr;
 }
 
 // ---------------------------------------------------------------------------
