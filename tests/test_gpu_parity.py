@@ -178,6 +178,9 @@ def test_batched_queries_large_batch(ctx, kind):
     P = np.concatenate([P, far, zero, big])
     counts = tree.count_points_in((kind, P))
     assert np.array_equal(counts, _noalloc_counts(tree, kind, P))
+    # batches of up to 64 shapes take a wave per shape, larger ones a lane
+    for k in (1, 64, 65):
+        assert np.array_equal(tree.count_points_in((kind, P[:k])), counts[:k])
     assert counts[-1] == len(x)
     assert (counts[-7:-4] == 0).all()
     for s in list(rng.integers(0, len(P), 150)) + list(range(len(P) - 7, len(P))):

@@ -473,6 +473,11 @@ int wb_tree_order(wb_tree *t, uint32_t *order)
 
 static int check_kind(int kind) { return kind == WB_SHAPE_CYLINDER || kind == WB_SHAPE_SPHERE || kind == WB_SHAPE_HYPERBOLOID; }
 
+// Batches of up to this many shapes take a wave per shape in the caller's
+// order: one wave would hold them all in the lane-per-shape kernel, and
+// ordering them first would cost more launches than the query.
+#define WB_QUERY_SORT_MIN 65
+
 // Events around a query's kernels on the context stream; ms() after the
 // stream has passed the second one.
 struct QueryTimer {
@@ -504,14 +509,14 @@ int wb_count_points_in(wb_tree *t, int kind, const double *params, uint64_t ns, 
   WB_CHECK(hipMemcpyAsync(dp, params, pb, hipMemcpyHostToDevice, c->stream));
   QueryTimer qt(c);
   qt.start();
-  uint32_t *order = wb_query_order(t, kind, dp, ns);
+  uint32_t *order = ns >= WB_QUERY_SORT_MIN ? wb_query_order(t, kind, dp, ns) : nullptr;
   wb_run_count(t, kind, dp, ns, dc, order);
   qt.stop();
   WB_CHECK(hipMemcpyAsync(counts, dc, 8 * ns, hipMemcpyDeviceToHost, c->stream));
   WB_CHECK(hipStreamSynchronize(c->stream));
   t->ms_query_count = qt.ms();
   t->ms_query_points = 0;
-  wb_free(c, order, 4 * ns);
+  if (order) wb_free(c, order, 4 * ns);
   wb_free(c, dp, pb);
   wb_free(c, dc, 8 * ns);
   return WB_OK;
@@ -562,7 +567,7 @@ int wb_points_in(wb_tree *t, int kind, const double *params, uint64_t ns, uint64
   WB_CHECK(hipMemcpyAsync(dp, params, pb, hipMemcpyHostToDevice, c->stream));
   QueryTimer qt(c);
   qt.start();
-  uint32_t *order = wb_query_order(t, kind, dp, ns);
+  uint32_t *order = ns >= WB_QUERY_SORT_MIN ? wb_query_order(t, kind, dp, ns) : nullptr;
   wb_run_count(t, kind, dp, ns, dc, order);
   qt.stop();
   std::vector<uint64_t> cnt(ns);
@@ -574,7 +579,7 @@ int wb_points_in(wb_tree *t, int kind, const double *params, uint64_t ns, uint64
   if (idx) {
     uint64_t total = offsets[ns];
     if (cap < total) {
-      wb_free(c, order, 4 * ns);
+      if (order) wb_free(c, order, 4 * ns);
       wb_free(c, dp, pb);
       wb_free(c, dc, 8 * (ns + 1));
       return fail(c, "index buffer too small", WB_EINVAL);
@@ -589,7 +594,7 @@ int wb_points_in(wb_tree *t, int kind, const double *params, uint64_t ns, uint64
     t->ms_query_points = qt.ms();
     wb_free(c, dout, 4 * total + 4);
   }
-  wb_free(c, order, 4 * ns);
+  if (order) wb_free(c, order, 4 * ns);
   wb_free(c, dp, pb);
   wb_free(c, dc, 8 * (ns + 1));
   return WB_OK;
